@@ -1,0 +1,347 @@
+// C ABI of include/eu3d.h: context lifetime, field exchange and the entry points, each a thin
+// wrapper that validates its arguments and enqueues the kernels of eu3d_kernels.hip.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "eu3d_internal.hpp"
+
+using namespace eu3d;
+
+namespace {
+
+int ncomp(int field) {
+    switch (field) {
+    case EU3D_FIELD_CS: case EU3D_FIELD_PT: case EU3D_FIELD_RHS:
+    case EU3D_FIELD_F: case EU3D_FIELD_G: case EU3D_FIELD_Q: return NC;
+    case EU3D_FIELD_YI: case EU3D_FIELD_DI: case EU3D_FIELD_MC:
+    case EU3D_FIELD_CMS: case EU3D_FIELD_MD: return NS;
+    default: return (field >= 0 && field < EU3D_FIELD_COUNT) ? 1 : 0;
+    }
+}
+
+int fail(eu3d_ctx* c, int code, const char* msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+int hip_fail(eu3d_ctx* c, hipError_t e, const char* where) {
+    if (c) {
+        c->err = std::string(where) + ": " + hipGetErrorString(e);
+    }
+    return EU3D_ERR_HIP;
+}
+
+#define HIPCHK(c, call, where)                          \
+    do {                                                \
+        hipError_t e_ = (call);                         \
+        if (e_ != hipSuccess) return hip_fail(c, e_, where); \
+    } while (0)
+
+bool same(double a, double b) { return std::memcmp(&a, &b, sizeof(double)) == 0; }
+
+// The kernels are compiled for one mechanism; accept only that one, bit for bit.
+bool mech_matches(const eu3d_mech* m, std::string& why) {
+    if (!m) { why = "mechanism is null"; return false; }
+    if (m->ns != NS || m->nr != NR) { why = "species/reaction count differs from the compiled mechanism"; return false; }
+    for (int s = 0; s < NS; s++) {
+        if (!same(m->mw[s], MW[s])) { why = "Mw differs"; return false; }
+        for (int r = 0; r < NR; r++) {
+            if (!same(m->stoi_f[s * NR + r], (double)SF[s][r]) || !same(m->stoi_b[s * NR + r], (double)SB[s][r])) {
+                why = "stoichiometry differs"; return false;
+            }
+            if (!same(m->react_tb[s * NR + r], TB[s][r])) { why = "third-body efficiencies differ"; return false; }
+        }
+        for (int k = 0; k < 9; k++)
+            if (!same(m->coeff0[k * NS + s], C0[k][s]) || !same(m->coeff1[k * NS + s], C1[k][s])) {
+                why = "NASA coefficients differ"; return false;
+            }
+    }
+    for (int r = 0; r < NR; r++)
+        if (!same(m->af[r], AF[r]) || !same(m->bf[r], BF[r]) || !same(m->eaf[r], EAF[r])) {
+            why = "Arrhenius parameters differ"; return false;
+        }
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int eu3d_create(eu3d_ctx** out, const eu3d_grid* grid, const eu3d_mech* mech, uint32_t quirks) {
+    if (!out || !grid) return EU3D_ERR_ARG;
+    *out = nullptr;
+    if (grid->bc != 2 || grid->ni < 1 || grid->nj < 1 || grid->nk < 1 || !grid->xnode || !grid->ynode || !grid->znode)
+        return EU3D_ERR_ARG;
+    std::string why;
+    if (!mech_matches(mech, why)) {
+        fprintf(stderr, "eu3d_create: %s\n", why.c_str());
+        return EU3D_ERR_MECH_MISMATCH;
+    }
+    eu3d_ctx* c = new eu3d_ctx();
+    c->quirks = quirks;
+    G3& g = c->g;
+    g.ni = grid->ni; g.nj = grid->nj; g.nk = grid->nk; g.bc = grid->bc;
+    g.NX = g.ni + 2 * g.bc; g.NY = g.nj + 2 * g.bc; g.NZ = g.nk + 2 * g.bc;
+    g.n3 = (size_t)g.NX * g.NY * g.NZ;
+    if (g.n3 >= (1ull << 32)) { delete c; return EU3D_ERR_ARG; }
+    c->dx0 = grid->xnode[1] - grid->xnode[0];
+    c->dy0 = grid->ynode[1] - grid->ynode[0];
+    c->dz0 = grid->znode[1] - grid->znode[0];
+    hipError_t e;
+    if ((e = hipGetDevice(&c->device)) != hipSuccess) { delete c; return EU3D_ERR_HIP; }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, c->device) == hipSuccess && prop.multiProcessorCount > 0)
+        c->num_cu = prop.multiProcessorCount;
+    if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) { delete c; return EU3D_ERR_HIP; }
+    for (int f = 0; f < EU3D_FIELD_COUNT; f++) {
+        const size_t bytes = g.n3 * ncomp(f) * sizeof(double);
+        if ((e = hipMalloc(&c->fl.f[f], bytes)) != hipSuccess) { eu3d_destroy(c); return EU3D_ERR_HIP; }
+        if ((e = hipMemsetAsync(c->fl.f[f], 0, bytes, c->stream)) != hipSuccess) { eu3d_destroy(c); return EU3D_ERR_HIP; }
+    }
+    const size_t nx = g.NX, ny = g.NY, nz = g.NZ;
+    if (hipMalloc(&c->xnode, nx * sizeof(double)) != hipSuccess || hipMalloc(&c->ynode, ny * sizeof(double)) != hipSuccess ||
+        hipMalloc(&c->znode, nz * sizeof(double)) != hipSuccess || hipMalloc(&c->sc, sizeof(Scalars)) != hipSuccess ||
+        hipMalloc(&c->active, g.n3 * sizeof(unsigned)) != hipSuccess ||
+        hipMalloc(&c->rk_save, g.n3 * NC * sizeof(double)) != hipSuccess) {
+        eu3d_destroy(c);
+        return EU3D_ERR_HIP;
+    }
+    (void)hipMemcpyAsync(c->xnode, grid->xnode, nx * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    (void)hipMemcpyAsync(c->ynode, grid->ynode, ny * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    (void)hipMemcpyAsync(c->znode, grid->znode, nz * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    (void)hipMemsetAsync(c->sc, 0, sizeof(Scalars), c->stream);
+    (void)launch_set_scalars(c, 0.0, 0.0);
+    if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) { eu3d_destroy(c); return EU3D_ERR_HIP; }
+    *out = c;
+    return EU3D_OK;
+}
+
+int eu3d_destroy(eu3d_ctx* c) {
+    if (!c) return EU3D_ERR_ARG;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (int f = 0; f < EU3D_FIELD_COUNT; f++)
+        if (c->fl.f[f]) (void)hipFree(c->fl.f[f]);
+    if (c->xnode) (void)hipFree(c->xnode);
+    if (c->ynode) (void)hipFree(c->ynode);
+    if (c->znode) (void)hipFree(c->znode);
+    if (c->sc) (void)hipFree(c->sc);
+    if (c->active) (void)hipFree(c->active);
+    if (c->rk_save) (void)hipFree(c->rk_save);
+    if (c->batch_active) (void)hipFree(c->batch_active);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return EU3D_OK;
+}
+
+const char* eu3d_last_error(const eu3d_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int eu3d_sync(eu3d_ctx* c) {
+    if (!c) return EU3D_ERR_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream), "eu3d_sync");
+    return EU3D_OK;
+}
+
+int eu3d_set_quirks(eu3d_ctx* c, uint32_t q) {
+    if (!c) return EU3D_ERR_ARG;
+    c->quirks = q;
+    return EU3D_OK;
+}
+
+int eu3d_field_components(const eu3d_ctx* c, int field) {
+    (void)c;
+    return ncomp(field);
+}
+
+int eu3d_set_field(eu3d_ctx* c, int field, const double* host) {
+    if (!c || !host || ncomp(field) == 0) return fail(c, EU3D_ERR_ARG, "eu3d_set_field: bad field or pointer");
+    const int nc = ncomp(field);
+    const size_t bytes = c->g.n3 * nc * sizeof(double);
+    // stage through the RHS-sized save buffer (n3 * NC doubles, the largest field)
+    HIPCHK(c, hipMemcpyAsync(c->rk_save, host, bytes, hipMemcpyHostToDevice, c->stream), "set_field copy");
+    HIPCHK(c, launch_aos_to_soa(c, c->rk_save, c->fl.f[field], nc), "set_field transpose");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "set_field sync");
+    return EU3D_OK;
+}
+
+int eu3d_get_field(eu3d_ctx* c, int field, double* host) {
+    if (!c || !host || ncomp(field) == 0) return fail(c, EU3D_ERR_ARG, "eu3d_get_field: bad field or pointer");
+    const int nc = ncomp(field);
+    const size_t bytes = c->g.n3 * nc * sizeof(double);
+    HIPCHK(c, launch_soa_to_aos(c, c->fl.f[field], c->rk_save, nc), "get_field transpose");
+    HIPCHK(c, hipMemcpyAsync(host, c->rk_save, bytes, hipMemcpyDeviceToHost, c->stream), "get_field copy");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "get_field sync");
+    return EU3D_OK;
+}
+
+int eu3d_set_time(eu3d_ctx* c, double dt, double time) {
+    if (!c) return EU3D_ERR_ARG;
+    HIPCHK(c, launch_set_scalars(c, dt, time), "set_time");
+    return EU3D_OK;
+}
+
+int eu3d_get_time(eu3d_ctx* c, double* dt, double* time) {
+    if (!c) return EU3D_ERR_ARG;
+    Scalars s;
+    HIPCHK(c, hipMemcpyAsync(&s, c->sc, sizeof(Scalars), hipMemcpyDeviceToHost, c->stream), "get_time");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "get_time sync");
+    if (dt) *dt = s.dt;
+    if (time) *time = s.time;
+    return EU3D_OK;
+}
+
+int eu3d_cfl_dt(eu3d_ctx* c, double cfl, double final_time) {
+    if (!c || !(cfl > 0)) return fail(c, EU3D_ERR_ARG, "eu3d_cfl_dt: cfl must be > 0");
+    HIPCHK(c, launch_cfl(c, cfl, final_time), "cfl");
+    return EU3D_OK;
+}
+
+int eu3d_fill_ghosts(eu3d_ctx* c, const int kinds[6]) {
+    if (!c || !kinds) return EU3D_ERR_ARG;
+    for (int f = 0; f < 6; f++)
+        if (kinds[f] < 0 || kinds[f] > EU3D_BC_NONE) return fail(c, EU3D_ERR_ARG, "bad boundary kind");
+    HIPCHK(c, launch_fill_ghosts(c, kinds), "fill_ghosts");
+    return EU3D_OK;
+}
+
+int eu3d_advection(eu3d_ctx* c, int diff, int limiter) {
+    if (!c || diff < 0 || diff > 1 || limiter < 0 || limiter > 4) return fail(c, EU3D_ERR_ARG, "bad scheme");
+    HIPCHK(c, launch_advection(c, diff, limiter), "advection");
+    return EU3D_OK;
+}
+
+int eu3d_update_after_adv(eu3d_ctx* c) {
+    if (!c) return EU3D_ERR_ARG;
+    HIPCHK(c, launch_update_after_adv(c), "update_after_adv");
+    return EU3D_OK;
+}
+
+int eu3d_update_after_cs(eu3d_ctx* c) {
+    if (!c) return EU3D_ERR_ARG;
+    HIPCHK(c, launch_update_after_cs(c), "update_after_cs");
+    return EU3D_OK;
+}
+
+int eu3d_field_initial(eu3d_ctx* c) {
+    if (!c) return EU3D_ERR_ARG;
+    HIPCHK(c, launch_field_initial(c), "field_initial");
+    return EU3D_OK;
+}
+
+int eu3d_get_partial_t(eu3d_ctx* c) {
+    if (!c) return EU3D_ERR_ARG;
+    HIPCHK(c, launch_partial_t(c), "partial_t");
+    return EU3D_OK;
+}
+
+int eu3d_explicit(eu3d_ctx* c) {
+    if (!c) return EU3D_ERR_ARG;
+    HIPCHK(c, launch_explicit(c), "explicit");
+    return EU3D_OK;
+}
+
+int eu3d_diagonalized(eu3d_ctx* c) {
+    if (!c) return EU3D_ERR_ARG;
+    HIPCHK(c, launch_diagonalized(c), "diagonalized");
+    return EU3D_OK;
+}
+
+int eu3d_update_imex(eu3d_ctx* c) {
+    if (!c) return EU3D_ERR_ARG;
+    HIPCHK(c, launch_update_imex(c), "update_imex");
+    return EU3D_OK;
+}
+
+int eu3d_trapezoid(eu3d_ctx* c) {
+    if (!c) return EU3D_ERR_ARG;
+    HIPCHK(c, launch_trapezoid(c), "trapezoid");
+    return EU3D_OK;
+}
+
+static int chem_status(eu3d_ctx* c) {
+    Scalars s;
+    HIPCHK(c, hipMemcpyAsync(&s, c->sc, sizeof(Scalars), hipMemcpyDeviceToHost, c->stream), "chem stats");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "chem sync");
+    c->stats.active_cells = (long long)s.chem_active;
+    c->stats.substeps_total = (long long)s.chem_substeps;
+    c->stats.nchem_max_seen = (int)s.chem_nmax;
+    if (s.nonfinite & 2) return fail(c, EU3D_ERR_NCHEM, "Nchem <= 0 (src/Reaction.cpp:664-668)");
+    return EU3D_OK;
+}
+
+int eu3d_trapezoid_prediction(eu3d_ctx* c, int nchem_max) {
+    if (!c) return EU3D_ERR_ARG;
+    HIPCHK(c, launch_trap_adaptive(c, nchem_max, true), "prediction");
+    c->stats.cells = (long long)c->g.ni * c->g.nj * c->g.nk;
+    return chem_status(c);
+}
+
+int eu3d_trapezoid_adaptive(eu3d_ctx* c, int nchem_max) {
+    if (!c) return EU3D_ERR_ARG;
+    HIPCHK(c, launch_trap_adaptive(c, nchem_max, false), "trapezoid_adaptive");
+    c->stats.cells = (long long)c->g.ni * c->g.nj * c->g.nk;
+    return chem_status(c);
+}
+
+int eu3d_check_finite(eu3d_ctx* c) {
+    if (!c) return EU3D_ERR_ARG;
+    HIPCHK(c, launch_check_finite(c), "check_finite");
+    Scalars s;
+    HIPCHK(c, hipMemcpyAsync(&s, c->sc, sizeof(Scalars), hipMemcpyDeviceToHost, c->stream), "check_finite copy");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "check_finite sync");
+    if (s.nonfinite & 1) return fail(c, EU3D_ERR_NONFINITE, "D or T is not finite (src/Euler.cpp:319-331)");
+    return EU3D_OK;
+}
+
+int eu3d_step(eu3d_ctx* c, const eu3d_step_params* p, int nsteps) {
+    if (!c || !p || nsteps < 0) return EU3D_ERR_ARG;
+    if (p->rk != 1 && p->rk != 2) return fail(c, EU3D_ERR_ARG, "rk must be 1 or 2");
+    if (p->rk == 2 && p->chem == EU3D_CHEM_IMEX) return fail(c, EU3D_ERR_UNSUPPORTED, "IMEX couples to the EE stage only");
+    if (p->diff < 0 || p->diff > 1 || p->limiter < 0 || p->limiter > 4) return fail(c, EU3D_ERR_ARG, "bad scheme");
+    for (int n = 0; n < nsteps; n++) HIPCHK(c, launch_step(c, p), "step");
+    return EU3D_OK;
+}
+
+int eu3d_get_chem_stats(eu3d_ctx* c, eu3d_chem_stats* out) {
+    if (!c || !out) return EU3D_ERR_ARG;
+    *out = c->stats;
+    return EU3D_OK;
+}
+
+int eu3d_chem_batch(eu3d_ctx* c, long long n, double* rho_s, double* T, double* E, double dtm, int nchem_max,
+                    int* nchem_out, eu3d_chem_stats* stats) {
+    if (!c || n <= 0 || !rho_s || !T || !E || !(dtm > 0)) return fail(c, EU3D_ERR_ARG, "eu3d_chem_batch: bad arguments");
+    if (n >= (1ll << 32)) return fail(c, EU3D_ERR_ARG, "eu3d_chem_batch: n too large");
+    HIPCHK(c, launch_chem_batch(c, n, rho_s, T, E, dtm, nchem_max, nchem_out), "chem_batch");
+    c->stats.cells = n;
+    int rc = chem_status(c);
+    if (stats) *stats = c->stats;
+    return rc;
+}
+
+int eu3d_chem_set_schedule(eu3d_ctx* c, const eu3d_chem_schedule* s) {
+    if (!c || !s) return EU3D_ERR_ARG;
+    if (s->block != 64 && s->block != 128 && s->block != 256) return fail(c, EU3D_ERR_ARG, "block must be 64/128/256");
+    if (s->blocks_per_cu < 1 || s->blocks_per_cu > 64) return fail(c, EU3D_ERR_ARG, "blocks_per_cu in [1,64]");
+    c->sched = *s;
+    if (c->sched.chunk < 1) c->sched.chunk = 1;
+    return EU3D_OK;
+}
+
+int eu3d_chem_get_schedule(eu3d_ctx* c, eu3d_chem_schedule* s) {
+    if (!c || !s) return EU3D_ERR_ARG;
+    *s = c->sched;
+    return EU3D_OK;
+}
+
+double* eu3d_device_field(eu3d_ctx* c, int field) {
+    if (!c || field < 0 || field >= EU3D_FIELD_COUNT) return nullptr;
+    return c->fl.f[field];
+}
+
+size_t eu3d_cells_with_ghosts(const eu3d_ctx* c) { return c ? c->g.n3 : 0; }
+
+void* eu3d_stream(eu3d_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+}  // extern "C"

@@ -1,0 +1,425 @@
+// Per-cell FP64 device math for the EU3D hot path on gfx950: NASA-9 thermo, the T-from-energy
+// Newton, MUSCL reconstruction, the AUSM face flux and the H2-air kinetics (rates, trapezoid,
+// diagonal Jacobian). All functions are inline and operate on registers; the kernels in
+// eu3d_kernels.hip compose them.
+//
+// Arithmetic is restructured for the GPU, not transcribed: powers of T are shared across
+// species and terms, the Newton works on per-cell mixture coefficients (one 8-term polynomial
+// per iteration instead of 9 species x 2 fits), and the kinetics loops are unrolled over the
+// compile-time mechanism so zero stoichiometry vanishes. The results agree with the reference
+// arithmetic to rounding (a few ulp), far inside the 1e-10 parity bound.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mech_gen.h"
+
+namespace eu3d {
+using namespace mech;
+
+constexpr double R_UNIV = 8.31434;   // include/Flowfield.hpp:54
+constexpr double RU_CAL = 1.987;     // include/Reaction.hpp:58
+constexpr double P0_PA = 101325.0;   // include/Reaction.hpp:59
+constexpr int NC = NS + 4;           // conserved components
+
+enum : uint32_t {
+    Q_F1 = 1u << 0, Q_F1B = 1u << 1, Q_F2 = 1u << 2, Q_F3 = 1u << 3, Q_F6 = 1u << 4,
+    Q_C1 = 1u << 5, Q_C2 = 1u << 6, Q_C3 = 1u << 7, Q_C4 = 1u << 8, Q_C6 = 1u << 9, Q_C7 = 1u << 10
+};
+
+__host__ __device__ constexpr double RI(int s) { return R_UNIV / MW[s]; }   // src/Reaction.cpp:353-354
+
+// ------------------------------------------------------------------ NASA-9 thermo
+struct TPow {
+    double T, Tm1, Tm2, T2, T3, T4, T5, lnT;
+};
+__device__ __forceinline__ TPow tpow(double T) {
+    TPow p;
+    p.T = T;
+    p.Tm1 = 1.0 / T;
+    p.Tm2 = p.Tm1 * p.Tm1;
+    p.T2 = T * T;
+    p.T3 = p.T2 * T;
+    p.T4 = p.T2 * p.T2;
+    p.T5 = p.T4 * T;
+    p.lnT = log(T);
+    return p;
+}
+
+// c_p/R of species s (Reaction::GetCpi, src/Reaction.cpp:893-904); range by T < 1000.
+__device__ __forceinline__ double cp_R(int s, bool lo, const TPow& p) {
+    const double a0 = lo ? C0[0][s] : C1[0][s], a1 = lo ? C0[1][s] : C1[1][s];
+    const double a2 = lo ? C0[2][s] : C1[2][s], a3 = lo ? C0[3][s] : C1[3][s];
+    const double a4 = lo ? C0[4][s] : C1[4][s], a5 = lo ? C0[5][s] : C1[5][s];
+    const double a6 = lo ? C0[6][s] : C1[6][s];
+    return a0 * p.Tm2 + a1 * p.Tm1 + a2 + a3 * p.T + a4 * p.T2 + a5 * p.T3 + a6 * p.T4;
+}
+// h/R (Reaction::GetHi, src/Reaction.cpp:916-926); quirk C4 keeps the low-range b1.
+__device__ __forceinline__ double h_R(int s, bool lo, bool c4, const TPow& p) {
+    const double a0 = lo ? C0[0][s] : C1[0][s], a1 = lo ? C0[1][s] : C1[1][s];
+    const double a2 = lo ? C0[2][s] : C1[2][s], a3 = lo ? C0[3][s] : C1[3][s];
+    const double a4 = lo ? C0[4][s] : C1[4][s], a5 = lo ? C0[5][s] : C1[5][s];
+    const double a6 = lo ? C0[6][s] : C1[6][s];
+    const double b1 = (lo || c4) ? C0[7][s] : C1[7][s];
+    return -a0 * p.Tm1 + a1 * p.lnT + a2 * p.T + a3 * p.T2 / 2 + a4 * p.T3 / 3 + a5 * p.T4 / 4 +
+           a6 * p.T5 / 5 + b1;
+}
+// s/R (Reaction::GetSi, src/Reaction.cpp:938-948); quirk C4 keeps the low-range b2.
+__device__ __forceinline__ double s_R(int s, bool lo, bool c4, const TPow& p) {
+    const double a0 = lo ? C0[0][s] : C1[0][s], a1 = lo ? C0[1][s] : C1[1][s];
+    const double a2 = lo ? C0[2][s] : C1[2][s], a3 = lo ? C0[3][s] : C1[3][s];
+    const double a4 = lo ? C0[4][s] : C1[4][s], a5 = lo ? C0[5][s] : C1[5][s];
+    const double a6 = lo ? C0[6][s] : C1[6][s];
+    const double b2 = (lo || c4) ? C0[8][s] : C1[8][s];
+    return -a0 * p.Tm2 / 2 - a1 * p.Tm1 + a2 * p.lnT + a3 * p.T + a4 * p.T2 / 2 + a5 * p.T3 / 3 +
+           a6 * p.T4 / 4 + b2;
+}
+
+// Mixture fit: sum_s w_s * (fit of s), for both ranges, w_s = Y_s * Ri_s [kJ/kg/K].
+struct MixFit {
+    double lo[8], hi[8];   // a1..a7, b1
+    double rmix;           // sum_s Y_s Ri_s
+};
+__device__ __forceinline__ MixFit mix_fit(const double (&Y)[NS], bool c4) {
+    MixFit m;
+#pragma unroll
+    for (int k = 0; k < 8; k++) { m.lo[k] = 0.0; m.hi[k] = 0.0; }
+    m.rmix = 0.0;
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        const double w = Y[s] * RI(s);
+        m.rmix += w;
+#pragma unroll
+        for (int k = 0; k < 7; k++) {
+            m.lo[k] += w * C0[k][s];
+            m.hi[k] += w * C1[k][s];
+        }
+        m.lo[7] += w * C0[7][s];
+        m.hi[7] += w * (c4 ? C0[7][s] : C1[7][s]);
+    }
+    return m;
+}
+
+// T from internal energy (Flowfield::Get_temp, src/Flowfield.cpp:1542-1566): Newton on
+// sum_s Y_s e_s(T) = E, at most 10 iterations, stop when |dT| < 1e-6 K. E in J/kg.
+__device__ __forceinline__ double newton_T(const MixFit& m, double T, double E, int* iters = nullptr) {
+    double T0 = T, Tn = T;
+    const double Ek = E * 1e-3;
+    int it = 0;
+    for (int count = 0; count < 10; count++) {
+        T0 = Tn;
+        const TPow p = tpow(T0);
+        const bool lo = T0 < 1000;
+        const double* c = lo ? m.lo : m.hi;
+        const double h = -c[0] * p.Tm1 + c[1] * p.lnT + c[2] * p.T + c[3] * p.T2 / 2 + c[4] * p.T3 / 3 +
+                         c[5] * p.T4 / 4 + c[6] * p.T5 / 5 + c[7];
+        const double cp = c[0] * p.Tm2 + c[1] * p.Tm1 + c[2] + c[3] * p.T + c[4] * p.T2 + c[5] * p.T3 +
+                          c[6] * p.T4;
+        const double f = (h - Ek) - m.rmix * T0;
+        const double fp = cp - m.rmix;
+        Tn = T0 - f / fp;
+        it++;
+        if (fabs(Tn - T0) < 1e-6) break;
+    }
+    if (iters) *iters = it;
+    return Tn;
+}
+
+// c_p of the mixture [kJ/kg/K] from the mixture fit.
+__device__ __forceinline__ double mix_cp(const MixFit& m, double T) {
+    const TPow p = tpow(T);
+    const double* c = (T < 1000) ? m.lo : m.hi;
+    return c[0] * p.Tm2 + c[1] * p.Tm1 + c[2] + c[3] * p.T + c[4] * p.T2 + c[5] * p.T3 + c[6] * p.T4;
+}
+
+// Primitive state of one cell, as Flowfield::Update_after_CS leaves it (src/Flowfield.cpp:1246-1284).
+struct Prim {
+    double D, U, V, W, E, T, P, H, Gamma, C, Rgas, Cp, Wav;
+    double Y[NS];
+};
+
+// Update_after_CS for one cell: CS[NC], previous T and E -> primitives. Quirk F3 solves T
+// from the previous E; FIXED recomputes E from CS first.
+__device__ __forceinline__ Prim prim_from_cs(const double (&cs)[NC], double T_old, double E_old, uint32_t q,
+                                              int* iters = nullptr) {
+    Prim o;
+    double D = 0;
+#pragma unroll
+    for (int s = 0; s < NS; s++) D += cs[s];
+    o.D = D;
+    const double rD = 1.0 / D;
+#pragma unroll
+    for (int s = 0; s < NS; s++) o.Y[s] = cs[s] / D;
+    o.U = cs[NS] * rD;
+    o.V = cs[NS + 1] * rD;
+    o.W = cs[NS + 2] * rD;
+    o.E = (cs[NS + 3] - 0.5 * D * (o.U * o.U + o.V * o.V + o.W * o.W)) / D;
+    const MixFit m = mix_fit(o.Y, q & Q_C4);
+    o.T = newton_T(m, T_old, (q & Q_F3) ? E_old : o.E, iters);
+    double sy = 0;
+#pragma unroll
+    for (int s = 0; s < NS; s++) sy += o.Y[s] / MW[s];
+    o.Wav = 1.0 / sy;
+    o.Rgas = R_UNIV * 1000 / o.Wav;
+    o.P = o.T * D * o.Rgas;
+    o.H = o.P / D + o.E;
+    o.Cp = mix_cp(m, o.T) * 1000;
+    o.Gamma = o.Cp / (o.Cp - R_UNIV * sy * 1000);
+    o.C = sqrt(o.Gamma * o.P / D);
+    return o;
+}
+
+// dT/drho_s (Flowfield::GetPartial_T, src/Flowfield.cpp:1578-1595); quirk C7 keeps h_s in kJ/kg.
+__device__ __forceinline__ void partial_T(const Prim& o, uint32_t q, double (&pt)[NS]) {
+    const TPow p = tpow(o.T);
+    const bool lo = o.T < 1000;
+    const double CV = o.Cp - o.Rgas;
+    const double ke = 0.5 * (o.U * o.U + o.V * o.V + o.W * o.W);
+    const double den = CV * o.D;
+    const double hs = (q & Q_C7) ? 1.0 : 1e3;
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        const double hi = RI(s) * h_R(s, lo, q & Q_C4, p) * hs;
+        pt[s] = (ke - hi + RI(s) * 1e3 * o.T) / den;
+    }
+}
+
+// ------------------------------------------------------------------ reconstruction
+// Limiters (include/Differ.hpp:33-82) and MUSCL_2 (include/Differ.hpp:149-238) for the face
+// between f0 and f1 with neighbours fm (behind f0) and f2 (beyond f1).
+__device__ __forceinline__ double lim(int limit, double R) {
+    switch (limit) {
+    case 1: return (R + fabs(R)) / (1.0 + fabs(R));
+    case 2: return (R * R + R) / (1.0 + R * R);
+    case 3: {
+        double r = 0.0, b = (1.0 < 2 * R) ? 1.0 : 2 * R, c = (2.0 < R) ? 2.0 : R;
+        if (r < b) r = b;
+        if (r < c) r = c;
+        return r;
+    }
+    case 4: {
+        if (!(R > 0)) return 0.0;
+        double r = 2 * R;
+        if (1.0 < r) r = 1.0;
+        const double c = (1 + R) / 2;
+        if (c < r) r = c;
+        return r;
+    }
+    default: return (R > 0) ? ((1.0 < R) ? 1.0 : R) : 0.0;
+    }
+}
+
+// kappa = 1/3 weights as the reference forms them: (1 - 1/3) and (1 + 1/3).
+constexpr double KM = 1.0 - 1.0 / 3.0;
+constexpr double KP = 1 + 1.0 / 3.0;
+
+// minmod branch of MUSCL_2 without the reciprocal divisions: for ratio r, the bracket
+// (1-k) phi(r) + (1+k) phi(1/r) r is 0 for r <= 0, (1-k) r + (1+k) r for 0 < r <= 1 and
+// (1-k) + (1+k) for r > 1 (the reference's (1/r)*r is 1 to rounding).
+__device__ __forceinline__ double mm_bracket(double r) {
+    if (!(r > 0)) return 0.0;
+    return (1.0 < r) ? (KM + KP) : (KM * r + KP * r);
+}
+
+__device__ __forceinline__ void muscl(int order, int limit, double fm, double f0, double f1, double f2,
+                                      double& fl, double& fr) {
+    if (order == 0) {
+        fl = f0;
+        fr = f1;
+        return;
+    }
+    const double A0 = f0 - fm, A1 = f1 - f0, A2 = f2 - f1;
+    const double L = A1 / (A0 + 1e-12);
+    const double R = A1 / (A2 + 1e-12);
+    double bl, br;
+    if (limit == 0) {
+        bl = mm_bracket(L);
+        br = mm_bracket(R);
+    } else {
+        bl = KM * lim(limit, L) + KP * lim(limit, 1.0 / L) * L;
+        br = KM * lim(limit, R) + KP * lim(limit, 1.0 / R) * R;
+    }
+    fl = f0 + 0.25 * bl * A0;
+    fr = f1 - 0.25 * br * A2;
+}
+
+// One face state (reconstructed).
+struct Face {
+    double P, D, U, V, W, H, G;
+    double Y[NS];
+};
+
+// AUSM-type interface flux (Flowfield::AUSM face loop, src/Flowfield.cpp:1394-1500).
+// dir: 0 = x, 1 = y, 2 = z. Writes NC flux components.
+__device__ __forceinline__ void ausm_flux(int dir, const Face& L, const Face& R, bool zero_y, double (&F)[NC]) {
+    const double CL = sqrt(L.G * L.P / L.D), CR = sqrt(R.G * R.P / R.D);
+    const double CI = 0.5 * (CL + CR);
+    const double unL = dir == 0 ? L.U : (dir == 1 ? L.V : L.W);
+    const double unR = dir == 0 ? R.U : (dir == 1 ? R.V : R.W);
+    const double MaL = unL / CI, MaR = unR / CI;
+    double B1, G1, B2, G2;
+    if (fabs(MaL) < 1.0) {
+        const double a = (MaL + 1.0) * (MaL + 1.0), b = MaL * MaL - 1.0;
+        B1 = 0.25 * a + 0.125 * (b * b);
+        G1 = 0.25 * a * (2.0 - MaL) + 3.0 / 16.0 * MaL * (b * b);
+    } else {
+        B1 = 0.5 * (MaL + fabs(MaL));
+        G1 = (MaL > 0) ? 1.0 : -0.0;    // B1 / MaL
+    }
+    if (fabs(MaR) < 1.0) {
+        const double a = (MaR - 1.0) * (MaR - 1.0), b = MaR * MaR - 1.0;
+        B2 = -0.25 * a - 0.125 * (b * b);
+        G2 = 0.25 * a * (2.0 + MaR) - 3.0 / 16.0 * MaR * (b * b);
+    } else {
+        B2 = 0.5 * (MaR - fabs(MaR));
+        G2 = (MaR > 0) ? 0.0 : 1.0;     // B2 / MaR
+    }
+    const double VI = (B1 + B2) * CI;
+    const bool up = VI >= 0.0;
+    const double Dx = up ? L.D : R.D, Ux = up ? L.U : R.U, Vx = up ? L.V : R.V, Wx = up ? L.W : R.W;
+    const double Hx = up ? L.H : R.H;
+    const double m = VI * Dx;
+#pragma unroll
+    for (int s = 0; s < NS; s++) F[s] = zero_y ? m * 0.0 : m * (up ? L.Y[s] : R.Y[s]);
+    const double pterm = G1 * L.P;
+    F[NS] = m * Ux;
+    F[NS + 1] = m * Vx;
+    F[NS + 2] = m * Wx;
+    F[NS + dir] = F[NS + dir] + pterm + G2 * R.P;
+    F[NS + 3] = VI * (Dx * Hx + 0.5 * Dx * (Ux * Ux + Vx * Vx + Wx * Wx));
+}
+
+// ------------------------------------------------------------------ kinetics
+// Rates of one cell (the common part of Reaction::Trapezoid, TrapezoidPrediction and
+// Diagonalized; src/Reaction.cpp:420-452,499-531,744-776). Inputs in reaction units:
+// Di [g/cm^3], Mc [mol/cm^3]. rtb: 0 keeps the summed third-body factor (quirk C1),
+// 1 forces it to 0 (quirk C2), 2 uses 1 where the sum is 0 (the correct default).
+struct Rates {
+    double KF[NR], KB[NR], RRF[NR], RRB[NR], RTB[NR], RR[NR];
+};
+
+__device__ __forceinline__ void kin_rates(double T, const double (&Mc)[NS], int rtb, bool c4, Rates& k) {
+    const TPow p = tpow(T);
+    const bool lo = T < 1000;
+    double G[NS];
+#pragma unroll
+    for (int s = 0; s < NS; s++) G[s] = R_UNIV * (h_R(s, lo, c4, p) - s_R(s, lo, c4, p) * T);
+    const double RT = R_UNIV * T;
+    const double rRT = 1.0 / RT;
+    const double rRuT = 1.0 / (RU_CAL * T);
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        // k_f = A T^b exp(-E / (Ru T))
+        double kf = AF[r];
+        if (BF[r] == 1.0) kf *= T;
+        else if (BF[r] == -1.0) kf *= p.Tm1;
+        else if (BF[r] == -2.0) kf *= p.Tm2;
+        else if (BF[r] != 0.0) kf *= exp(BF[r] * p.lnT);
+        if (EAF[r] != 0.0) kf *= exp(-EAF[r] * rRuT);
+        double rf = 1.0, rb = 1.0, tb = 0.0, dg = 0.0;
+        int xr = 0;
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+            const int nf = SF[s][r], nb = SB[s][r];
+            if (nb - nf != 0) dg += (double)(nb - nf) * G[s];
+            xr += nb - nf;
+            if (nf == 1) rf *= Mc[s];
+            else if (nf == 2) rf *= Mc[s] * Mc[s];
+            if (nb == 1) rb *= Mc[s];
+            else if (nb == 2) rb *= Mc[s] * Mc[s];
+            if (TB[s][r] != 0.0) tb += Mc[s] * TB[s][r];
+        }
+        // K_p = exp(-dg/RT) (P0 1e-6)^xr, K_c = K_p (R T)^-xr, k_b = k_f / K_c
+        double kp = exp(-dg * rRT);
+        if (xr == 1) kp *= (P0_PA * 1e-6);
+        else if (xr == -1) kp *= 1.0 / (P0_PA * 1e-6);
+        double kc = kp;
+        if (xr == 1) kc *= rRT;
+        else if (xr == -1) kc *= RT;
+        const double kb = kf / kc;
+        if (rtb == 1) tb = 0.0;
+        else if (rtb == 2 && tb == 0.0) tb = 1.0;
+        k.KF[r] = kf;
+        k.KB[r] = kb;
+        k.RRF[r] = rf;
+        k.RRB[r] = rb;
+        k.RTB[r] = tb;
+        k.RR[r] = kf * rf - kb * rb;
+    }
+}
+
+// Net mass production w_s = Mw_s sum_r (nu''-nu') R_TB RR [g/cm^3/s].
+__device__ __forceinline__ void kin_wdot(const Rates& k, double (&w)[NS]) {
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        double t = 0.0;
+#pragma unroll
+        for (int r = 0; r < NR; r++)
+            if (SB[s][r] - SF[s][r] != 0) t += (double)(SB[s][r] - SF[s][r]) * k.RTB[r] * k.RR[r];
+        w[s] = MW[s] * t;
+    }
+}
+
+// Point-implicit trapezoid (src/Reaction.cpp:464-482, 569-588): Di' [kg/m^3] from Di [g/cm^3].
+// Quirk C3 applies R_TB to the second term of Q and of P only.
+__device__ __forceinline__ void kin_trap(const Rates& k, const double (&Di)[NS], const double (&Mc)[NS], double dt,
+                                         bool c3, double (&Dout)[NS]) {
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        double P = 0.0, Q = 0.0;
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            const int nf = SF[s][r], nb = SB[s][r];
+            if (nf == 0 && nb == 0) continue;
+            const double qf = nb ? (double)nb * k.KF[r] * k.RRF[r] : 0.0;
+            const double qb = nf ? (double)nf * k.KB[r] * k.RRB[r] : 0.0;
+            const double pb = nb ? (double)nb * k.KB[r] * k.RRB[r] : 0.0;
+            const double pf = nf ? (double)nf * k.KF[r] * k.RRF[r] : 0.0;
+            if (c3) {
+                Q += qf + qb * k.RTB[r];
+                P += pb + pf * k.RTB[r];
+            } else {
+                Q += (qf + qb) * k.RTB[r];
+                P += (pb + pf) * k.RTB[r];
+            }
+        }
+        Q = MW[s] * Q;
+        P = (Mc[s] == 0) ? 0.0 : P / Mc[s];
+        Dout[s] = ((1.0 - dt / 2.0 * P) * Di[s] + dt * Q) / (1.0 + dt / 2.0 * P) * 1e3;
+    }
+}
+
+// Diagonal of the chemical Jacobian (src/Reaction.cpp:789-809). pt: dT/drho_s as stored by
+// GetPartial_T; quirk C6 uses it unscaled, FIXED multiplies the temperature path by 1e3.
+__device__ __forceinline__ void kin_diag(const Rates& k, double T, const double (&Di)[NS], const double (&pt)[NS],
+                                         bool c6, double (&md)[NS]) {
+    const double rRuT2 = 1.0 / (RU_CAL * (T * T));
+    double dRdT[NR];
+#pragma unroll
+    for (int r = 0; r < NR; r++) dRdT[r] = k.RR[r] * (BF[r] / T + EAF[r] * rRuT2);
+    const double tsc = c6 ? 1.0 : 1e3;
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        const double pts = pt[s] * tsc;
+        double t = 0.0;
+        const bool dz = (Di[s] == 0);
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            const int nf = SF[s][r], nb = SB[s][r];
+            const int dn = nb - nf;
+            if (dn == 0) continue;
+            double w1 = 0.0;
+            if (!dz) {
+                double a = nf ? k.KF[r] * (double)nf * k.RRF[r] / Di[s] : 0.0;
+                double b = nb ? k.KB[r] * (double)nb * k.RRB[r] / Di[s] : 0.0;
+                w1 = a - b;
+            }
+            double term = (w1 + dRdT[r] * pts) * k.RTB[r];
+            if (TB[s][r] != 0.0) term += k.RR[r] * TB[s][r] / MW[s];
+            t += (double)dn * term;
+        }
+        md[s] = MW[s] * t;
+    }
+}
+
+}  // namespace eu3d

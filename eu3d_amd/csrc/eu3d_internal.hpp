@@ -1,0 +1,78 @@
+// Context and launch declarations shared by the C-ABI layer and the kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "../../include/eu3d.h"
+#include "eu3d_device.hpp"
+
+namespace eu3d {
+
+// Ghost-inclusive grid extents; device fields are SoA with x fastest:
+// field[c * n3 + (k * NY + j) * NX + i].
+struct G3 {
+    int ni, nj, nk, bc, NX, NY, NZ;
+    size_t n3;
+};
+
+struct Fields {
+    double* f[EU3D_FIELD_COUNT];
+};
+
+// Device-resident scalars of a context.
+struct Scalars {
+    double dt, time, dtm;            // dtm: chemistry interval of the current adaptive call
+    unsigned long long dtmin_bits;   // min over cells of the CFL bound, as ordered bits
+    int nonfinite;
+    int pad;
+    unsigned long long chem_head;    // persistent-kernel work queue head
+    unsigned long long chem_active;  // number of queued cells
+    unsigned long long chem_substeps;
+    unsigned long long chem_nmax;
+};
+
+}  // namespace eu3d
+
+struct eu3d_ctx {
+    eu3d::G3 g{};
+    uint32_t quirks = 0;
+    int device = 0;
+    int num_cu = 256;
+    hipStream_t stream = nullptr;
+    eu3d::Fields fl{};
+    double *xnode = nullptr, *ynode = nullptr, *znode = nullptr;   // device, ghost-inclusive
+    double dx0 = 0, dy0 = 0, dz0 = 0;                              // node(1) - node(0)
+    eu3d::Scalars* sc = nullptr;                                  // device
+    unsigned int* active = nullptr;                                // compaction list (interior cells)
+    double* rk_save = nullptr;                                     // RK2 copy of CS, T, E
+    // batched chemistry scratch
+    unsigned int* batch_active = nullptr;
+    long long batch_cap = 0;
+    eu3d_chem_schedule sched{256, 4, 1};
+    eu3d_chem_stats stats{};
+    std::string err;
+};
+
+namespace eu3d {
+// Launchers (eu3d_kernels.hip). All enqueue on ctx->stream and return hipError_t.
+hipError_t launch_aos_to_soa(eu3d_ctx* c, const double* aos, double* soa, int nc);
+hipError_t launch_soa_to_aos(eu3d_ctx* c, const double* soa, double* aos, int nc);
+hipError_t launch_update_after_cs(eu3d_ctx* c);
+hipError_t launch_partial_t(eu3d_ctx* c);
+hipError_t launch_field_initial(eu3d_ctx* c);
+hipError_t launch_fill_ghosts(eu3d_ctx* c, const int kinds[6]);
+hipError_t launch_advection(eu3d_ctx* c, int diff, int limiter);
+hipError_t launch_cfl(eu3d_ctx* c, double cfl, double final_time);
+hipError_t launch_update_after_adv(eu3d_ctx* c);
+hipError_t launch_explicit(eu3d_ctx* c);
+hipError_t launch_diagonalized(eu3d_ctx* c);
+hipError_t launch_update_imex(eu3d_ctx* c);
+hipError_t launch_trapezoid(eu3d_ctx* c);
+hipError_t launch_trap_adaptive(eu3d_ctx* c, int nchem_max, bool predict_only);
+hipError_t launch_check_finite(eu3d_ctx* c);
+hipError_t launch_set_scalars(eu3d_ctx* c, double dt, double time);
+hipError_t launch_step(eu3d_ctx* c, const eu3d_step_params* p);
+hipError_t launch_chem_batch(eu3d_ctx* c, long long n, double* rho, double* T, double* E, double dtm,
+                             int nchem_max, int* nchem_out);
+}  // namespace eu3d

@@ -1,0 +1,946 @@
+// HIP kernels for the EU3D hot path on gfx950 (FP64, SoA, x fastest) and their launchers.
+//
+// Each kernel is the body of one reference method over the interior cells, on registers:
+// the per-cell math lives in eu3d_device.hpp. Launch shapes: interior kernels use 64x4 blocks
+// over (i, j) and one grid layer per k, so a wavefront covers 64 consecutive x cells and every
+// field access is a coalesced 512-byte row segment.
+#include "eu3d_internal.hpp"
+
+using namespace eu3d;
+
+#define IDX(g, i, j, k) ((((size_t)(k)) * (size_t)(g).NY + (size_t)(j)) * (size_t)(g).NX + (size_t)(i))
+#define FLD(F, id) ((F).f[id])
+
+namespace {
+
+constexpr int BX = 64, BY = 4;
+
+inline dim3 interior_grid(const G3& g) { return dim3((g.ni + BX - 1) / BX, (g.nj + BY - 1) / BY, g.nk); }
+
+__device__ __forceinline__ bool interior_ijk(const G3& g, int& i, int& j, int& k) {
+    i = g.bc + (int)(blockIdx.x * BX + threadIdx.x);
+    j = g.bc + (int)(blockIdx.y * BY + threadIdx.y);
+    k = g.bc + (int)blockIdx.z;
+    return i < g.ni + g.bc && j < g.nj + g.bc;
+}
+
+__device__ __forceinline__ void load_cs(const Fields& F, const G3& g, size_t c, double (&cs)[NC]) {
+    const double* p = FLD(F, EU3D_FIELD_CS);
+#pragma unroll
+    for (int s = 0; s < NC; s++) cs[s] = p[s * g.n3 + c];
+}
+
+// Write everything Update_after_CS leaves behind for one cell.
+__device__ __forceinline__ void store_prims(const Fields& F, const G3& g, size_t c, const double (&cs)[NC],
+                                            const Prim& o) {
+    FLD(F, EU3D_FIELD_D)[c] = o.D;
+    FLD(F, EU3D_FIELD_T)[c] = o.T;
+    FLD(F, EU3D_FIELD_U)[c] = o.U;
+    FLD(F, EU3D_FIELD_V)[c] = o.V;
+    FLD(F, EU3D_FIELD_W)[c] = o.W;
+    FLD(F, EU3D_FIELD_E)[c] = o.E;
+    FLD(F, EU3D_FIELD_WAV)[c] = o.Wav;
+    FLD(F, EU3D_FIELD_RGAS)[c] = o.Rgas;
+    FLD(F, EU3D_FIELD_P)[c] = o.P;
+    FLD(F, EU3D_FIELD_H)[c] = o.H;
+    FLD(F, EU3D_FIELD_CP)[c] = o.Cp;
+    FLD(F, EU3D_FIELD_GAMMA)[c] = o.Gamma;
+    FLD(F, EU3D_FIELD_C)[c] = o.C;
+    FLD(F, EU3D_FIELD_MA)[c] = sqrt(o.U * o.U + o.V * o.V + o.W * o.W) / o.C;
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        FLD(F, EU3D_FIELD_DI)[s * g.n3 + c] = cs[s];
+        FLD(F, EU3D_FIELD_MC)[s * g.n3 + c] = cs[s] / MW[s] * 1000;
+        FLD(F, EU3D_FIELD_YI)[s * g.n3 + c] = o.Y[s];
+    }
+}
+
+// ---------------------------------------------------------------- layout conversion
+__global__ void k_aos_to_soa(G3 g, const double* __restrict__ aos, double* __restrict__ soa, int nc) {
+    const size_t total = g.n3 * (size_t)nc;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const size_t c = t / g.n3, cell = t - c * g.n3;
+        const size_t i = cell % g.NX, jk = cell / g.NX, j = jk % g.NY, k = jk / g.NY;
+        soa[t] = aos[((i * g.NY + j) * g.NZ + k) * nc + c];
+    }
+}
+
+__global__ void k_soa_to_aos(G3 g, const double* __restrict__ soa, double* __restrict__ aos, int nc) {
+    const size_t total = g.n3 * (size_t)nc;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const size_t c = t % nc, cell = t / nc;
+        const size_t k = cell % g.NZ, ij = cell / g.NZ, j = ij % g.NY, i = ij / g.NY;
+        aos[t] = soa[c * g.n3 + (k * g.NY + j) * g.NX + i];
+    }
+}
+
+// ---------------------------------------------------------------- primitive recovery
+// Flowfield::Update_after_CS, src/Flowfield.cpp:1246-1284
+__global__ void __launch_bounds__(256) k_update_after_cs(G3 g, Fields F, uint32_t q) {
+    int i, j, k;
+    if (!interior_ijk(g, i, j, k)) return;
+    const size_t c = IDX(g, i, j, k);
+    double cs[NC];
+    load_cs(F, g, c, cs);
+    const Prim o = prim_from_cs(cs, FLD(F, EU3D_FIELD_T)[c], FLD(F, EU3D_FIELD_E)[c], q);
+    store_prims(F, g, c, cs, o);
+}
+
+// Flowfield::GetPartial_T, src/Flowfield.cpp:1578-1595
+__global__ void __launch_bounds__(256) k_partial_t(G3 g, Fields F, uint32_t q) {
+    int i, j, k;
+    if (!interior_ijk(g, i, j, k)) return;
+    const size_t c = IDX(g, i, j, k);
+    Prim o;
+    o.D = FLD(F, EU3D_FIELD_D)[c];
+    o.U = FLD(F, EU3D_FIELD_U)[c];
+    o.V = FLD(F, EU3D_FIELD_V)[c];
+    o.W = FLD(F, EU3D_FIELD_W)[c];
+    o.T = FLD(F, EU3D_FIELD_T)[c];
+    o.Cp = FLD(F, EU3D_FIELD_CP)[c];
+    o.Rgas = FLD(F, EU3D_FIELD_RGAS)[c];
+    double pt[NS];
+    partial_T(o, q, pt);
+    double* PT = FLD(F, EU3D_FIELD_PT);
+#pragma unroll
+    for (int s = 0; s < NS; s++) PT[s * g.n3 + c] = pt[s];
+    const double CV = o.Cp - o.Rgas;
+    PT[(NS + 1) * g.n3 + c] = -o.U / (CV * o.D);
+    PT[(NS + 2) * g.n3 + c] = 1.0 / (CV * o.D);
+}
+
+// Flowfield::FieldInitial second loop (src/Flowfield.cpp:193-226) over every cell including
+// ghosts: from T, P, U, V, W, Yi to Wav, Rgas, D, Di, Mc, Cp, H, E, Gamma, C and CS.
+__global__ void k_field_initial(G3 g, Fields F, uint32_t q) {
+    const size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (c >= g.n3) return;
+    double Y[NS];
+#pragma unroll
+    for (int s = 0; s < NS; s++) Y[s] = FLD(F, EU3D_FIELD_YI)[s * g.n3 + c];
+    const double T = FLD(F, EU3D_FIELD_T)[c], P = FLD(F, EU3D_FIELD_P)[c];
+    const double U = FLD(F, EU3D_FIELD_U)[c], V = FLD(F, EU3D_FIELD_V)[c], W = FLD(F, EU3D_FIELD_W)[c];
+    const TPow p = tpow(T);
+    const bool lo = T < 1000;
+    double sy = 0, cp = 0, h = 0, e = 0;
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        sy += Y[s] / MW[s];
+        const double hi = RI(s) * h_R(s, lo, q & Q_C4, p);
+        cp += RI(s) * cp_R(s, lo, p) * Y[s];
+        h += Y[s] * hi;
+        e += Y[s] * (hi - RI(s) * T);
+    }
+    const double Wav = 1.0 / sy, Rg = R_UNIV * 1000 / Wav, D = P / (Rg * T);
+    const double Cp = cp * 1000, E = e * 1000;
+    const double Gm = Cp / (Cp - R_UNIV * sy * 1000);
+    FLD(F, EU3D_FIELD_WAV)[c] = Wav;
+    FLD(F, EU3D_FIELD_RGAS)[c] = Rg;
+    FLD(F, EU3D_FIELD_D)[c] = D;
+    FLD(F, EU3D_FIELD_CP)[c] = Cp;
+    FLD(F, EU3D_FIELD_H)[c] = h * 1000;
+    FLD(F, EU3D_FIELD_E)[c] = E;
+    FLD(F, EU3D_FIELD_GAMMA)[c] = Gm;
+    FLD(F, EU3D_FIELD_C)[c] = sqrt(Gm * P / D);
+    double* CS = FLD(F, EU3D_FIELD_CS);
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        const double di = Y[s] * D;
+        FLD(F, EU3D_FIELD_DI)[s * g.n3 + c] = di;
+        FLD(F, EU3D_FIELD_MC)[s * g.n3 + c] = di / MW[s] * 1000;
+        CS[s * g.n3 + c] = D * Y[s];
+    }
+    CS[NS * g.n3 + c] = D * U;
+    CS[(NS + 1) * g.n3 + c] = D * V;
+    CS[(NS + 2) * g.n3 + c] = D * W;
+    CS[(NS + 3) * g.n3 + c] = D * E + 0.5 * (U * U + V * V + W * W) * D;
+}
+
+// ---------------------------------------------------------------- ghost fill
+__device__ __forceinline__ int ghost_src(int kind, bool lo, int gi, int n, int bc) {
+    if (lo) {
+        if (kind == EU3D_BC_OUTFLOW) return bc;
+        if (kind == EU3D_BC_PERIODIC) return n + gi;
+        return 2 * bc - 1 - gi;                       // RSBI, REFLECT
+    }
+    if (kind == EU3D_BC_OUTFLOW) return n + bc - 1;
+    if (kind == EU3D_BC_PERIODIC) return bc + gi;
+    return n + bc - 1 - gi;
+}
+
+__device__ __forceinline__ void copy_prims(const Fields& F, const G3& g, size_t d, size_t s, int flip) {
+    const int ids[9] = {EU3D_FIELD_U, EU3D_FIELD_V, EU3D_FIELD_W, EU3D_FIELD_P, EU3D_FIELD_D,
+                        EU3D_FIELD_T, EU3D_FIELD_H, EU3D_FIELD_GAMMA, EU3D_FIELD_E};
+#pragma unroll
+    for (int t = 0; t < 9; t++) FLD(F, ids[t])[d] = FLD(F, ids[t])[s];
+    if (flip == 1) FLD(F, EU3D_FIELD_U)[d] = -FLD(F, EU3D_FIELD_U)[s];
+    if (flip == 2) FLD(F, EU3D_FIELD_V)[d] = -FLD(F, EU3D_FIELD_V)[s];
+    if (flip == 3) FLD(F, EU3D_FIELD_W)[d] = -FLD(F, EU3D_FIELD_W)[s];
+    double* Y = FLD(F, EU3D_FIELD_YI);
+#pragma unroll
+    for (int sp = 0; sp < NS; sp++) Y[sp * g.n3 + d] = Y[sp * g.n3 + s];
+}
+
+// x faces over interior (j, k); src/Flowfield.cpp:564-610
+__global__ void k_fill_x(G3 g, Fields F, int klo, int khi) {
+    const int j = g.bc + blockIdx.x * blockDim.x + threadIdx.x, k = g.bc + blockIdx.y;
+    const int side = blockIdx.z / g.bc, gi = blockIdx.z % g.bc;
+    if (j >= g.nj + g.bc) return;
+    const int kind = side == 0 ? klo : khi;
+    if (kind == EU3D_BC_NONE) return;
+    const int di = side == 0 ? gi : g.ni + g.bc + gi;
+    const int si = ghost_src(kind, side == 0, gi, g.ni, g.bc);
+    copy_prims(F, g, IDX(g, di, j, k), IDX(g, si, j, k), kind == EU3D_BC_REFLECT ? 1 : 0);
+}
+
+// y faces over all i and interior k; src/Flowfield.cpp:616-673. RSBI on y- recomputes the
+// thermodynamic state from (p, T, Y) with the NASA fits (:616-647).
+__global__ void k_fill_y(G3 g, Fields F, int klo, int khi, uint32_t q) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, k = g.bc + blockIdx.y;
+    const int side = blockIdx.z / g.bc, gi = blockIdx.z % g.bc;
+    if (i >= g.NX) return;
+    const int kind = side == 0 ? klo : khi;
+    if (kind == EU3D_BC_NONE) return;
+    const int dj = side == 0 ? gi : g.nj + g.bc + gi;
+    const int sj = ghost_src(kind, side == 0, gi, g.nj, g.bc);
+    const size_t d = IDX(g, i, dj, k), s = IDX(g, i, sj, k);
+    if (kind == EU3D_BC_RSBI && side == 0) {
+        const int ids[5] = {EU3D_FIELD_U, EU3D_FIELD_V, EU3D_FIELD_W, EU3D_FIELD_P, EU3D_FIELD_T};
+#pragma unroll
+        for (int t = 0; t < 5; t++) FLD(F, ids[t])[d] = FLD(F, ids[t])[s];
+        const double T = FLD(F, EU3D_FIELD_T)[d];
+        const TPow p = tpow(T);
+        const bool lo = T < 1000;
+        double rg = 0, cp = 0, h = 0, e = 0;
+#pragma unroll
+        for (int sp = 0; sp < NS; sp++) {
+            const double y = FLD(F, EU3D_FIELD_YI)[sp * g.n3 + s];
+            FLD(F, EU3D_FIELD_YI)[sp * g.n3 + d] = y;
+            FLD(F, EU3D_FIELD_MC)[sp * g.n3 + d] = FLD(F, EU3D_FIELD_MC)[sp * g.n3 + s];
+            const double hi = RI(sp) * h_R(sp, lo, q & Q_C4, p);
+            rg += y * RI(sp);
+            cp += RI(sp) * cp_R(sp, lo, p) * y;
+            h += y * hi;
+            e += y * (hi - RI(sp) * T);
+        }
+        const double Rg = rg * 1000;
+        FLD(F, EU3D_FIELD_RGAS)[d] = Rg;
+        FLD(F, EU3D_FIELD_D)[d] = FLD(F, EU3D_FIELD_P)[d] / (T * Rg);
+        FLD(F, EU3D_FIELD_CP)[d] = cp * 1000;
+        FLD(F, EU3D_FIELD_H)[d] = h * 1000;
+        FLD(F, EU3D_FIELD_E)[d] = e * 1000;
+        FLD(F, EU3D_FIELD_GAMMA)[d] = (cp * 1000) / (cp * 1000 - Rg);
+    } else {
+        copy_prims(F, g, d, s, kind == EU3D_BC_REFLECT ? 2 : 0);
+    }
+}
+
+// z faces over all (i, j); src/Flowfield.cpp:679-769
+__global__ void k_fill_z(G3 g, Fields F, int klo, int khi) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    const int side = blockIdx.z / g.bc, gi = blockIdx.z % g.bc;
+    if (i >= g.NX) return;
+    const int kind = side == 0 ? klo : khi;
+    if (kind == EU3D_BC_NONE) return;
+    const int dk = side == 0 ? gi : g.nk + g.bc + gi;
+    const int sk = ghost_src(kind, side == 0, gi, g.nk, g.bc);
+    copy_prims(F, g, IDX(g, i, j, dk), IDX(g, i, j, sk), kind == EU3D_BC_REFLECT ? 3 : 0);
+}
+
+// ---------------------------------------------------------------- advection
+// One face per thread: MUSCL on the 16 primitives from the 4-cell stencil, then the AUSM flux
+// (Flowfield::AUSM, src/Flowfield.cpp:1330-1502). Faces along dir cover [bc-1, n+bc); the other
+// two indices cover the interior, which is every face TimeAdv::EE reads.
+__global__ void __launch_bounds__(256) k_face_flux(G3 g, Fields F, int dir, int order, int limiter, uint32_t q,
+                                                   double* __restrict__ Fo) {
+    const int i = (dir == 0 ? g.bc - 1 : g.bc) + (int)(blockIdx.x * BX + threadIdx.x);
+    const int j = (dir == 1 ? g.bc - 1 : g.bc) + (int)(blockIdx.y * BY + threadIdx.y);
+    const int k = (dir == 2 ? g.bc - 1 : g.bc) + (int)blockIdx.z;
+    if (i >= g.ni + g.bc || j >= g.nj + g.bc || k >= g.nk + g.bc) return;
+    const size_t c = IDX(g, i, j, k);
+    const size_t st = dir == 0 ? 1 : (dir == 1 ? (size_t)g.NX : (size_t)g.NX * g.NY);
+    Face L, R;
+    auto rec = [&](const double* f, double& l, double& r) {
+        muscl(order, limiter, f[c - st], f[c], f[c + st], f[c + 2 * st], l, r);
+    };
+    rec(FLD(F, EU3D_FIELD_P), L.P, R.P);
+    rec(FLD(F, EU3D_FIELD_D), L.D, R.D);
+    rec(FLD(F, EU3D_FIELD_U), L.U, R.U);
+    rec(FLD(F, EU3D_FIELD_V), L.V, R.V);
+    rec(FLD(F, EU3D_FIELD_W), L.W, R.W);
+    rec(FLD(F, EU3D_FIELD_H), L.H, R.H);
+    rec(FLD(F, EU3D_FIELD_GAMMA), L.G, R.G);
+    const double* Y = FLD(F, EU3D_FIELD_YI);
+#pragma unroll
+    for (int s = 0; s < NS; s++) rec(Y + s * g.n3, L.Y[s], R.Y[s]);
+    double fl[NC];
+    ausm_flux(dir, L, R, (q & Q_F1B) && i >= g.ni, fl);
+#pragma unroll
+    for (int s = 0; s < NC; s++) Fo[s * g.n3 + c] = fl[s];
+}
+
+// TimeAdv::EE, src/TimeAdv.cpp:29-43
+__global__ void __launch_bounds__(256) k_ee(G3 g, Fields F, const double* __restrict__ xn, const double* __restrict__ yn,
+                                            const double* __restrict__ zn, const Scalars* __restrict__ sc) {
+    int i, j, k;
+    if (!interior_ijk(g, i, j, k)) return;
+    const size_t c = IDX(g, i, j, k);
+    const double dt = sc->dt;
+    const double cx = -dt / (xn[i] - xn[i - 1]);
+    const double cy = dt / (yn[j] - yn[j - 1]);
+    const double cz = dt / (zn[k] - zn[k - 1]);
+    const size_t sy = g.NX, sz = (size_t)g.NX * g.NY;
+    const double *Fx = FLD(F, EU3D_FIELD_F), *Fy = FLD(F, EU3D_FIELD_G), *Fz = FLD(F, EU3D_FIELD_Q);
+    double* R = FLD(F, EU3D_FIELD_RHS);
+#pragma unroll
+    for (int s = 0; s < NC; s++) {
+        const size_t o = s * g.n3 + c;
+        R[o] = cx * (Fx[o] - Fx[o - 1]) - cy * (Fy[o] - Fy[o - sy]) - cz * (Fz[o] - Fz[o - sz]);
+    }
+}
+
+// ---------------------------------------------------------------- CFL
+// Flowfield::CFLcondition, src/Flowfield.cpp:242-272: per-block min, then one 64-bit atomic min
+// on the ordered bit pattern (positive doubles order as unsigned integers).
+__global__ void __launch_bounds__(256) k_cfl(G3 g, Fields F, double dx, double dy, double dz, uint32_t q,
+                                             Scalars* sc) {
+    int i, j, k;
+    double v = 1e10;
+    if (interior_ijk(g, i, j, k)) {
+        const size_t c = IDX(g, i, j, k);
+        const double C = FLD(F, EU3D_FIELD_C)[c];
+        const double a = dx / (fabs(FLD(F, EU3D_FIELD_U)[c]) + C), b = dy / (fabs(FLD(F, EU3D_FIELD_V)[c]) + C);
+        v = (b < a) ? b : a;
+        if (!(q & Q_F6)) {
+            const double z = dz / (fabs(FLD(F, EU3D_FIELD_W)[c]) + C);
+            v = (z < v) ? z : v;
+        }
+        if (!(v < 1e10)) v = 1e10;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(v, off);
+        v = (o < v) ? o : v;
+    }
+    __shared__ double red[4];
+    const int t = threadIdx.y * BX + threadIdx.x;
+    if ((t & 63) == 0) red[t >> 6] = v;
+    __syncthreads();
+    if (t == 0) {
+        double m = red[0];
+        for (int w = 1; w < 4; w++) m = (red[w] < m) ? red[w] : m;
+        atomicMin(&sc->dtmin_bits, (unsigned long long)__double_as_longlong(m));
+    }
+}
+
+__global__ void k_cfl_finish(Scalars* sc, double cfl, double final_time) {
+    double dt = cfl * __longlong_as_double((long long)sc->dtmin_bits);
+    if (sc->time + dt > final_time) dt = final_time - sc->time;
+    sc->dt = dt;
+    sc->time += dt;
+    sc->dtmin_bits = (unsigned long long)__double_as_longlong(1e10);
+}
+
+__global__ void k_set_scalars(Scalars* sc, double dt, double time) {
+    sc->dt = dt;
+    sc->time = time;
+    sc->dtmin_bits = (unsigned long long)__double_as_longlong(1e10);
+    sc->nonfinite = 0;
+}
+
+__global__ void k_advance_time(Scalars* sc) { sc->time += sc->dt; }
+
+// ---------------------------------------------------------------- state updates
+// Flowfield::Update_after_Adv, src/Flowfield.cpp:1140-1157 (quirk F1: species only)
+__global__ void __launch_bounds__(256) k_add_rhs(G3 g, Fields F, int ncomp) {
+    int i, j, k;
+    if (!interior_ijk(g, i, j, k)) return;
+    const size_t c = IDX(g, i, j, k);
+    double* CS = FLD(F, EU3D_FIELD_CS);
+    const double* R = FLD(F, EU3D_FIELD_RHS);
+    for (int s = 0; s < ncomp; s++) CS[s * g.n3 + c] = CS[s * g.n3 + c] + R[s * g.n3 + c];
+}
+
+// Flowfield::Explicit(), src/Flowfield.cpp:1169-1186 (quirk F2: rhoE into the rho*w slot)
+__global__ void __launch_bounds__(256) k_explicit(G3 g, Fields F, uint32_t q) {
+    int i, j, k;
+    if (!interior_ijk(g, i, j, k)) return;
+    const size_t c = IDX(g, i, j, k);
+    double* CS = FLD(F, EU3D_FIELD_CS);
+    const double* Di = FLD(F, EU3D_FIELD_DI);
+    double D = 0.0;
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        const double v = Di[s * g.n3 + c];
+        CS[s * g.n3 + c] = v;
+        D += v;
+    }
+    FLD(F, EU3D_FIELD_D)[c] = D;
+    const double U = FLD(F, EU3D_FIELD_U)[c], V = FLD(F, EU3D_FIELD_V)[c], W = FLD(F, EU3D_FIELD_W)[c];
+    const double et = D * FLD(F, EU3D_FIELD_E)[c] + 0.5 * (U * U + V * V + W * W) * D;
+    CS[NS * g.n3 + c] = U * D;
+    CS[(NS + 1) * g.n3 + c] = V * D;
+    if (q & Q_F2) {
+        CS[(NS + 2) * g.n3 + c] = et;
+    } else {
+        CS[(NS + 2) * g.n3 + c] = W * D;
+        CS[(NS + 3) * g.n3 + c] = et;
+    }
+}
+
+// ---------------------------------------------------------------- chemistry
+__device__ __forceinline__ void react_units(const Fields& F, const G3& g, size_t c, double (&Mc)[NS],
+                                            double (&Dr)[NS]) {
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        Mc[s] = FLD(F, EU3D_FIELD_MC)[s * g.n3 + c] * 1e-6;
+        Dr[s] = FLD(F, EU3D_FIELD_DI)[s * g.n3 + c] * 1e-3;
+    }
+}
+
+// Reaction::Diagonalized, src/Reaction.cpp:736-811
+__global__ void __launch_bounds__(256) k_diagonalized(G3 g, Fields F, uint32_t q) {
+    int i, j, k;
+    if (!interior_ijk(g, i, j, k)) return;
+    const size_t c = IDX(g, i, j, k);
+    double Mc[NS], Dr[NS], w[NS], pt[NS], md[NS];
+    react_units(F, g, c, Mc, Dr);
+    const double T = FLD(F, EU3D_FIELD_T)[c];
+    Rates kr;
+    kin_rates(T, Mc, 2, q & Q_C4, kr);
+    kin_wdot(kr, w);
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        FLD(F, EU3D_FIELD_CMS)[s * g.n3 + c] = w[s] * 1e3;
+        pt[s] = FLD(F, EU3D_FIELD_PT)[s * g.n3 + c];
+    }
+    kin_diag(kr, T, Dr, pt, q & Q_C6, md);
+#pragma unroll
+    for (int s = 0; s < NS; s++) FLD(F, EU3D_FIELD_MD)[s * g.n3 + c] = md[s];
+}
+
+// Flowfield::Update_IMEX conserved update, src/Flowfield.cpp:1219-1228. MD(c,c) for c >= NS is
+// never written by the reference (quirk I2) and is 0, so those components add RHS unchanged.
+__global__ void __launch_bounds__(256) k_update_imex(G3 g, Fields F, const Scalars* __restrict__ sc) {
+    int i, j, k;
+    if (!interior_ijk(g, i, j, k)) return;
+    const size_t c = IDX(g, i, j, k);
+    const double dt = sc->dt;
+    double* CS = FLD(F, EU3D_FIELD_CS);
+    double* R = FLD(F, EU3D_FIELD_RHS);
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        const size_t o = s * g.n3 + c;
+        const double r = R[o] + FLD(F, EU3D_FIELD_CMS)[o] * dt;
+        R[o] = r;
+        CS[o] = CS[o] + r / (1.0 - dt * FLD(F, EU3D_FIELD_MD)[o]);
+    }
+#pragma unroll
+    for (int s = NS; s < NC; s++) CS[s * g.n3 + c] = CS[s * g.n3 + c] + R[s * g.n3 + c];
+}
+
+// Reaction::Trapezoid full field, src/Reaction.cpp:412-485 (quirk C1 for the R_TB default)
+__global__ void __launch_bounds__(256) k_trapezoid(G3 g, Fields F, uint32_t q, const Scalars* __restrict__ sc) {
+    int i, j, k;
+    if (!interior_ijk(g, i, j, k)) return;
+    const size_t c = IDX(g, i, j, k);
+    double Mc[NS], Dr[NS], w[NS], Dn[NS];
+    react_units(F, g, c, Mc, Dr);
+    Rates kr;
+    kin_rates(FLD(F, EU3D_FIELD_T)[c], Mc, (q & Q_C1) ? 0 : 2, q & Q_C4, kr);
+    kin_wdot(kr, w);
+    kin_trap(kr, Dr, Mc, sc->dt, q & Q_C3, Dn);
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        FLD(F, EU3D_FIELD_CMS)[s * g.n3 + c] = w[s] * 1e3;
+        FLD(F, EU3D_FIELD_DI)[s * g.n3 + c] = Dn[s];
+    }
+}
+
+__global__ void __launch_bounds__(256) k_check_finite(G3 g, Fields F, Scalars* sc) {
+    int i, j, k;
+    if (!interior_ijk(g, i, j, k)) return;
+    const size_t c = IDX(g, i, j, k);
+    const double d = FLD(F, EU3D_FIELD_D)[c], t = FLD(F, EU3D_FIELD_T)[c];
+    if (!isfinite(d) || !isfinite(t)) atomicOr(&sc->nonfinite, 1);
+}
+
+// SSP-RK2 second stage: U^{n+1} = 1/2 U^n + 1/2 (U* + dt L(U*)), with U* in CS and
+// dt L(U*) in RHS; T and E of U* seed the Newton.
+__global__ void __launch_bounds__(256) k_rk2_combine(G3 g, Fields F, const double* __restrict__ save) {
+    int i, j, k;
+    if (!interior_ijk(g, i, j, k)) return;
+    const size_t c = IDX(g, i, j, k);
+    double* CS = FLD(F, EU3D_FIELD_CS);
+    const double* R = FLD(F, EU3D_FIELD_RHS);
+#pragma unroll
+    for (int s = 0; s < NC; s++) {
+        const size_t o = s * g.n3 + c;
+        CS[o] = 0.5 * save[o] + 0.5 * (CS[o] + R[o]);
+    }
+}
+
+__global__ void k_copy(const double* __restrict__ a, double* __restrict__ b, size_t n) {
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
+        b[t] = a[t];
+}
+
+// ---------------------------------------------------------------- adaptive trapezoid
+// Cell state carried through the sub-steps of Euler::Trapezoid(type 1): partial densities
+// [kg/m^3], velocity, internal energy, temperature.
+struct ChemCell {
+    double Di[NS];
+    double U, V, W, E, T;
+};
+
+// Reaction-unit inputs for one sub-step, as Update_after_CS leaves Mc and the reaction code
+// rescales it (src/Flowfield.cpp:1261, src/Reaction.cpp:502-503).
+__device__ __forceinline__ void chem_inputs(const ChemCell& c, double (&Mc)[NS], double (&Dr)[NS]) {
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        Mc[s] = (c.Di[s] / MW[s] * 1000) * 1e-6;
+        Dr[s] = c.Di[s] * 1e-3;
+    }
+}
+
+// Sub-step tail: the trapezoid update with rates kr, then Flowfield::Explicit(cell) and
+// Update_after_CS(cell) (src/Flowfield.cpp:1189-1203,1287-1319). Leaves the conserved state in cs.
+__device__ __forceinline__ Prim chem_apply(ChemCell& c, const Rates& kr, const double (&Mc)[NS],
+                                           const double (&Dr)[NS], double dt, uint32_t q, double (&cs)[NC]) {
+    double Dn[NS];
+    kin_trap(kr, Dr, Mc, dt, q & Q_C3, Dn);
+    double D = 0.0;
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        cs[s] = Dn[s];
+        D += cs[s];
+    }
+    cs[NS] = c.U * D;
+    cs[NS + 1] = c.V * D;
+    cs[NS + 2] = c.W * D;
+    cs[NS + 3] = D * c.E + 0.5 * (c.U * c.U + c.V * c.V + c.W * c.W) * D;
+    const Prim o = prim_from_cs(cs, c.T, c.E, q);
+#pragma unroll
+    for (int s = 0; s < NS; s++) c.Di[s] = cs[s];
+    c.U = o.U;
+    c.V = o.V;
+    c.W = o.W;
+    c.E = o.E;
+    c.T = o.T;
+    return o;
+}
+
+// Reaction::TrapezoidPrediction sub-step count (src/Reaction.cpp:650-668) from the same rates
+// the first sub-step uses. nmax caps the count.
+__device__ __forceinline__ int chem_nchem(const Rates& kr, const double (&Dr)[NS], const double (&Y)[NS],
+                                          double dtm, int nmax) {
+    double w[NS];
+    kin_wdot(kr, w);
+    double dt = dtm;
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        if (Y[s] >= 1e-6 && w[s] != 0) {
+            const double t = fabs(-Dr[s] / w[s]);
+            if (t < dt) dt = t;
+        }
+    }
+    double nd = ceil(dtm / dt);
+    if (nd > (double)nmax) nd = (double)nmax;
+    return (int)nd;
+}
+
+// Grid accessor for the persistent kernel.
+struct GridCells {
+    G3 g;
+    Fields F;
+    __device__ void load(unsigned id, ChemCell& c, int& n) const {
+        const size_t cc = id;
+#pragma unroll
+        for (int s = 0; s < NS; s++) c.Di[s] = FLD(F, EU3D_FIELD_DI)[s * g.n3 + cc];
+        c.U = FLD(F, EU3D_FIELD_U)[cc];
+        c.V = FLD(F, EU3D_FIELD_V)[cc];
+        c.W = FLD(F, EU3D_FIELD_W)[cc];
+        c.E = FLD(F, EU3D_FIELD_E)[cc];
+        c.T = FLD(F, EU3D_FIELD_T)[cc];
+        n = (int)FLD(F, EU3D_FIELD_NCHEM)[cc];
+    }
+    __device__ void store(unsigned id, const double (&cs)[NC], const Prim& o, const Rates& kr) const {
+        const size_t cc = id;
+        double* CS = FLD(F, EU3D_FIELD_CS);
+#pragma unroll
+        for (int s = 0; s < NC; s++) CS[s * g.n3 + cc] = cs[s];
+        store_prims(F, g, cc, cs, o);
+        double w[NS];
+        kin_wdot(kr, w);
+#pragma unroll
+        for (int s = 0; s < NS; s++) FLD(F, EU3D_FIELD_CMS)[s * g.n3 + cc] = w[s] * 1e3;
+    }
+};
+
+// Flat batch accessor: rho[s * n + id], T[id], E[id]; the cells are at rest.
+struct BatchCells {
+    long long n;
+    double *rho, *T, *E;
+    const int* nchem;
+    __device__ void load(unsigned id, ChemCell& c, int& nn) const {
+#pragma unroll
+        for (int s = 0; s < NS; s++) c.Di[s] = rho[(size_t)s * n + id];
+        c.U = c.V = c.W = 0.0;
+        c.E = E[id];
+        c.T = T[id];
+        nn = nchem[id];
+    }
+    __device__ void store(unsigned id, const double (&cs)[NC], const Prim& o, const Rates&) const {
+#pragma unroll
+        for (int s = 0; s < NS; s++) rho[(size_t)s * n + id] = cs[s];
+        T[id] = o.T;
+        E[id] = o.E;
+    }
+};
+
+__device__ __forceinline__ void chem_count(Scalars* sc, int nch, bool active, unsigned* list, unsigned id) {
+    // wave-aggregated compaction of the active cells and the sub-step statistics
+    const unsigned long long am = __ballot(active);
+    const int lane = __lane_id();
+    const int leader = __ffsll((unsigned long long)__ballot(1)) - 1;
+    unsigned long long base = 0;
+    int tot = nch, mx = nch;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        tot += __shfl_xor(tot, off);
+        const int o = __shfl_xor(mx, off);
+        mx = o > mx ? o : mx;
+    }
+    if (lane == leader) {
+        if (am) base = atomicAdd(&sc->chem_active, (unsigned long long)__popcll(am));
+        atomicAdd(&sc->chem_substeps, (unsigned long long)tot);
+        atomicMax(&sc->chem_nmax, (unsigned long long)mx);
+    }
+    base = __shfl(base, leader);
+    if (active) list[base + __popcll(am & ((1ull << lane) - 1ull))] = id;
+}
+
+// Prediction + first sub-step for every interior cell; cells needing more sub-steps are
+// compacted into the work list.
+__global__ void __launch_bounds__(256) k_trap_first(G3 g, Fields F, uint32_t q, const Scalars* __restrict__ scc,
+                                                    Scalars* sc, unsigned* list, int nmax, int predict_only) {
+    int i, j, k;
+    const bool in = interior_ijk(g, i, j, k);
+    const size_t c = in ? IDX(g, i, j, k) : 0;
+    int nch = 0;
+    bool active = false;
+    if (in) {
+        ChemCell cell;
+#pragma unroll
+        for (int s = 0; s < NS; s++) cell.Di[s] = FLD(F, EU3D_FIELD_DI)[s * g.n3 + c];
+        cell.U = FLD(F, EU3D_FIELD_U)[c];
+        cell.V = FLD(F, EU3D_FIELD_V)[c];
+        cell.W = FLD(F, EU3D_FIELD_W)[c];
+        cell.E = FLD(F, EU3D_FIELD_E)[c];
+        cell.T = FLD(F, EU3D_FIELD_T)[c];
+        double Mc[NS], Dr[NS], Y[NS];
+        react_units(F, g, c, Mc, Dr);
+#pragma unroll
+        for (int s = 0; s < NS; s++) Y[s] = FLD(F, EU3D_FIELD_YI)[s * g.n3 + c];
+        Rates kr;
+        kin_rates(cell.T, Mc, (q & Q_C2) ? 1 : 2, q & Q_C4, kr);
+        const double dtm = scc->dt;
+        nch = chem_nchem(kr, Dr, Y, dtm, nmax);
+        FLD(F, EU3D_FIELD_NCHEM)[c] = (double)nch;
+        if (nch <= 0) atomicOr(&sc->nonfinite, 2);
+        if (!predict_only && nch > 0) {
+            double cs[NC];
+            const Prim o = chem_apply(cell, kr, Mc, Dr, dtm / nch, q, cs);
+            GridCells acc{g, F};
+            acc.store((unsigned)c, cs, o, kr);
+            active = nch > 1;
+        }
+    }
+    chem_count(sc, in ? nch : 0, active, list, (unsigned)c);
+}
+
+// Persistent chemistry kernel: every lane pulls cells from the compacted list through a
+// wave-aggregated atomic queue and runs that cell's remaining sub-steps, taking a new cell as
+// soon as one finishes. All lanes execute the same sub-step code on different cells, so cells of
+// very different sub-step counts do not serialise a wavefront.
+template <class Acc>
+__global__ void __launch_bounds__(256) k_chem_persist(Acc acc, const unsigned* __restrict__ list, Scalars* sc,
+                                                      uint32_t q, const double* __restrict__ dtm_p) {
+    const unsigned long long n_active = sc->chem_active;
+    const double dtm = *dtm_p;
+    const int lane = __lane_id();
+    ChemCell cell;
+    int rem = 0, nch = 1;
+    unsigned id = 0;
+    bool have = false, drained = false;
+    while (true) {
+        const bool need = !have && !drained;
+        const unsigned long long nm = __ballot(need);
+        if (nm) {
+            const int leader = __ffsll(nm) - 1;
+            unsigned long long base = 0;
+            if (lane == leader) base = atomicAdd(&sc->chem_head, (unsigned long long)__popcll(nm));
+            base = __shfl(base, leader);
+            if (need) {
+                const unsigned long long slot = base + __popcll(nm & ((1ull << lane) - 1ull));
+                if (slot < n_active) {
+                    id = list[slot];
+                    acc.load(id, cell, nch);
+                    rem = nch - 1;   // the first sub-step ran in the compaction pass
+                    have = rem > 0;
+                } else {
+                    drained = true;
+                }
+            }
+        }
+        if (__ballot(have) == 0) {
+            if (__ballot(!drained) == 0) break;
+            continue;
+        }
+        if (have) {
+            double Mc[NS], Dr[NS], cs[NC];
+            chem_inputs(cell, Mc, Dr);
+            Rates kr;
+            kin_rates(cell.T, Mc, (q & Q_C2) ? 1 : 2, q & Q_C4, kr);
+            const Prim o = chem_apply(cell, kr, Mc, Dr, dtm / nch, q, cs);
+            if (--rem == 0) {
+                acc.store(id, cs, o, kr);
+                have = false;
+            }
+        }
+    }
+}
+
+// Batch: prediction + first sub-step on flat cells at rest.
+__global__ void __launch_bounds__(256) k_batch_first(BatchCells acc, int* nchem_out, int* nchem_keep, uint32_t q,
+                                                     double dtm, int nmax, Scalars* sc, unsigned* list) {
+    const long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    const bool in = id < acc.n;
+    int nch = 0;
+    bool active = false;
+    if (in) {
+        ChemCell cell;
+#pragma unroll
+        for (int s = 0; s < NS; s++) cell.Di[s] = acc.rho[(size_t)s * acc.n + id];
+        cell.U = cell.V = cell.W = 0.0;
+        cell.E = acc.E[id];
+        cell.T = acc.T[id];
+        double Mc[NS], Dr[NS], Y[NS], D = 0;
+        chem_inputs(cell, Mc, Dr);
+#pragma unroll
+        for (int s = 0; s < NS; s++) D += cell.Di[s];
+#pragma unroll
+        for (int s = 0; s < NS; s++) Y[s] = cell.Di[s] / D;
+        Rates kr;
+        kin_rates(cell.T, Mc, (q & Q_C2) ? 1 : 2, q & Q_C4, kr);
+        nch = chem_nchem(kr, Dr, Y, dtm, nmax);
+        nchem_keep[id] = nch;
+        if (nchem_out) nchem_out[id] = nch;
+        if (nch <= 0) atomicOr(&sc->nonfinite, 2);
+        if (nch > 0) {
+            double cs[NC];
+            const Prim o = chem_apply(cell, kr, Mc, Dr, dtm / nch, q, cs);
+            acc.store((unsigned)id, cs, o, kr);
+            active = nch > 1;
+        }
+    }
+    chem_count(sc, in ? nch : 0, active, list, (unsigned)id);
+}
+
+__global__ void k_chem_reset(Scalars* sc, double* dtm_slot, double dtm, int use_dt) {
+    sc->chem_head = 0;
+    sc->chem_active = 0;
+    sc->chem_substeps = 0;
+    sc->chem_nmax = 0;
+    *dtm_slot = use_dt ? sc->dt : dtm;
+}
+
+inline G3 G(const eu3d_ctx* c) { return c->g; }
+
+}  // namespace
+
+namespace eu3d {
+
+static int grid1d(size_t n, int blk) {
+    size_t b = (n + blk - 1) / blk;
+    if (b > 65536 * 8) b = 65536 * 8;
+    return (int)(b ? b : 1);
+}
+
+hipError_t launch_aos_to_soa(eu3d_ctx* c, const double* aos, double* soa, int nc) {
+    k_aos_to_soa<<<grid1d(c->g.n3 * nc, 256), 256, 0, c->stream>>>(G(c), aos, soa, nc);
+    return hipGetLastError();
+}
+
+hipError_t launch_soa_to_aos(eu3d_ctx* c, const double* soa, double* aos, int nc) {
+    k_soa_to_aos<<<grid1d(c->g.n3 * nc, 256), 256, 0, c->stream>>>(G(c), soa, aos, nc);
+    return hipGetLastError();
+}
+
+hipError_t launch_update_after_cs(eu3d_ctx* c) {
+    k_update_after_cs<<<interior_grid(c->g), dim3(BX, BY), 0, c->stream>>>(G(c), c->fl, c->quirks);
+    return hipGetLastError();
+}
+
+hipError_t launch_field_initial(eu3d_ctx* c) {
+    k_field_initial<<<(unsigned)((c->g.n3 + 255) / 256), 256, 0, c->stream>>>(G(c), c->fl, c->quirks);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_partial_t(c);
+}
+
+hipError_t launch_partial_t(eu3d_ctx* c) {
+    k_partial_t<<<interior_grid(c->g), dim3(BX, BY), 0, c->stream>>>(G(c), c->fl, c->quirks);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_ghosts(eu3d_ctx* c, const int kinds[6]) {
+    const G3& g = c->g;
+    k_fill_x<<<dim3((g.nj + 63) / 64, g.nk, 2 * g.bc), 64, 0, c->stream>>>(g, c->fl, kinds[0], kinds[1]);
+    k_fill_y<<<dim3((g.NX + 63) / 64, g.nk, 2 * g.bc), 64, 0, c->stream>>>(g, c->fl, kinds[2], kinds[3], c->quirks);
+    k_fill_z<<<dim3((g.NX + 63) / 64, g.NY, 2 * g.bc), 64, 0, c->stream>>>(g, c->fl, kinds[4], kinds[5]);
+    return hipGetLastError();
+}
+
+hipError_t launch_advection(eu3d_ctx* c, int diff, int limiter) {
+    const G3& g = c->g;
+    const int fid[3] = {EU3D_FIELD_F, EU3D_FIELD_G, EU3D_FIELD_Q};
+    for (int d = 0; d < 3; d++) {
+        const int nx = g.ni + (d == 0), ny = g.nj + (d == 1), nz = g.nk + (d == 2);
+        dim3 grid((nx + BX - 1) / BX, (ny + BY - 1) / BY, nz);
+        k_face_flux<<<grid, dim3(BX, BY), 0, c->stream>>>(g, c->fl, d, diff, limiter, c->quirks, c->fl.f[fid[d]]);
+    }
+    k_ee<<<interior_grid(g), dim3(BX, BY), 0, c->stream>>>(g, c->fl, c->xnode, c->ynode, c->znode, c->sc);
+    return hipGetLastError();
+}
+
+hipError_t launch_cfl(eu3d_ctx* c, double cfl, double final_time) {
+    k_cfl<<<interior_grid(c->g), dim3(BX, BY), 0, c->stream>>>(G(c), c->fl, c->dx0, c->dy0, c->dz0, c->quirks, c->sc);
+    k_cfl_finish<<<1, 1, 0, c->stream>>>(c->sc, cfl, final_time);
+    return hipGetLastError();
+}
+
+hipError_t launch_set_scalars(eu3d_ctx* c, double dt, double time) {
+    k_set_scalars<<<1, 1, 0, c->stream>>>(c->sc, dt, time);
+    return hipGetLastError();
+}
+
+hipError_t launch_update_after_adv(eu3d_ctx* c) {
+    k_add_rhs<<<interior_grid(c->g), dim3(BX, BY), 0, c->stream>>>(G(c), c->fl, (c->quirks & Q_F1) ? NS : NC);
+    return launch_update_after_cs(c);
+}
+
+hipError_t launch_explicit(eu3d_ctx* c) {
+    k_explicit<<<interior_grid(c->g), dim3(BX, BY), 0, c->stream>>>(G(c), c->fl, c->quirks);
+    return launch_update_after_cs(c);
+}
+
+hipError_t launch_diagonalized(eu3d_ctx* c) {
+    k_diagonalized<<<interior_grid(c->g), dim3(BX, BY), 0, c->stream>>>(G(c), c->fl, c->quirks);
+    return hipGetLastError();
+}
+
+hipError_t launch_update_imex(eu3d_ctx* c) {
+    k_update_imex<<<interior_grid(c->g), dim3(BX, BY), 0, c->stream>>>(G(c), c->fl, c->sc);
+    hipError_t e = launch_update_after_cs(c);
+    if (e != hipSuccess) return e;
+    return launch_partial_t(c);
+}
+
+hipError_t launch_trapezoid(eu3d_ctx* c) {
+    k_trapezoid<<<interior_grid(c->g), dim3(BX, BY), 0, c->stream>>>(G(c), c->fl, c->quirks, c->sc);
+    return hipGetLastError();
+}
+
+hipError_t launch_check_finite(eu3d_ctx* c) {
+    k_check_finite<<<interior_grid(c->g), dim3(BX, BY), 0, c->stream>>>(G(c), c->fl, c->sc);
+    return hipGetLastError();
+}
+
+hipError_t launch_trap_adaptive(eu3d_ctx* c, int nchem_max, bool predict_only) {
+    const int nmax = nchem_max > 0 ? nchem_max : 10000;
+    double* slot = &c->sc->dtm;
+    k_chem_reset<<<1, 1, 0, c->stream>>>(c->sc, slot, 0.0, 1);
+    k_trap_first<<<interior_grid(c->g), dim3(BX, BY), 0, c->stream>>>(G(c), c->fl, c->quirks, c->sc, c->sc,
+                                                                     c->active, nmax, predict_only ? 1 : 0);
+    if (!predict_only) {
+        GridCells acc{c->g, c->fl};
+        const int blk = c->sched.block;
+        const int nb = c->sched.blocks_per_cu * c->num_cu;
+        k_chem_persist<GridCells><<<nb, blk, 0, c->stream>>>(acc, c->active, c->sc, c->quirks, slot);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_chem_batch(eu3d_ctx* c, long long n, double* rho, double* T, double* E, double dtm, int nchem_max,
+                             int* nchem_out) {
+    const int nmax = nchem_max > 0 ? nchem_max : 10000;
+    if (c->batch_cap < n) {
+        if (c->batch_active) (void)hipFree(c->batch_active);
+        c->batch_active = nullptr;
+        c->batch_cap = 0;
+        hipError_t e = hipMalloc(&c->batch_active, (size_t)n * (sizeof(unsigned) + sizeof(int)));
+        if (e != hipSuccess) return e;
+        c->batch_cap = n;
+    }
+    unsigned* list = c->batch_active;
+    int* keep = reinterpret_cast<int*>(list + c->batch_cap);
+    double* slot = &c->sc->dtm;
+    k_chem_reset<<<1, 1, 0, c->stream>>>(c->sc, slot, dtm, 0);
+    BatchCells acc{n, rho, T, E, keep};
+    k_batch_first<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(acc, nchem_out, keep, c->quirks, dtm, nmax,
+                                                                       c->sc, list);
+    const int nb = c->sched.blocks_per_cu * c->num_cu;
+    k_chem_persist<BatchCells><<<nb, c->sched.block, 0, c->stream>>>(acc, list, c->sc, c->quirks, slot);
+    return hipGetLastError();
+}
+
+hipError_t launch_step(eu3d_ctx* c, const eu3d_step_params* p) {
+    hipError_t e;
+    if (p->cfl > 0) {
+        if ((e = launch_cfl(c, p->cfl, p->final_time)) != hipSuccess) return e;
+    } else {
+        k_advance_time<<<1, 1, 0, c->stream>>>(c->sc);
+    }
+    if ((e = launch_fill_ghosts(c, p->bc_kinds)) != hipSuccess) return e;
+    if (p->rk == 2) {
+        const size_t n = c->g.n3 * NC;
+        k_copy<<<grid1d(n, 256), 256, 0, c->stream>>>(c->fl.f[EU3D_FIELD_CS], c->rk_save, n);
+        if ((e = launch_advection(c, p->diff, p->limiter)) != hipSuccess) return e;
+        if ((e = launch_update_after_adv(c)) != hipSuccess) return e;
+        if ((e = launch_fill_ghosts(c, p->bc_kinds)) != hipSuccess) return e;
+        if ((e = launch_advection(c, p->diff, p->limiter)) != hipSuccess) return e;
+        k_rk2_combine<<<interior_grid(c->g), dim3(BX, BY), 0, c->stream>>>(G(c), c->fl, c->rk_save);
+        if ((e = launch_update_after_cs(c)) != hipSuccess) return e;
+        switch (p->chem) {
+        case EU3D_CHEM_TRAPEZOID:
+            if ((e = launch_trapezoid(c)) != hipSuccess) return e;
+            e = launch_explicit(c);
+            break;
+        case EU3D_CHEM_TRAP_ADAPTIVE: e = launch_trap_adaptive(c, p->nchem_max, false); break;
+        default: e = hipSuccess;
+        }
+        if (e != hipSuccess) return e;
+    } else {
+        if ((e = launch_advection(c, p->diff, p->limiter)) != hipSuccess) return e;
+        switch (p->chem) {
+        case EU3D_CHEM_IMEX:
+            if ((e = launch_diagonalized(c)) != hipSuccess) return e;
+            e = launch_update_imex(c);
+            break;
+        case EU3D_CHEM_TRAPEZOID:
+            if ((e = launch_update_after_adv(c)) != hipSuccess) return e;
+            if ((e = launch_trapezoid(c)) != hipSuccess) return e;
+            e = launch_explicit(c);
+            break;
+        case EU3D_CHEM_TRAP_ADAPTIVE:
+            if ((e = launch_update_after_adv(c)) != hipSuccess) return e;
+            e = launch_trap_adaptive(c, p->nchem_max, false);
+            break;
+        default: e = launch_update_after_adv(c);
+        }
+        if (e != hipSuccess) return e;
+    }
+    return launch_check_finite(c);
+}
+
+}  // namespace eu3d

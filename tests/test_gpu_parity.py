@@ -1,0 +1,138 @@
+"""GPU parity against the CPU oracle through the C ABI, per reference method and over steps."""
+import numpy as np
+import pytest
+
+from tests.common import compare, make_pair, say, small_bubble, small_det, small_zn
+
+pytestmark = pytest.mark.gpu
+
+PRIMS = ["D", "T", "P", "U", "V", "W", "E", "H", "GAMMA", "C", "YI", "CS"]
+Q_REF, Q_FIXED = 0x7FF, 0
+
+
+@pytest.mark.parametrize("quirks", [Q_FIXED, Q_REF])
+def test_field_initial(quirks):
+    say("case field_initial quirks", quirks)
+    g, o = make_pair(small_bubble(), quirks)
+    compare(g, o, ["CS", "D", "E", "H", "GAMMA", "C", "MC", "DI", "PT"])
+
+
+@pytest.mark.parametrize("quirks", [Q_FIXED, Q_REF])
+def test_update_after_cs(quirks):
+    say("case update_after_cs quirks", quirks)
+    g, o = make_pair(small_bubble(), quirks)
+    # perturb the conserved state so the Newton has work to do
+    cs = o.get("CS")
+    rng = np.random.default_rng(3)
+    cs[..., -1] *= 1.0 + 0.05 * rng.uniform(-1, 1, cs.shape[:-1])
+    for obj in (g, o):
+        obj.set("CS", cs)
+        obj.update_after_cs()
+    compare(g, o, PRIMS + ["MA", "RGAS", "CP", "WAV", "MC", "DI"])
+
+
+@pytest.mark.parametrize("quirks", [Q_FIXED, Q_REF])
+@pytest.mark.parametrize("diff,limiter", [(1, 0), (0, 0), (1, 1), (1, 2), (1, 3), (1, 4)])
+def test_advection(quirks, diff, limiter):
+    say("case advection quirks", quirks, "diff", diff, "limiter", limiter)
+    g, o = make_pair(small_bubble(), quirks)
+    for obj in (g, o):
+        obj.set_time(2e-8, 0.0)
+        obj.fill_ghosts((0, 0, 0, 0, 0, 0) if quirks else (1, 1, 2, 2, 2, 2))
+        obj.advection(diff, limiter)
+    compare(g, o, ["RHS"])
+
+
+@pytest.mark.parametrize("kinds", [(0,) * 6, (1, 1, 2, 2, 2, 2), (2, 1, 3, 3, 3, 3)])
+def test_fill_ghosts(kinds):
+    say("case fill_ghosts", kinds)
+    q = Q_REF if kinds[0] == 0 else Q_FIXED
+    g, o = make_pair(small_bubble(), q)
+    for obj in (g, o):
+        obj.fill_ghosts(kinds)
+    whole = lambda x: x
+    compare(g, o, ["U", "V", "W", "P", "D", "T", "H", "GAMMA", "E", "YI"], region=whole)
+
+
+@pytest.mark.parametrize("quirks", [Q_FIXED, Q_REF])
+def test_cfl(quirks):
+    say("case cfl quirks", quirks)
+    g, o = make_pair(small_bubble(), quirks)
+    for obj in (g, o):
+        obj.cfl(0.5, 1.0)
+    (dg, tg), (do, to) = g.get_time(), o.get_time()
+    assert abs(dg - do) <= 1e-14 * do and abs(tg - to) <= 1e-14 * to
+
+
+@pytest.mark.parametrize("quirks", [Q_FIXED, Q_REF])
+def test_imex_kernels(quirks):
+    say("case imex quirks", quirks)
+    g, o = make_pair(small_det(), quirks)
+    for obj in (g, o):
+        obj.set_time(1e-8, 0.0)
+        obj.fill_ghosts((2, 1, 3, 3, 3, 3))
+        obj.advection(1, 0)
+        obj.diagonalized()
+    compare(g, o, ["CMS", "MD"])
+    for obj in (g, o):
+        obj.update_imex()
+    compare(g, o, PRIMS + ["PT"])
+
+
+@pytest.mark.parametrize("quirks", [Q_FIXED, Q_REF])
+def test_trapezoid_full(quirks):
+    say("case trapezoid quirks", quirks)
+    g, o = make_pair(small_det(), quirks)
+    for obj in (g, o):
+        obj.set_time(5e-9, 0.0)
+        obj.trapezoid()
+    compare(g, o, ["DI", "CMS"])
+    for obj in (g, o):
+        obj.explicit()
+    compare(g, o, ["CS", "D", "T", "E"] if quirks == Q_FIXED else ["CS", "D"])
+
+
+@pytest.mark.parametrize("quirks", [Q_FIXED, Q_REF])
+def test_trapezoid_adaptive(quirks):
+    say("case trapezoid_adaptive quirks", quirks)
+    g, o = make_pair(small_det(), quirks)
+    for obj in (g, o):
+        obj.set_time(2e-8, 0.0)
+        obj.trapezoid_adaptive(200)
+    nch_g, nch_o = g.get("NCHEM"), o.get("NCHEM")
+    assert (nch_g[2:-2, 2:-2, 2:-2] == nch_o[2:-2, 2:-2, 2:-2]).all()
+    if quirks == Q_REF:
+        assert (nch_o[2:-2, 2:-2, 2:-2] == 1).all()
+    compare(g, o, PRIMS + ["CMS"])
+
+
+@pytest.mark.parametrize("chem", [1, 2, 3])
+def test_steps_fixed(chem):
+    say("case steps chem", chem)
+    wl = small_det()
+    g, o = make_pair(wl, Q_FIXED)
+    kw = dict(nsteps=4, cfl=0.5, bc_kinds=wl.bc_kinds, rk=1, chem=chem, nchem_max=200)
+    g.step(**kw)
+    o.step(**kw)
+    assert g.check_finite() == 0
+    compare(g, o, ["CS", "T", "D", "P", "E", "YI"])
+
+
+def test_steps_rk2_zn1d():
+    say("case rk2 zn1d")
+    wl = small_zn()
+    g, o = make_pair(wl, Q_FIXED)
+    kw = dict(nsteps=5, cfl=0.5, bc_kinds=wl.bc_kinds, rk=2, chem=3, nchem_max=200)
+    g.step(**kw)
+    o.step(**kw)
+    compare(g, o, ["CS", "T", "D", "P", "E", "YI"])
+
+
+def test_steps_imex_bubble_ref():
+    say("case imex bubble REF")
+    wl = small_bubble()
+    g, o = make_pair(wl, Q_REF)
+    kw = dict(nsteps=2, cfl=0.1, bc_kinds=(0,) * 6, rk=1, chem=1)
+    g.step(**kw)
+    o.step(**kw)
+    compare(g, o, ["CS", "T", "D", "P", "E", "YI"])
