@@ -34,7 +34,7 @@ FIELDS = {"CS": 0, "T": 1, "E": 2, "U": 3, "V": 4, "W": 5, "P": 6, "D": 7, "H": 
 EXPORTED = [
     "eu3d_create", "eu3d_destroy", "eu3d_last_error", "eu3d_sync", "eu3d_set_quirks",
     "eu3d_field_components", "eu3d_set_field", "eu3d_get_field", "eu3d_set_time", "eu3d_get_time",
-    "eu3d_field_initial", "eu3d_cfl_dt", "eu3d_fill_ghosts", "eu3d_advection",
+    "eu3d_field_initial", "eu3d_cfl_dt", "eu3d_fill_ghosts", "eu3d_advection", "eu3d_advection_fused",
     "eu3d_update_after_adv", "eu3d_update_after_cs", "eu3d_get_partial_t", "eu3d_explicit",
     "eu3d_diagonalized", "eu3d_update_imex", "eu3d_trapezoid", "eu3d_trapezoid_prediction",
     "eu3d_trapezoid_adaptive", "eu3d_check_finite", "eu3d_step", "eu3d_get_chem_stats",
@@ -108,6 +108,7 @@ def load_library(path: str = LIB_PATH):
     lib.eu3d_cfl_dt.argtypes = [vp, C.c_double, C.c_double]
     lib.eu3d_fill_ghosts.argtypes = [vp, ip]
     lib.eu3d_advection.argtypes = [vp, C.c_int, C.c_int]
+    lib.eu3d_advection_fused.argtypes = [vp, C.c_int, C.c_int]
     lib.eu3d_trapezoid_prediction.argtypes = [vp, C.c_int]
     lib.eu3d_trapezoid_adaptive.argtypes = [vp, C.c_int]
     lib.eu3d_step.argtypes = [vp, C.POINTER(StepParams), C.c_int]
@@ -219,6 +220,9 @@ class Context:
 
     def advection(self, diff: int = 1, limiter: int = 0):
         self._chk(self.lib.eu3d_advection(self.h, diff, limiter), "advection")
+
+    def advection_fused(self, diff: int = 1, limiter: int = 0):
+        self._chk(self.lib.eu3d_advection_fused(self.h, diff, limiter), "advection_fused")
 
     def update_after_adv(self):
         self._chk(self.lib.eu3d_update_after_adv(self.h), "update_after_adv")

@@ -104,6 +104,7 @@ int eu3d_create(eu3d_ctx** out, const eu3d_grid* grid, const eu3d_mech* mech, ui
     if (hipMalloc(&c->xnode, nx * sizeof(double)) != hipSuccess || hipMalloc(&c->ynode, ny * sizeof(double)) != hipSuccess ||
         hipMalloc(&c->znode, nz * sizeof(double)) != hipSuccess || hipMalloc(&c->sc, sizeof(Scalars)) != hipSuccess ||
         hipMalloc(&c->active, g.n3 * sizeof(unsigned)) != hipSuccess ||
+        hipMalloc(&c->part, (g.n3 / 64 + 64) * sizeof(double)) != hipSuccess ||
         hipMalloc(&c->rk_save, g.n3 * NC * sizeof(double)) != hipSuccess) {
         eu3d_destroy(c);
         return EU3D_ERR_HIP;
@@ -128,6 +129,7 @@ int eu3d_destroy(eu3d_ctx* c) {
     if (c->znode) (void)hipFree(c->znode);
     if (c->sc) (void)hipFree(c->sc);
     if (c->active) (void)hipFree(c->active);
+    if (c->part) (void)hipFree(c->part);
     if (c->rk_save) (void)hipFree(c->rk_save);
     if (c->batch_active) (void)hipFree(c->batch_active);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -208,6 +210,12 @@ int eu3d_fill_ghosts(eu3d_ctx* c, const int kinds[6]) {
 int eu3d_advection(eu3d_ctx* c, int diff, int limiter) {
     if (!c || diff < 0 || diff > 1 || limiter < 0 || limiter > 4) return fail(c, EU3D_ERR_ARG, "bad scheme");
     HIPCHK(c, launch_advection(c, diff, limiter), "advection");
+    return EU3D_OK;
+}
+
+int eu3d_advection_fused(eu3d_ctx* c, int diff, int limiter) {
+    if (!c || diff < 0 || diff > 1 || limiter < 0 || limiter > 4) return fail(c, EU3D_ERR_ARG, "bad scheme");
+    HIPCHK(c, launch_flux_fused(c, diff, limiter), "advection_fused");
     return EU3D_OK;
 }
 
@@ -299,7 +307,11 @@ int eu3d_step(eu3d_ctx* c, const eu3d_step_params* p, int nsteps) {
     if (p->rk != 1 && p->rk != 2) return fail(c, EU3D_ERR_ARG, "rk must be 1 or 2");
     if (p->rk == 2 && p->chem == EU3D_CHEM_IMEX) return fail(c, EU3D_ERR_UNSUPPORTED, "IMEX couples to the EE stage only");
     if (p->diff < 0 || p->diff > 1 || p->limiter < 0 || p->limiter > 4) return fail(c, EU3D_ERR_ARG, "bad scheme");
-    for (int n = 0; n < nsteps; n++) HIPCHK(c, launch_step(c, p), "step");
+    for (int n = 0; n < nsteps; n++) HIPCHK(c, launch_step(c, p, n == 0), "step");
+    if (nsteps > 0) {
+        HIPCHK(c, launch_derive(c), "step derive");
+        HIPCHK(c, launch_check_finite(c), "step finite");
+    }
     return EU3D_OK;
 }
 

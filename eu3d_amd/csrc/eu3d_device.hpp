@@ -290,38 +290,63 @@ __device__ __forceinline__ void ausm_flux(int dir, const Face& L, const Face& R,
 }
 
 // ------------------------------------------------------------------ kinetics
-// Rates of one cell (the common part of Reaction::Trapezoid, TrapezoidPrediction and
-// Diagonalized; src/Reaction.cpp:420-452,499-531,744-776). Inputs in reaction units:
-// Di [g/cm^3], Mc [mol/cm^3]. rtb: 0 keeps the summed third-body factor (quirk C1),
-// 1 forces it to 0 (quirk C2), 2 uses 1 where the sum is 0 (the correct default).
-struct Rates {
-    double KF[NR], KB[NR], RRF[NR], RRB[NR], RTB[NR], RR[NR];
+// One pass over the reactions of one cell (the common part of Reaction::Trapezoid,
+// TrapezoidPrediction and Diagonalized; src/Reaction.cpp:420-482,499-588,744-809), accumulating
+// per species as each reaction is evaluated, in reaction order as the reference sums:
+//   w  = Mw sum_r (nu''-nu') R_TB RR                     [g/cm^3/s]   (all modes)
+//   Q, P: production and destruction of the point-implicit trapezoid          (TRAP)
+//   md: diagonal of the chemical Jacobian                                      (DIAG)
+// Inputs in reaction units: Mc [mol/cm^3], Dr = Di [g/cm^3]. rtb: 0 keeps the summed
+// third-body factor (quirk C1), 1 forces it to 0 (quirk C2), 2 uses 1 where the sum is 0.
+// k_b is formed without the reference's two-step K_p, K_c division:
+//   k_b = k_f / K_c = k_f exp(sum_s nu_s G_s / RT) (RT / (P0 1e-6))^dnu,
+// with G_s / RT = h_s/(RT) - s_s/R from the NASA fits; the exponents of k_f (b ln T - E/RuT)
+// are folded into one exp. Zero stoichiometry is skipped at compile time.
+struct KinOut {
+    double w[NS], P[NS], Q[NS], md[NS];
 };
 
-__device__ __forceinline__ void kin_rates(double T, const double (&Mc)[NS], int rtb, bool c4, Rates& k) {
+template <bool TRAP, bool DIAG, bool C3 = false>
+__device__ __forceinline__ void kinetics(double T, const double (&Mc)[NS], const double (&Dr)[NS],
+                                         const double (&pts)[NS], int rtb, bool c4, KinOut& o) {
     const TPow p = tpow(T);
     const bool lo = T < 1000;
-    double G[NS];
+    double gx[NS];   // G_s / (R T)
 #pragma unroll
-    for (int s = 0; s < NS; s++) G[s] = R_UNIV * (h_R(s, lo, c4, p) - s_R(s, lo, c4, p) * T);
+    for (int s = 0; s < NS; s++) {
+        gx[s] = h_R(s, lo, c4, p) * p.Tm1 - s_R(s, lo, c4, p);
+        o.w[s] = 0.0;
+        if (TRAP) { o.P[s] = 0.0; o.Q[s] = 0.0; }
+        if (DIAG) o.md[s] = 0.0;
+    }
+    double rDr[NS];
+    if (DIAG) {
+#pragma unroll
+        for (int s = 0; s < NS; s++) rDr[s] = 1.0 / Dr[s];
+    }
     const double RT = R_UNIV * T;
-    const double rRT = 1.0 / RT;
+    constexpr double P0R = P0_PA * 1e-6;
+    const double rt_p0 = RT / P0R, p0_rt = P0R / RT;
     const double rRuT = 1.0 / (RU_CAL * T);
+    const double rRuT2 = rRuT * p.Tm1;
 #pragma unroll
     for (int r = 0; r < NR; r++) {
-        // k_f = A T^b exp(-E / (Ru T))
+        // k_f = A T^b exp(-E/(Ru T))
         double kf = AF[r];
+        double ex = 0.0;
+        bool use_exp = false;
         if (BF[r] == 1.0) kf *= T;
         else if (BF[r] == -1.0) kf *= p.Tm1;
         else if (BF[r] == -2.0) kf *= p.Tm2;
-        else if (BF[r] != 0.0) kf *= exp(BF[r] * p.lnT);
-        if (EAF[r] != 0.0) kf *= exp(-EAF[r] * rRuT);
-        double rf = 1.0, rb = 1.0, tb = 0.0, dg = 0.0;
+        else if (BF[r] != 0.0) { ex = BF[r] * p.lnT; use_exp = true; }
+        if (EAF[r] != 0.0) { ex -= EAF[r] * rRuT; use_exp = true; }
+        if (use_exp) kf *= exp(ex);
+        double xk = 0.0, rf = 1.0, rb = 1.0, tb = 0.0;
         int xr = 0;
 #pragma unroll
         for (int s = 0; s < NS; s++) {
             const int nf = SF[s][r], nb = SB[s][r];
-            if (nb - nf != 0) dg += (double)(nb - nf) * G[s];
+            if (nb - nf != 0) xk += (double)(nb - nf) * gx[s];
             xr += nb - nf;
             if (nf == 1) rf *= Mc[s];
             else if (nf == 2) rf *= Mc[s] * Mc[s];
@@ -329,97 +354,62 @@ __device__ __forceinline__ void kin_rates(double T, const double (&Mc)[NS], int 
             else if (nb == 2) rb *= Mc[s] * Mc[s];
             if (TB[s][r] != 0.0) tb += Mc[s] * TB[s][r];
         }
-        // K_p = exp(-dg/RT) (P0 1e-6)^xr, K_c = K_p (R T)^-xr, k_b = k_f / K_c
-        double kp = exp(-dg * rRT);
-        if (xr == 1) kp *= (P0_PA * 1e-6);
-        else if (xr == -1) kp *= 1.0 / (P0_PA * 1e-6);
-        double kc = kp;
-        if (xr == 1) kc *= rRT;
-        else if (xr == -1) kc *= RT;
-        const double kb = kf / kc;
+        double kb = kf * exp(xk);
+        if (xr == 1) kb *= rt_p0;
+        else if (xr == -1) kb *= p0_rt;
         if (rtb == 1) tb = 0.0;
         else if (rtb == 2 && tb == 0.0) tb = 1.0;
-        k.KF[r] = kf;
-        k.KB[r] = kb;
-        k.RRF[r] = rf;
-        k.RRB[r] = rb;
-        k.RTB[r] = tb;
-        k.RR[r] = kf * rf - kb * rb;
-    }
-}
-
-// Net mass production w_s = Mw_s sum_r (nu''-nu') R_TB RR [g/cm^3/s].
-__device__ __forceinline__ void kin_wdot(const Rates& k, double (&w)[NS]) {
+        const double rr = kf * rf - kb * rb;
+        const double dRdT = DIAG ? rr * (BF[r] * p.Tm1 + EAF[r] * rRuT2) : 0.0;
 #pragma unroll
-    for (int s = 0; s < NS; s++) {
-        double t = 0.0;
-#pragma unroll
-        for (int r = 0; r < NR; r++)
-            if (SB[s][r] - SF[s][r] != 0) t += (double)(SB[s][r] - SF[s][r]) * k.RTB[r] * k.RR[r];
-        w[s] = MW[s] * t;
-    }
-}
-
-// Point-implicit trapezoid (src/Reaction.cpp:464-482, 569-588): Di' [kg/m^3] from Di [g/cm^3].
-// Quirk C3 applies R_TB to the second term of Q and of P only.
-__device__ __forceinline__ void kin_trap(const Rates& k, const double (&Di)[NS], const double (&Mc)[NS], double dt,
-                                         bool c3, double (&Dout)[NS]) {
-#pragma unroll
-    for (int s = 0; s < NS; s++) {
-        double P = 0.0, Q = 0.0;
-#pragma unroll
-        for (int r = 0; r < NR; r++) {
-            const int nf = SF[s][r], nb = SB[s][r];
-            if (nf == 0 && nb == 0) continue;
-            const double qf = nb ? (double)nb * k.KF[r] * k.RRF[r] : 0.0;
-            const double qb = nf ? (double)nf * k.KB[r] * k.RRB[r] : 0.0;
-            const double pb = nb ? (double)nb * k.KB[r] * k.RRB[r] : 0.0;
-            const double pf = nf ? (double)nf * k.KF[r] * k.RRF[r] : 0.0;
-            if (c3) {
-                Q += qf + qb * k.RTB[r];
-                P += pb + pf * k.RTB[r];
-            } else {
-                Q += (qf + qb) * k.RTB[r];
-                P += (pb + pf) * k.RTB[r];
+        for (int s = 0; s < NS; s++) {
+            const int nf = SF[s][r], nb = SB[s][r], dn = nb - nf;
+            if (dn != 0) o.w[s] += (double)dn * tb * rr;
+            if (TRAP && (nf != 0 || nb != 0)) {
+                const double qf = nb ? (double)nb * kf * rf : 0.0;
+                const double qb = nf ? (double)nf * kb * rb : 0.0;
+                const double pb = nb ? (double)nb * kb * rb : 0.0;
+                const double pf = nf ? (double)nf * kf * rf : 0.0;
+                if (C3) {
+                    o.Q[s] += qf + qb * tb;
+                    o.P[s] += pb + pf * tb;
+                } else {
+                    o.Q[s] += (qf + qb) * tb;
+                    o.P[s] += (pb + pf) * tb;
+                }
+            }
+            if (DIAG && dn != 0) {
+                double w1 = 0.0;
+                if (Dr[s] != 0) {
+                    const double a = nf ? (kf * (double)nf * rf) * rDr[s] : 0.0;
+                    const double b = nb ? (kb * (double)nb * rb) * rDr[s] : 0.0;
+                    w1 = a - b;
+                }
+                double term = (w1 + dRdT * pts[s]) * tb;
+                if (TB[s][r] != 0.0) term += rr * (TB[s][r] / MW[s]);
+                o.md[s] += (double)dn * term;
             }
         }
-        Q = MW[s] * Q;
-        P = (Mc[s] == 0) ? 0.0 : P / Mc[s];
-        Dout[s] = ((1.0 - dt / 2.0 * P) * Di[s] + dt * Q) / (1.0 + dt / 2.0 * P) * 1e3;
+        // keep each reaction's temporaries local: without this the scheduler hoists every
+        // reaction's exp() ahead and the accumulators spill
+        __builtin_amdgcn_sched_barrier(0);
+    }
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        o.w[s] *= MW[s];
+        if (TRAP) {
+            o.Q[s] *= MW[s];
+            o.P[s] = (Mc[s] == 0) ? 0.0 : o.P[s] / Mc[s];
+        }
+        if (DIAG) o.md[s] *= MW[s];
     }
 }
 
-// Diagonal of the chemical Jacobian (src/Reaction.cpp:789-809). pt: dT/drho_s as stored by
-// GetPartial_T; quirk C6 uses it unscaled, FIXED multiplies the temperature path by 1e3.
-__device__ __forceinline__ void kin_diag(const Rates& k, double T, const double (&Di)[NS], const double (&pt)[NS],
-                                         bool c6, double (&md)[NS]) {
-    const double rRuT2 = 1.0 / (RU_CAL * (T * T));
-    double dRdT[NR];
+// Point-implicit trapezoid update (src/Reaction.cpp:481,587): Di' [kg/m^3] from Di [g/cm^3].
+__device__ __forceinline__ void trap_update(const KinOut& k, const double (&Dr)[NS], double dt, double (&Dout)[NS]) {
 #pragma unroll
-    for (int r = 0; r < NR; r++) dRdT[r] = k.RR[r] * (BF[r] / T + EAF[r] * rRuT2);
-    const double tsc = c6 ? 1.0 : 1e3;
-#pragma unroll
-    for (int s = 0; s < NS; s++) {
-        const double pts = pt[s] * tsc;
-        double t = 0.0;
-        const bool dz = (Di[s] == 0);
-#pragma unroll
-        for (int r = 0; r < NR; r++) {
-            const int nf = SF[s][r], nb = SB[s][r];
-            const int dn = nb - nf;
-            if (dn == 0) continue;
-            double w1 = 0.0;
-            if (!dz) {
-                double a = nf ? k.KF[r] * (double)nf * k.RRF[r] / Di[s] : 0.0;
-                double b = nb ? k.KB[r] * (double)nb * k.RRB[r] / Di[s] : 0.0;
-                w1 = a - b;
-            }
-            double term = (w1 + dRdT[r] * pts) * k.RTB[r];
-            if (TB[s][r] != 0.0) term += k.RR[r] * TB[s][r] / MW[s];
-            t += (double)dn * term;
-        }
-        md[s] = MW[s] * t;
-    }
+    for (int s = 0; s < NS; s++)
+        Dout[s] = ((1.0 - dt / 2.0 * k.P[s]) * Dr[s] + dt * k.Q[s]) / (1.0 + dt / 2.0 * k.P[s]) * 1e3;
 }
 
 }  // namespace eu3d

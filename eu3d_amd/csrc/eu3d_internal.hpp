@@ -23,7 +23,6 @@ struct Fields {
 // Device-resident scalars of a context.
 struct Scalars {
     double dt, time, dtm;            // dtm: chemistry interval of the current adaptive call
-    unsigned long long dtmin_bits;   // min over cells of the CFL bound, as ordered bits
     int nonfinite;
     int pad;
     unsigned long long chem_head;    // persistent-kernel work queue head
@@ -45,6 +44,7 @@ struct eu3d_ctx {
     double dx0 = 0, dy0 = 0, dz0 = 0;                              // node(1) - node(0)
     eu3d::Scalars* sc = nullptr;                                  // device
     unsigned int* active = nullptr;                                // compaction list (interior cells)
+    double* part = nullptr;                                        // per-block CFL partial minima
     double* rk_save = nullptr;                                     // RK2 copy of CS, T, E
     // batched chemistry scratch
     unsigned int* batch_active = nullptr;
@@ -64,6 +64,10 @@ hipError_t launch_field_initial(eu3d_ctx* c);
 hipError_t launch_fill_ghosts(eu3d_ctx* c, const int kinds[6]);
 hipError_t launch_advection(eu3d_ctx* c, int diff, int limiter);
 hipError_t launch_cfl(eu3d_ctx* c, double cfl, double final_time);
+hipError_t launch_cfl_finish(eu3d_ctx* c, double cfl, double final_time);
+hipError_t launch_flux_fused(eu3d_ctx* c, int diff, int limiter);
+hipError_t launch_imex_fused(eu3d_ctx* c);
+hipError_t launch_derive(eu3d_ctx* c);
 hipError_t launch_update_after_adv(eu3d_ctx* c);
 hipError_t launch_explicit(eu3d_ctx* c);
 hipError_t launch_diagonalized(eu3d_ctx* c);
@@ -72,7 +76,7 @@ hipError_t launch_trapezoid(eu3d_ctx* c);
 hipError_t launch_trap_adaptive(eu3d_ctx* c, int nchem_max, bool predict_only);
 hipError_t launch_check_finite(eu3d_ctx* c);
 hipError_t launch_set_scalars(eu3d_ctx* c, double dt, double time);
-hipError_t launch_step(eu3d_ctx* c, const eu3d_step_params* p);
+hipError_t launch_step(eu3d_ctx* c, const eu3d_step_params* p, bool first);
 hipError_t launch_chem_batch(eu3d_ctx* c, long long n, double* rho, double* T, double* E, double dtm,
                              int nchem_max, int* nchem_out);
 }  // namespace eu3d

@@ -250,7 +250,8 @@ __global__ void k_fill_z(G3 g, Fields F, int klo, int khi) {
 // One face per thread: MUSCL on the 16 primitives from the 4-cell stencil, then the AUSM flux
 // (Flowfield::AUSM, src/Flowfield.cpp:1330-1502). Faces along dir cover [bc-1, n+bc); the other
 // two indices cover the interior, which is every face TimeAdv::EE reads.
-__global__ void __launch_bounds__(256) k_face_flux(G3 g, Fields F, int dir, int order, int limiter, uint32_t q,
+template <int dir>
+__global__ void __launch_bounds__(256) k_face_flux(G3 g, Fields F, int order, int limiter, uint32_t q,
                                                    double* __restrict__ Fo) {
     const int i = (dir == 0 ? g.bc - 1 : g.bc) + (int)(blockIdx.x * BX + threadIdx.x);
     const int j = (dir == 1 ? g.bc - 1 : g.bc) + (int)(blockIdx.y * BY + threadIdx.y);
@@ -299,51 +300,75 @@ __global__ void __launch_bounds__(256) k_ee(G3 g, Fields F, const double* __rest
 }
 
 // ---------------------------------------------------------------- CFL
-// Flowfield::CFLcondition, src/Flowfield.cpp:242-272: per-block min, then one 64-bit atomic min
-// on the ordered bit pattern (positive doubles order as unsigned integers).
-__global__ void __launch_bounds__(256) k_cfl(G3 g, Fields F, double dx, double dy, double dz, uint32_t q,
-                                             Scalars* sc) {
-    int i, j, k;
-    double v = 1e10;
-    if (interior_ijk(g, i, j, k)) {
-        const size_t c = IDX(g, i, j, k);
-        const double C = FLD(F, EU3D_FIELD_C)[c];
-        const double a = dx / (fabs(FLD(F, EU3D_FIELD_U)[c]) + C), b = dy / (fabs(FLD(F, EU3D_FIELD_V)[c]) + C);
-        v = (b < a) ? b : a;
-        if (!(q & Q_F6)) {
-            const double z = dz / (fabs(FLD(F, EU3D_FIELD_W)[c]) + C);
-            v = (z < v) ? z : v;
-        }
-        if (!(v < 1e10)) v = 1e10;
+// Flowfield::CFLcondition, src/Flowfield.cpp:242-272: each block reduces its cells' bound and
+// writes one partial; k_cfl_finish reduces the partials in one block (no contended atomics).
+__device__ __forceinline__ double cfl_bound(double dx, double dy, double dz, double U, double V, double W, double C,
+                                            uint32_t q) {
+    const double a = dx / (fabs(U) + C), b = dy / (fabs(V) + C);
+    double v = (b < a) ? b : a;
+    if (!(q & Q_F6)) {
+        const double z = dz / (fabs(W) + C);
+        v = (z < v) ? z : v;
     }
+    return (v < 1e10) ? v : 1e10;
+}
+
+// block-wide min of v (blockDim.x * blockDim.y threads, multiple of 64); thread 0 writes it
+__device__ __forceinline__ void block_min_store(double v, double* out) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         const double o = __shfl_xor(v, off);
         v = (o < v) ? o : v;
     }
-    __shared__ double red[4];
-    const int t = threadIdx.y * BX + threadIdx.x;
+    __shared__ double red[16];
+    const int t = threadIdx.y * blockDim.x + threadIdx.x;
+    const int nw = (blockDim.x * blockDim.y) >> 6;
     if ((t & 63) == 0) red[t >> 6] = v;
     __syncthreads();
     if (t == 0) {
         double m = red[0];
-        for (int w = 1; w < 4; w++) m = (red[w] < m) ? red[w] : m;
-        atomicMin(&sc->dtmin_bits, (unsigned long long)__double_as_longlong(m));
+        for (int w = 1; w < nw; w++) m = (red[w] < m) ? red[w] : m;
+        out[(blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = m;
     }
 }
 
-__global__ void k_cfl_finish(Scalars* sc, double cfl, double final_time) {
-    double dt = cfl * __longlong_as_double((long long)sc->dtmin_bits);
-    if (sc->time + dt > final_time) dt = final_time - sc->time;
-    sc->dt = dt;
-    sc->time += dt;
-    sc->dtmin_bits = (unsigned long long)__double_as_longlong(1e10);
+__global__ void __launch_bounds__(256) k_cfl(G3 g, Fields F, double dx, double dy, double dz, uint32_t q,
+                                             double* part) {
+    int i, j, k;
+    double v = 1e10;
+    if (interior_ijk(g, i, j, k)) {
+        const size_t c = IDX(g, i, j, k);
+        v = cfl_bound(dx, dy, dz, FLD(F, EU3D_FIELD_U)[c], FLD(F, EU3D_FIELD_V)[c], FLD(F, EU3D_FIELD_W)[c],
+                      FLD(F, EU3D_FIELD_C)[c], q);
+    }
+    block_min_store(v, part);
+}
+
+__global__ void __launch_bounds__(1024) k_cfl_finish(Scalars* sc, const double* __restrict__ part, int n, double cfl,
+                                                     double final_time) {
+    double v = 1e10;
+    for (int t = threadIdx.x; t < n; t += blockDim.x) v = (part[t] < v) ? part[t] : v;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(v, off);
+        v = (o < v) ? o : v;
+    }
+    __shared__ double red[16];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = red[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = (red[w] < m) ? red[w] : m;
+        double dt = cfl * m;
+        if (sc->time + dt > final_time) dt = final_time - sc->time;
+        sc->dt = dt;
+        sc->time += dt;
+    }
 }
 
 __global__ void k_set_scalars(Scalars* sc, double dt, double time) {
     sc->dt = dt;
     sc->time = time;
-    sc->dtmin_bits = (unsigned long long)__double_as_longlong(1e10);
     sc->nonfinite = 0;
 }
 
@@ -387,6 +412,207 @@ __global__ void __launch_bounds__(256) k_explicit(G3 g, Fields F, uint32_t q) {
     }
 }
 
+// ---------------------------------------------------------------- fused flux
+// Flowfield::Advection + TimeAdv::EE in one pass (src/Flowfield.cpp:1096-1128,
+// src/TimeAdv.cpp:29-43). A block of 64 x 8 threads owns a 64 x 8 column of cells and marches
+// through a chunk of FZ planes in z. Per plane every thread computes the x face on its left and
+// the y face below it and shares them through LDS, so each face is evaluated once; the z face
+// above a cell is computed by the cell's own thread and the one below is carried in registers
+// from the previous plane. The first 63 x 7 cells of the column are written (the last lane and
+// row only supply faces). RHS is accumulated in EE's order.
+constexpr int FX = 64, FY = 8, FZ = 16;
+
+template <int dir>
+__device__ __forceinline__ void face_flux_at(const G3& g, const Fields& F, size_t c0, int order, int limiter,
+                                             bool zero_y, double (&fl)[NC]) {
+    const size_t st = dir == 0 ? 1 : (dir == 1 ? (size_t)g.NX : (size_t)g.NX * g.NY);
+    Face L, R;
+    auto rec = [&](const double* f, double& l, double& r) {
+        muscl(order, limiter, f[c0 - st], f[c0], f[c0 + st], f[c0 + 2 * st], l, r);
+    };
+    rec(FLD(F, EU3D_FIELD_P), L.P, R.P);
+    rec(FLD(F, EU3D_FIELD_D), L.D, R.D);
+    rec(FLD(F, EU3D_FIELD_U), L.U, R.U);
+    rec(FLD(F, EU3D_FIELD_V), L.V, R.V);
+    rec(FLD(F, EU3D_FIELD_W), L.W, R.W);
+    rec(FLD(F, EU3D_FIELD_H), L.H, R.H);
+    rec(FLD(F, EU3D_FIELD_GAMMA), L.G, R.G);
+    const double* Y = FLD(F, EU3D_FIELD_YI);
+#pragma unroll
+    for (int s = 0; s < NS; s++) rec(Y + s * g.n3, L.Y[s], R.Y[s]);
+    ausm_flux(dir, L, R, zero_y, fl);
+}
+
+__global__ void __launch_bounds__(512) k_flux_fused(G3 g, Fields F, int order, int limiter, uint32_t q,
+                                                    const double* __restrict__ xn, const double* __restrict__ yn,
+                                                    const double* __restrict__ zn, const Scalars* __restrict__ sc) {
+    __shared__ double sf[NC][FX * FY];
+    const int tx = threadIdx.x, ty = threadIdx.y, t = ty * FX + tx;
+    const int x = g.bc + blockIdx.x * (FX - 1) + tx;
+    const int y = g.bc + blockIdx.y * (FY - 1) + ty;
+    const int z0 = g.bc + blockIdx.z * FZ;
+    const int z1 = min(z0 + FZ, g.nk + g.bc);
+    const int xe = g.ni + g.bc, ye = g.nj + g.bc;
+    const bool out = tx < FX - 1 && ty < FY - 1 && x < xe && y < ye;
+    const bool xface = x <= xe && y < ye;   // face between x-1 and x
+    const bool yface = x < xe && y <= ye;   // face between y-1 and y
+    const bool f1b = q & Q_F1B;
+    const double dt = sc->dt;
+    double cx = 0, cy = 0;
+    if (out) {
+        cx = -dt / (xn[x] - xn[x - 1]);
+        cy = dt / (yn[y] - yn[y - 1]);
+    }
+    const size_t sz = (size_t)g.NX * g.NY;
+    double fzb[NC];
+    if (out) face_flux_at<2>(g, F, IDX(g, x, y, z0 - 1), order, limiter, f1b && x >= g.ni, fzb);
+    double* Rh = FLD(F, EU3D_FIELD_RHS);
+    for (int z = z0; z < z1; z++) {
+        double rhs[NC], f[NC];
+        if (xface) {
+            face_flux_at<0>(g, F, IDX(g, x - 1, y, z), order, limiter, f1b && x - 1 >= g.ni, f);
+#pragma unroll
+            for (int c = 0; c < NC; c++) sf[c][t] = f[c];
+        }
+        __syncthreads();
+        if (out) {
+#pragma unroll
+            for (int c = 0; c < NC; c++) rhs[c] = cx * (sf[c][t + 1] - sf[c][t]);
+        }
+        __syncthreads();
+        if (yface) {
+            face_flux_at<1>(g, F, IDX(g, x, y - 1, z), order, limiter, f1b && x >= g.ni, f);
+#pragma unroll
+            for (int c = 0; c < NC; c++) sf[c][t] = f[c];
+        }
+        __syncthreads();
+        if (out) {
+#pragma unroll
+            for (int c = 0; c < NC; c++) rhs[c] -= cy * (sf[c][t + FX] - sf[c][t]);
+            const double cz = dt / (zn[z] - zn[z - 1]);
+            face_flux_at<2>(g, F, IDX(g, x, y, z), order, limiter, f1b && x >= g.ni, f);
+            const size_t cc = IDX(g, x, y, z);
+#pragma unroll
+            for (int c = 0; c < NC; c++) {
+                rhs[c] -= cz * (f[c] - fzb[c]);
+                fzb[c] = f[c];
+                Rh[c * g.n3 + cc] = rhs[c];
+            }
+        }
+        __syncthreads();
+    }
+    (void)sz;
+}
+
+// ---------------------------------------------------------------- fused IMEX
+// Reaction::Diagonalized + Flowfield::Update_IMEX + Update_after_CS + GetPartial_T
+// (src/Reaction.cpp:736-811, src/Flowfield.cpp:1215-1234,1246-1284,1578-1595) for one cell in
+// registers. The inputs of Diagonalized (Mc, Di, dT/drho_s) are recomputed from the state at
+// the start of the step (CS, T) instead of being read back; the outputs are the new CS, T, E and
+// the primitives the next advection and CFL bound read. Each block writes its CFL partial.
+__global__ void __launch_bounds__(256) k_imex_fused(G3 g, Fields F, uint32_t q, const Scalars* __restrict__ sc,
+                                                    double dx, double dy, double dz, double* part) {
+    int i, j, k;
+    double v = 1e10;
+    if (interior_ijk(g, i, j, k)) {
+        const size_t c = IDX(g, i, j, k);
+        double cs[NC];
+        load_cs(F, g, c, cs);
+        const double T0 = FLD(F, EU3D_FIELD_T)[c], E0 = FLD(F, EU3D_FIELD_E)[c];
+        double D = 0;
+#pragma unroll
+        for (int s = 0; s < NS; s++) D += cs[s];
+        Prim o;
+        o.D = D;
+        o.U = cs[NS] / D;
+        o.V = cs[NS + 1] / D;
+        o.W = cs[NS + 2] / D;
+        o.T = T0;
+        double Y[NS], Mc[NS], Dr[NS], sy = 0;
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+            Y[s] = cs[s] / D;
+            sy += Y[s] / MW[s];
+            Mc[s] = (cs[s] / MW[s] * 1000) * 1e-6;
+            Dr[s] = cs[s] * 1e-3;
+        }
+        o.Rgas = R_UNIV * 1000 / (1.0 / sy);
+        o.Cp = mix_cp(mix_fit(Y, q & Q_C4), T0) * 1000;
+        double pt[NS];
+        partial_T(o, q, pt);
+        const double tsc = (q & Q_C6) ? 1.0 : 1e3;
+#pragma unroll
+        for (int s = 0; s < NS; s++) pt[s] *= tsc;
+        KinOut ko;
+        kinetics<false, true>(T0, Mc, Dr, pt, 2, q & Q_C4, ko);
+        const double dt = sc->dt;
+        const double* R = FLD(F, EU3D_FIELD_RHS);
+#pragma unroll
+        for (int s = 0; s < NS; s++) cs[s] = cs[s] + (R[s * g.n3 + c] + ko.w[s] * 1e3 * dt) / (1.0 - dt * ko.md[s]);
+#pragma unroll
+        for (int s = NS; s < NC; s++) cs[s] = cs[s] + R[s * g.n3 + c];
+        const Prim n = prim_from_cs(cs, T0, E0, q);
+        double* CS = FLD(F, EU3D_FIELD_CS);
+#pragma unroll
+        for (int s = 0; s < NC; s++) CS[s * g.n3 + c] = cs[s];
+        FLD(F, EU3D_FIELD_T)[c] = n.T;
+        FLD(F, EU3D_FIELD_E)[c] = n.E;
+        FLD(F, EU3D_FIELD_D)[c] = n.D;
+        FLD(F, EU3D_FIELD_P)[c] = n.P;
+        FLD(F, EU3D_FIELD_U)[c] = n.U;
+        FLD(F, EU3D_FIELD_V)[c] = n.V;
+        FLD(F, EU3D_FIELD_W)[c] = n.W;
+        FLD(F, EU3D_FIELD_H)[c] = n.H;
+        FLD(F, EU3D_FIELD_GAMMA)[c] = n.Gamma;
+        FLD(F, EU3D_FIELD_C)[c] = n.C;
+#pragma unroll
+        for (int s = 0; s < NS; s++) FLD(F, EU3D_FIELD_YI)[s * g.n3 + c] = n.Y[s];
+        v = cfl_bound(dx, dy, dz, n.U, n.V, n.W, n.C, q);
+    }
+    block_min_store(v, part);
+}
+
+// Derived fields the fused step does not write (Di, Mc, Wav, Rgas, Cp, Ma, Partion_T), refreshed
+// from CS and T once at the end of eu3d_step so every field is consistent for the caller.
+__global__ void __launch_bounds__(256) k_derive(G3 g, Fields F, uint32_t q) {
+    int i, j, k;
+    if (!interior_ijk(g, i, j, k)) return;
+    const size_t c = IDX(g, i, j, k);
+    double cs[NC], Y[NS], sy = 0;
+    load_cs(F, g, c, cs);
+    double D = 0;
+#pragma unroll
+    for (int s = 0; s < NS; s++) D += cs[s];
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        Y[s] = cs[s] / D;
+        sy += Y[s] / MW[s];
+        FLD(F, EU3D_FIELD_DI)[s * g.n3 + c] = cs[s];
+        FLD(F, EU3D_FIELD_MC)[s * g.n3 + c] = cs[s] / MW[s] * 1000;
+    }
+    Prim o;
+    o.D = D;
+    o.U = FLD(F, EU3D_FIELD_U)[c];
+    o.V = FLD(F, EU3D_FIELD_V)[c];
+    o.W = FLD(F, EU3D_FIELD_W)[c];
+    o.T = FLD(F, EU3D_FIELD_T)[c];
+    o.Wav = 1.0 / sy;
+    o.Rgas = R_UNIV * 1000 / o.Wav;
+    o.Cp = mix_cp(mix_fit(Y, q & Q_C4), o.T) * 1000;
+    FLD(F, EU3D_FIELD_WAV)[c] = o.Wav;
+    FLD(F, EU3D_FIELD_RGAS)[c] = o.Rgas;
+    FLD(F, EU3D_FIELD_CP)[c] = o.Cp;
+    FLD(F, EU3D_FIELD_MA)[c] = sqrt(o.U * o.U + o.V * o.V + o.W * o.W) / FLD(F, EU3D_FIELD_C)[c];
+    double pt[NS];
+    partial_T(o, q, pt);
+    double* PT = FLD(F, EU3D_FIELD_PT);
+#pragma unroll
+    for (int s = 0; s < NS; s++) PT[s * g.n3 + c] = pt[s];
+    const double CV = o.Cp - o.Rgas;
+    PT[(NS + 1) * g.n3 + c] = -o.U / (CV * o.D);
+    PT[(NS + 2) * g.n3 + c] = 1.0 / (CV * o.D);
+}
+
 // ---------------------------------------------------------------- chemistry
 __device__ __forceinline__ void react_units(const Fields& F, const G3& g, size_t c, double (&Mc)[NS],
                                             double (&Dr)[NS]) {
@@ -402,20 +628,18 @@ __global__ void __launch_bounds__(256) k_diagonalized(G3 g, Fields F, uint32_t q
     int i, j, k;
     if (!interior_ijk(g, i, j, k)) return;
     const size_t c = IDX(g, i, j, k);
-    double Mc[NS], Dr[NS], w[NS], pt[NS], md[NS];
+    double Mc[NS], Dr[NS], pt[NS];
     react_units(F, g, c, Mc, Dr);
-    const double T = FLD(F, EU3D_FIELD_T)[c];
-    Rates kr;
-    kin_rates(T, Mc, 2, q & Q_C4, kr);
-    kin_wdot(kr, w);
+    const double tsc = (q & Q_C6) ? 1.0 : 1e3;
+#pragma unroll
+    for (int s = 0; s < NS; s++) pt[s] = FLD(F, EU3D_FIELD_PT)[s * g.n3 + c] * tsc;
+    KinOut ko;
+    kinetics<false, true>(FLD(F, EU3D_FIELD_T)[c], Mc, Dr, pt, 2, q & Q_C4, ko);
 #pragma unroll
     for (int s = 0; s < NS; s++) {
-        FLD(F, EU3D_FIELD_CMS)[s * g.n3 + c] = w[s] * 1e3;
-        pt[s] = FLD(F, EU3D_FIELD_PT)[s * g.n3 + c];
+        FLD(F, EU3D_FIELD_CMS)[s * g.n3 + c] = ko.w[s] * 1e3;
+        FLD(F, EU3D_FIELD_MD)[s * g.n3 + c] = ko.md[s];
     }
-    kin_diag(kr, T, Dr, pt, q & Q_C6, md);
-#pragma unroll
-    for (int s = 0; s < NS; s++) FLD(F, EU3D_FIELD_MD)[s * g.n3 + c] = md[s];
 }
 
 // Flowfield::Update_IMEX conserved update, src/Flowfield.cpp:1219-1228. MD(c,c) for c >= NS is
@@ -439,19 +663,19 @@ __global__ void __launch_bounds__(256) k_update_imex(G3 g, Fields F, const Scala
 }
 
 // Reaction::Trapezoid full field, src/Reaction.cpp:412-485 (quirk C1 for the R_TB default)
+template <bool C3>
 __global__ void __launch_bounds__(256) k_trapezoid(G3 g, Fields F, uint32_t q, const Scalars* __restrict__ sc) {
     int i, j, k;
     if (!interior_ijk(g, i, j, k)) return;
     const size_t c = IDX(g, i, j, k);
-    double Mc[NS], Dr[NS], w[NS], Dn[NS];
+    double Mc[NS], Dr[NS], Dn[NS];
     react_units(F, g, c, Mc, Dr);
-    Rates kr;
-    kin_rates(FLD(F, EU3D_FIELD_T)[c], Mc, (q & Q_C1) ? 0 : 2, q & Q_C4, kr);
-    kin_wdot(kr, w);
-    kin_trap(kr, Dr, Mc, sc->dt, q & Q_C3, Dn);
+    KinOut ko;
+    kinetics<true, false, C3>(FLD(F, EU3D_FIELD_T)[c], Mc, Dr, Dr, (q & Q_C1) ? 0 : 2, q & Q_C4, ko);
+    trap_update(ko, Dr, sc->dt, Dn);
 #pragma unroll
     for (int s = 0; s < NS; s++) {
-        FLD(F, EU3D_FIELD_CMS)[s * g.n3 + c] = w[s] * 1e3;
+        FLD(F, EU3D_FIELD_CMS)[s * g.n3 + c] = ko.w[s] * 1e3;
         FLD(F, EU3D_FIELD_DI)[s * g.n3 + c] = Dn[s];
     }
 }
@@ -502,12 +726,12 @@ __device__ __forceinline__ void chem_inputs(const ChemCell& c, double (&Mc)[NS],
     }
 }
 
-// Sub-step tail: the trapezoid update with rates kr, then Flowfield::Explicit(cell) and
+// Sub-step tail: the trapezoid update from the kinetics k, then Flowfield::Explicit(cell) and
 // Update_after_CS(cell) (src/Flowfield.cpp:1189-1203,1287-1319). Leaves the conserved state in cs.
-__device__ __forceinline__ Prim chem_apply(ChemCell& c, const Rates& kr, const double (&Mc)[NS],
-                                           const double (&Dr)[NS], double dt, uint32_t q, double (&cs)[NC]) {
+__device__ __forceinline__ Prim chem_apply(ChemCell& c, const KinOut& k, const double (&Dr)[NS], double dt,
+                                           uint32_t q, double (&cs)[NC]) {
     double Dn[NS];
-    kin_trap(kr, Dr, Mc, dt, q & Q_C3, Dn);
+    trap_update(k, Dr, dt, Dn);
     double D = 0.0;
 #pragma unroll
     for (int s = 0; s < NS; s++) {
@@ -531,15 +755,13 @@ __device__ __forceinline__ Prim chem_apply(ChemCell& c, const Rates& kr, const d
 
 // Reaction::TrapezoidPrediction sub-step count (src/Reaction.cpp:650-668) from the same rates
 // the first sub-step uses. nmax caps the count.
-__device__ __forceinline__ int chem_nchem(const Rates& kr, const double (&Dr)[NS], const double (&Y)[NS],
+__device__ __forceinline__ int chem_nchem(const KinOut& k, const double (&Dr)[NS], const double (&Y)[NS],
                                           double dtm, int nmax) {
-    double w[NS];
-    kin_wdot(kr, w);
     double dt = dtm;
 #pragma unroll
     for (int s = 0; s < NS; s++) {
-        if (Y[s] >= 1e-6 && w[s] != 0) {
-            const double t = fabs(-Dr[s] / w[s]);
+        if (Y[s] >= 1e-6 && k.w[s] != 0) {
+            const double t = fabs(-Dr[s] / k.w[s]);
             if (t < dt) dt = t;
         }
     }
@@ -563,16 +785,14 @@ struct GridCells {
         c.T = FLD(F, EU3D_FIELD_T)[cc];
         n = (int)FLD(F, EU3D_FIELD_NCHEM)[cc];
     }
-    __device__ void store(unsigned id, const double (&cs)[NC], const Prim& o, const Rates& kr) const {
+    __device__ void store(unsigned id, const double (&cs)[NC], const Prim& o, const KinOut& k) const {
         const size_t cc = id;
         double* CS = FLD(F, EU3D_FIELD_CS);
 #pragma unroll
         for (int s = 0; s < NC; s++) CS[s * g.n3 + cc] = cs[s];
         store_prims(F, g, cc, cs, o);
-        double w[NS];
-        kin_wdot(kr, w);
 #pragma unroll
-        for (int s = 0; s < NS; s++) FLD(F, EU3D_FIELD_CMS)[s * g.n3 + cc] = w[s] * 1e3;
+        for (int s = 0; s < NS; s++) FLD(F, EU3D_FIELD_CMS)[s * g.n3 + cc] = k.w[s] * 1e3;
     }
 };
 
@@ -589,7 +809,7 @@ struct BatchCells {
         c.T = T[id];
         nn = nchem[id];
     }
-    __device__ void store(unsigned id, const double (&cs)[NC], const Prim& o, const Rates&) const {
+    __device__ void store(unsigned id, const double (&cs)[NC], const Prim& o, const KinOut&) const {
 #pragma unroll
         for (int s = 0; s < NS; s++) rho[(size_t)s * n + id] = cs[s];
         T[id] = o.T;
@@ -621,6 +841,7 @@ __device__ __forceinline__ void chem_count(Scalars* sc, int nch, bool active, un
 
 // Prediction + first sub-step for every interior cell; cells needing more sub-steps are
 // compacted into the work list.
+template <bool C3>
 __global__ void __launch_bounds__(256) k_trap_first(G3 g, Fields F, uint32_t q, const Scalars* __restrict__ scc,
                                                     Scalars* sc, unsigned* list, int nmax, int predict_only) {
     int i, j, k;
@@ -641,17 +862,17 @@ __global__ void __launch_bounds__(256) k_trap_first(G3 g, Fields F, uint32_t q, 
         react_units(F, g, c, Mc, Dr);
 #pragma unroll
         for (int s = 0; s < NS; s++) Y[s] = FLD(F, EU3D_FIELD_YI)[s * g.n3 + c];
-        Rates kr;
-        kin_rates(cell.T, Mc, (q & Q_C2) ? 1 : 2, q & Q_C4, kr);
+        KinOut ko;
+        kinetics<true, false, C3>(cell.T, Mc, Dr, Dr, (q & Q_C2) ? 1 : 2, q & Q_C4, ko);
         const double dtm = scc->dt;
-        nch = chem_nchem(kr, Dr, Y, dtm, nmax);
+        nch = chem_nchem(ko, Dr, Y, dtm, nmax);
         FLD(F, EU3D_FIELD_NCHEM)[c] = (double)nch;
         if (nch <= 0) atomicOr(&sc->nonfinite, 2);
         if (!predict_only && nch > 0) {
             double cs[NC];
-            const Prim o = chem_apply(cell, kr, Mc, Dr, dtm / nch, q, cs);
+            const Prim o = chem_apply(cell, ko, Dr, dtm / nch, q, cs);
             GridCells acc{g, F};
-            acc.store((unsigned)c, cs, o, kr);
+            acc.store((unsigned)c, cs, o, ko);
             active = nch > 1;
         }
     }
@@ -662,7 +883,7 @@ __global__ void __launch_bounds__(256) k_trap_first(G3 g, Fields F, uint32_t q, 
 // wave-aggregated atomic queue and runs that cell's remaining sub-steps, taking a new cell as
 // soon as one finishes. All lanes execute the same sub-step code on different cells, so cells of
 // very different sub-step counts do not serialise a wavefront.
-template <class Acc>
+template <class Acc, bool C3>
 __global__ void __launch_bounds__(256) k_chem_persist(Acc acc, const unsigned* __restrict__ list, Scalars* sc,
                                                       uint32_t q, const double* __restrict__ dtm_p) {
     const unsigned long long n_active = sc->chem_active;
@@ -699,11 +920,11 @@ __global__ void __launch_bounds__(256) k_chem_persist(Acc acc, const unsigned* _
         if (have) {
             double Mc[NS], Dr[NS], cs[NC];
             chem_inputs(cell, Mc, Dr);
-            Rates kr;
-            kin_rates(cell.T, Mc, (q & Q_C2) ? 1 : 2, q & Q_C4, kr);
-            const Prim o = chem_apply(cell, kr, Mc, Dr, dtm / nch, q, cs);
+            KinOut ko;
+            kinetics<true, false, C3>(cell.T, Mc, Dr, Dr, (q & Q_C2) ? 1 : 2, q & Q_C4, ko);
+            const Prim o = chem_apply(cell, ko, Dr, dtm / nch, q, cs);
             if (--rem == 0) {
-                acc.store(id, cs, o, kr);
+                acc.store(id, cs, o, ko);
                 have = false;
             }
         }
@@ -711,6 +932,7 @@ __global__ void __launch_bounds__(256) k_chem_persist(Acc acc, const unsigned* _
 }
 
 // Batch: prediction + first sub-step on flat cells at rest.
+template <bool C3>
 __global__ void __launch_bounds__(256) k_batch_first(BatchCells acc, int* nchem_out, int* nchem_keep, uint32_t q,
                                                      double dtm, int nmax, Scalars* sc, unsigned* list) {
     const long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x;
@@ -730,16 +952,16 @@ __global__ void __launch_bounds__(256) k_batch_first(BatchCells acc, int* nchem_
         for (int s = 0; s < NS; s++) D += cell.Di[s];
 #pragma unroll
         for (int s = 0; s < NS; s++) Y[s] = cell.Di[s] / D;
-        Rates kr;
-        kin_rates(cell.T, Mc, (q & Q_C2) ? 1 : 2, q & Q_C4, kr);
-        nch = chem_nchem(kr, Dr, Y, dtm, nmax);
+        KinOut ko;
+        kinetics<true, false, C3>(cell.T, Mc, Dr, Dr, (q & Q_C2) ? 1 : 2, q & Q_C4, ko);
+        nch = chem_nchem(ko, Dr, Y, dtm, nmax);
         nchem_keep[id] = nch;
         if (nchem_out) nchem_out[id] = nch;
         if (nch <= 0) atomicOr(&sc->nonfinite, 2);
         if (nch > 0) {
             double cs[NC];
-            const Prim o = chem_apply(cell, kr, Mc, Dr, dtm / nch, q, cs);
-            acc.store((unsigned)id, cs, o, kr);
+            const Prim o = chem_apply(cell, ko, Dr, dtm / nch, q, cs);
+            acc.store((unsigned)id, cs, o, ko);
             active = nch > 1;
         }
     }
@@ -807,15 +1029,47 @@ hipError_t launch_advection(eu3d_ctx* c, int diff, int limiter) {
     for (int d = 0; d < 3; d++) {
         const int nx = g.ni + (d == 0), ny = g.nj + (d == 1), nz = g.nk + (d == 2);
         dim3 grid((nx + BX - 1) / BX, (ny + BY - 1) / BY, nz);
-        k_face_flux<<<grid, dim3(BX, BY), 0, c->stream>>>(g, c->fl, d, diff, limiter, c->quirks, c->fl.f[fid[d]]);
+        if (d == 0) k_face_flux<0><<<grid, dim3(BX, BY), 0, c->stream>>>(g, c->fl, diff, limiter, c->quirks, c->fl.f[fid[d]]);
+        if (d == 1) k_face_flux<1><<<grid, dim3(BX, BY), 0, c->stream>>>(g, c->fl, diff, limiter, c->quirks, c->fl.f[fid[d]]);
+        if (d == 2) k_face_flux<2><<<grid, dim3(BX, BY), 0, c->stream>>>(g, c->fl, diff, limiter, c->quirks, c->fl.f[fid[d]]);
     }
     k_ee<<<interior_grid(g), dim3(BX, BY), 0, c->stream>>>(g, c->fl, c->xnode, c->ynode, c->znode, c->sc);
     return hipGetLastError();
 }
 
+static int n_interior_blocks(const G3& g) {
+    const dim3 b = interior_grid(g);
+    return (int)(b.x * b.y * b.z);
+}
+
 hipError_t launch_cfl(eu3d_ctx* c, double cfl, double final_time) {
-    k_cfl<<<interior_grid(c->g), dim3(BX, BY), 0, c->stream>>>(G(c), c->fl, c->dx0, c->dy0, c->dz0, c->quirks, c->sc);
-    k_cfl_finish<<<1, 1, 0, c->stream>>>(c->sc, cfl, final_time);
+    k_cfl<<<interior_grid(c->g), dim3(BX, BY), 0, c->stream>>>(G(c), c->fl, c->dx0, c->dy0, c->dz0, c->quirks,
+                                                              c->part);
+    k_cfl_finish<<<1, 1024, 0, c->stream>>>(c->sc, c->part, n_interior_blocks(c->g), cfl, final_time);
+    return hipGetLastError();
+}
+
+hipError_t launch_flux_fused(eu3d_ctx* c, int diff, int limiter) {
+    const G3& g = c->g;
+    dim3 grid((g.ni + FX - 2) / (FX - 1), (g.nj + FY - 2) / (FY - 1), (g.nk + FZ - 1) / FZ);
+    k_flux_fused<<<grid, dim3(FX, FY), 0, c->stream>>>(g, c->fl, diff, limiter, c->quirks, c->xnode, c->ynode,
+                                                        c->znode, c->sc);
+    return hipGetLastError();
+}
+
+hipError_t launch_imex_fused(eu3d_ctx* c) {
+    k_imex_fused<<<interior_grid(c->g), dim3(BX, BY), 0, c->stream>>>(G(c), c->fl, c->quirks, c->sc, c->dx0, c->dy0,
+                                                                     c->dz0, c->part);
+    return hipGetLastError();
+}
+
+hipError_t launch_cfl_finish(eu3d_ctx* c, double cfl, double final_time) {
+    k_cfl_finish<<<1, 1024, 0, c->stream>>>(c->sc, c->part, n_interior_blocks(c->g), cfl, final_time);
+    return hipGetLastError();
+}
+
+hipError_t launch_derive(eu3d_ctx* c) {
+    k_derive<<<interior_grid(c->g), dim3(BX, BY), 0, c->stream>>>(G(c), c->fl, c->quirks);
     return hipGetLastError();
 }
 
@@ -847,7 +1101,10 @@ hipError_t launch_update_imex(eu3d_ctx* c) {
 }
 
 hipError_t launch_trapezoid(eu3d_ctx* c) {
-    k_trapezoid<<<interior_grid(c->g), dim3(BX, BY), 0, c->stream>>>(G(c), c->fl, c->quirks, c->sc);
+    if (c->quirks & Q_C3)
+        k_trapezoid<true><<<interior_grid(c->g), dim3(BX, BY), 0, c->stream>>>(G(c), c->fl, c->quirks, c->sc);
+    else
+        k_trapezoid<false><<<interior_grid(c->g), dim3(BX, BY), 0, c->stream>>>(G(c), c->fl, c->quirks, c->sc);
     return hipGetLastError();
 }
 
@@ -860,13 +1117,21 @@ hipError_t launch_trap_adaptive(eu3d_ctx* c, int nchem_max, bool predict_only) {
     const int nmax = nchem_max > 0 ? nchem_max : 10000;
     double* slot = &c->sc->dtm;
     k_chem_reset<<<1, 1, 0, c->stream>>>(c->sc, slot, 0.0, 1);
-    k_trap_first<<<interior_grid(c->g), dim3(BX, BY), 0, c->stream>>>(G(c), c->fl, c->quirks, c->sc, c->sc,
-                                                                     c->active, nmax, predict_only ? 1 : 0);
+    const bool c3 = c->quirks & Q_C3;
+    if (c3)
+        k_trap_first<true><<<interior_grid(c->g), dim3(BX, BY), 0, c->stream>>>(G(c), c->fl, c->quirks, c->sc, c->sc,
+                                                                           c->active, nmax, predict_only ? 1 : 0);
+    else
+        k_trap_first<false><<<interior_grid(c->g), dim3(BX, BY), 0, c->stream>>>(G(c), c->fl, c->quirks, c->sc, c->sc,
+                                                                            c->active, nmax, predict_only ? 1 : 0);
     if (!predict_only) {
         GridCells acc{c->g, c->fl};
         const int blk = c->sched.block;
         const int nb = c->sched.blocks_per_cu * c->num_cu;
-        k_chem_persist<GridCells><<<nb, blk, 0, c->stream>>>(acc, c->active, c->sc, c->quirks, slot);
+        if (c3)
+            k_chem_persist<GridCells, true><<<nb, blk, 0, c->stream>>>(acc, c->active, c->sc, c->quirks, slot);
+        else
+            k_chem_persist<GridCells, false><<<nb, blk, 0, c->stream>>>(acc, c->active, c->sc, c->quirks, slot);
     }
     return hipGetLastError();
 }
@@ -887,17 +1152,25 @@ hipError_t launch_chem_batch(eu3d_ctx* c, long long n, double* rho, double* T, d
     double* slot = &c->sc->dtm;
     k_chem_reset<<<1, 1, 0, c->stream>>>(c->sc, slot, dtm, 0);
     BatchCells acc{n, rho, T, E, keep};
-    k_batch_first<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(acc, nchem_out, keep, c->quirks, dtm, nmax,
-                                                                       c->sc, list);
     const int nb = c->sched.blocks_per_cu * c->num_cu;
-    k_chem_persist<BatchCells><<<nb, c->sched.block, 0, c->stream>>>(acc, list, c->sc, c->quirks, slot);
+    const unsigned gb = (unsigned)((n + 255) / 256);
+    if (c->quirks & Q_C3) {
+        k_batch_first<true><<<gb, 256, 0, c->stream>>>(acc, nchem_out, keep, c->quirks, dtm, nmax, c->sc, list);
+        k_chem_persist<BatchCells, true><<<nb, c->sched.block, 0, c->stream>>>(acc, list, c->sc, c->quirks, slot);
+    } else {
+        k_batch_first<false><<<gb, 256, 0, c->stream>>>(acc, nchem_out, keep, c->quirks, dtm, nmax, c->sc, list);
+        k_chem_persist<BatchCells, false><<<nb, c->sched.block, 0, c->stream>>>(acc, list, c->sc, c->quirks, slot);
+    }
     return hipGetLastError();
 }
 
-hipError_t launch_step(eu3d_ctx* c, const eu3d_step_params* p) {
+hipError_t launch_step(eu3d_ctx* c, const eu3d_step_params* p, bool first) {
     hipError_t e;
+    const bool imex = p->rk == 1 && p->chem == EU3D_CHEM_IMEX;
     if (p->cfl > 0) {
-        if ((e = launch_cfl(c, p->cfl, p->final_time)) != hipSuccess) return e;
+        // the fused IMEX epilogue leaves the CFL partials of the new state behind
+        e = (imex && !first) ? launch_cfl_finish(c, p->cfl, p->final_time) : launch_cfl(c, p->cfl, p->final_time);
+        if (e != hipSuccess) return e;
     } else {
         k_advance_time<<<1, 1, 0, c->stream>>>(c->sc);
     }
@@ -905,10 +1178,10 @@ hipError_t launch_step(eu3d_ctx* c, const eu3d_step_params* p) {
     if (p->rk == 2) {
         const size_t n = c->g.n3 * NC;
         k_copy<<<grid1d(n, 256), 256, 0, c->stream>>>(c->fl.f[EU3D_FIELD_CS], c->rk_save, n);
-        if ((e = launch_advection(c, p->diff, p->limiter)) != hipSuccess) return e;
+        if ((e = launch_flux_fused(c, p->diff, p->limiter)) != hipSuccess) return e;
         if ((e = launch_update_after_adv(c)) != hipSuccess) return e;
         if ((e = launch_fill_ghosts(c, p->bc_kinds)) != hipSuccess) return e;
-        if ((e = launch_advection(c, p->diff, p->limiter)) != hipSuccess) return e;
+        if ((e = launch_flux_fused(c, p->diff, p->limiter)) != hipSuccess) return e;
         k_rk2_combine<<<interior_grid(c->g), dim3(BX, BY), 0, c->stream>>>(G(c), c->fl, c->rk_save);
         if ((e = launch_update_after_cs(c)) != hipSuccess) return e;
         switch (p->chem) {
@@ -919,28 +1192,23 @@ hipError_t launch_step(eu3d_ctx* c, const eu3d_step_params* p) {
         case EU3D_CHEM_TRAP_ADAPTIVE: e = launch_trap_adaptive(c, p->nchem_max, false); break;
         default: e = hipSuccess;
         }
-        if (e != hipSuccess) return e;
-    } else {
-        if ((e = launch_advection(c, p->diff, p->limiter)) != hipSuccess) return e;
-        switch (p->chem) {
-        case EU3D_CHEM_IMEX:
-            if ((e = launch_diagonalized(c)) != hipSuccess) return e;
-            e = launch_update_imex(c);
-            break;
-        case EU3D_CHEM_TRAPEZOID:
-            if ((e = launch_update_after_adv(c)) != hipSuccess) return e;
-            if ((e = launch_trapezoid(c)) != hipSuccess) return e;
-            e = launch_explicit(c);
-            break;
-        case EU3D_CHEM_TRAP_ADAPTIVE:
-            if ((e = launch_update_after_adv(c)) != hipSuccess) return e;
-            e = launch_trap_adaptive(c, p->nchem_max, false);
-            break;
-        default: e = launch_update_after_adv(c);
-        }
-        if (e != hipSuccess) return e;
+        return e;
     }
-    return launch_check_finite(c);
+    if ((e = launch_flux_fused(c, p->diff, p->limiter)) != hipSuccess) return e;
+    switch (p->chem) {
+    case EU3D_CHEM_IMEX: e = launch_imex_fused(c); break;
+    case EU3D_CHEM_TRAPEZOID:
+        if ((e = launch_update_after_adv(c)) != hipSuccess) return e;
+        if ((e = launch_trapezoid(c)) != hipSuccess) return e;
+        e = launch_explicit(c);
+        break;
+    case EU3D_CHEM_TRAP_ADAPTIVE:
+        if ((e = launch_update_after_adv(c)) != hipSuccess) return e;
+        e = launch_trap_adaptive(c, p->nchem_max, false);
+        break;
+    default: e = launch_update_after_adv(c);
+    }
+    return e;
 }
 
 }  // namespace eu3d

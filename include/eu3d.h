@@ -129,6 +129,9 @@ int eu3d_fill_ghosts(eu3d_ctx *ctx, const int kinds[6]);
  * AUSM (src/Flowfield.cpp:1330-1502) per direction + TimeAdv::EE (src/TimeAdv.cpp:29-43).
  * diff: 0 = MUSCL_1, 1 = MUSCL_2; limiter 0..4 as include/Differ.hpp:188-220. */
 int eu3d_advection(eu3d_ctx *ctx, int diff, int limiter);
+/* The same RHS as eu3d_advection from one fused pass (no face-flux arrays): the form
+ * eu3d_step uses. F, G and Q are left untouched. */
+int eu3d_advection_fused(eu3d_ctx *ctx, int diff, int limiter);
 /* Flowfield::Update_after_Adv, src/Flowfield.cpp:1140-1157 */
 int eu3d_update_after_adv(eu3d_ctx *ctx);
 /* Flowfield::Update_after_CS + Get_temp, src/Flowfield.cpp:1246-1284,1542-1566 */

@@ -169,6 +169,9 @@ class OracleFlow:
     def advection(self, diff=1, limiter=0):
         self.lib.orc_advection(self._F(), 2 if diff == 1 else 1, limiter)
 
+    def advection_fused(self, diff=1, limiter=0):
+        self.advection(diff, limiter)
+
     def update_after_adv(self):
         self.lib.orc_update_after_adv(self._F(), self.om.ref())
 
