@@ -1,0 +1,152 @@
+"""Benchmark: plain FP64 cell-updates/s of the EU3D hot path on one MI355X.
+
+Workload: the 3-D shock / H2-bubble interaction, 384 x 128 x 128 cells, dx = 1e-4 m, Mach-2 shock
+in air, stoichiometric H2-air bubble of radius 25 cells, IMEX chemistry, CFL 0.5 (seeded
+synthetic initial condition). A step is one full time step: CFL dt, ghost fill, MUSCL/AUSM
+advection of all three directions with the explicit update, and the IMEX chemistry with the
+temperature Newton. Inputs are resident in HBM before the timed region.
+
+Prints one JSON line. `roofline` is for the dominant kernel (the fused advection pass),
+timed live with HIP events on the context's stream; `cpu_baseline` times the CPU restatement
+of the reference arithmetic (the oracle) on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "cell-updates/s (flux+chem), plain FP64 vs MLP-chem surrogate at stated error"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+# algorithmic bytes of the fused advection pass per interior cell: read the 16 reconstructed
+# primitives (P, rho, u, v, w, H, gamma, Y_1..9) once, write the 13 RHS components once
+FLUX_BYTES_PER_CELL = (16 + 13) * 8
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--grid", type=str, default="384x128x128")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-grid", type=str, default="192x96x96")
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    return ap.parse_args()
+
+
+def hip_timer(stream_ptr):
+    import torch
+    s = torch.cuda.ExternalStream(stream_ptr)
+
+    def t():
+        e = torch.cuda.Event(enable_timing=True)
+        e.record(s)
+        return e
+    return t
+
+
+def cpu_baseline(grid, steps):
+    """Oracle (CPU restatement, OpenMP over cells) on a bounded sample of the same workload."""
+    from eu3d_amd import configs
+    import oracle
+    ni, nj, nk = (int(v) for v in grid.split("x"))
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    cores = min(cores, int(os.environ.get("OMP_NUM_THREADS", cores)))
+    oracle.set_threads(cores)
+    wl = configs.bubble3d(ni, nj, nk, radius_cells=25.0 * ni / 384)
+    o = oracle.OracleFlow(wl.xnode, wl.ynode, wl.znode, quirks=0)
+    for k in ("T", "P", "U", "V", "W", "YI"):
+        o.set(k, wl.prims[k])
+    o.field_initial()
+    kw = dict(cfl=0.5, bc_kinds=wl.bc_kinds, rk=1, chem=1)
+    o.step(1, **kw)
+    t0 = time.perf_counter()
+    o.step(steps, **kw)
+    el = time.perf_counter() - t0
+    return {"value": wl.ncells * steps / el, "unit": "cell-updates/s", "cores": cores, "kind": "port",
+            "sample": f"bubble3d {ni}x{nj}x{nk} (same generator, bubble radius scaled), {steps} IMEX steps "
+                      f"after 1 warm-up step, {el:.1f} s"}
+
+
+def read_traffic():
+    """HBM bytes per launch of the fused advection pass from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), or None."""
+    p = os.path.join(ROOT, "profiles", "flux_fused_bubble3d_384x128x128_pmc.json")
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        d = json.load(f)
+    return d.get("hbm_bytes_per_launch")
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it "
+                                   "(see DESIGN.md for how the path would shard)", "n_gpus": a.gpus}))
+        sys.exit(2)
+    import torch
+    from eu3d_amd import configs
+    from eu3d_amd.api import Context
+    assert torch.cuda.is_available(), "bench.py needs the GPU"
+    ni, nj, nk = (int(v) for v in a.grid.split("x"))
+    wl = configs.bubble3d(ni, nj, nk)
+    g = Context(wl.xnode, wl.ynode, wl.znode, quirks=0)
+    for k in ("T", "P", "U", "V", "W", "YI"):
+        g.set(k, wl.prims[k])
+    g.field_initial()
+    g.sync()
+    kw = dict(cfl=0.5, bc_kinds=wl.bc_kinds, rk=1, chem=1, diff=1, limiter=0)
+    ev = hip_timer(g.stream())
+    g.step(max(a.warmup, 1), **kw)
+    g.sync()
+    torch.cuda.synchronize()
+    t0 = ev()
+    w0 = time.perf_counter()
+    g.step(a.steps, **kw)
+    t1 = ev()
+    g.sync()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - w0
+    ms = t0.elapsed_time(t1) / a.steps
+    value = wl.ncells / (ms * 1e-3)
+    rc = g.check_finite()
+    # dominant kernel: the fused advection pass, timed alone on the context stream
+    reps = 10
+    g.advection_fused(1, 0)
+    g.sync()
+    k0 = ev()
+    for _ in range(reps):
+        g.advection_fused(1, 0)
+    k1 = ev()
+    g.sync()
+    kern_s = k0.elapsed_time(k1) * 1e-3 / reps
+    alg = FLUX_BYTES_PER_CELL * wl.ncells
+    achieved = alg / kern_s / 1e9
+    res = {
+        "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"bubble3d {ni}x{nj}x{nk} (3-D Mach-2 shock / H2-air bubble, IMEX, "
+                               f"FP64, MUSCL-minmod + AUSM, CFL 0.5)", "cells": wl.ncells,
+                   "chemistry": "IMEX", "quirks": "FIXED"},
+        "roofline": {"kernel": "k_flux_fused (advection, 3 directions + EE)", "bound": "hbm",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(),
+                     "algorithmic_bytes_per_launch": alg, "avg_launch_ms": kern_s * 1e3},
+        "finite": rc == 0, "wall_s_timed": wall,
+    }
+    if not a.no_cpu:
+        res["cpu_baseline"] = cpu_baseline(a.cpu_grid, a.cpu_steps)
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()
