@@ -40,7 +40,12 @@ EXPORTED = [
     "eu3d_trapezoid_adaptive", "eu3d_check_finite", "eu3d_step", "eu3d_get_chem_stats",
     "eu3d_chem_batch", "eu3d_chem_set_schedule", "eu3d_chem_get_schedule", "eu3d_device_field",
     "eu3d_cells_with_ghosts", "eu3d_stream",
+    "eu3d_mlp_create", "eu3d_mlp_destroy", "eu3d_mlp_param_count", "eu3d_mlp_get_params", "eu3d_mlp_set_params",
+    "eu3d_mlp_get_grads", "eu3d_mlp_forward", "eu3d_mlp_train_step", "eu3d_gemm_bf16", "eu3d_mlp_stream",
+    "eu3d_mlp_set_surrogate", "eu3d_surrogate_features", "eu3d_surrogate_targets", "eu3d_surrogate_batch",
+    "eu3d_set_surrogate", "eu3d_get_surrogate_stats",
 ]
+CHEM_SURROGATE = 4
 
 
 class Eu3dError(RuntimeError):
@@ -77,6 +82,18 @@ class ChemStats(C.Structure):
 
 class ChemSchedule(C.Structure):
     _fields_ = [("block", C.c_int), ("blocks_per_cu", C.c_int), ("chunk", C.c_int)]
+
+
+class SurrogateParams(C.Structure):
+    _fields_ = [("feat_mean", C.c_float * 12), ("feat_std", C.c_float * 12), ("out_mean", C.c_float * 9),
+                ("out_std", C.c_float * 9), ("hull_lo", C.c_float * 12), ("hull_hi", C.c_float * 12),
+                ("dT_max", C.c_double), ("elem_tol", C.c_double), ("y_neg_tol", C.c_double),
+                ("T_cold", C.c_double)]
+
+
+class SurrogateStats(C.Structure):
+    _fields_ = [("cells", C.c_longlong), ("fallback", C.c_longlong), ("hull", C.c_longlong),
+                ("negative", C.c_longlong), ("element", C.c_longlong), ("dT", C.c_longlong)]
 
 
 _lib = None
@@ -123,6 +140,25 @@ def load_library(path: str = LIB_PATH):
     lib.eu3d_cells_with_ghosts.restype = C.c_size_t
     lib.eu3d_stream.argtypes = [vp]
     lib.eu3d_stream.restype = vp
+    fp = C.POINTER(C.c_float)
+    lib.eu3d_mlp_create.argtypes = [C.POINTER(vp), C.c_uint64]
+    lib.eu3d_mlp_destroy.argtypes = [vp]
+    lib.eu3d_mlp_param_count.argtypes = [vp]
+    lib.eu3d_mlp_param_count.restype = C.c_size_t
+    for n in ("eu3d_mlp_get_params", "eu3d_mlp_set_params", "eu3d_mlp_get_grads"):
+        getattr(lib, n).argtypes = [vp, fp]
+    lib.eu3d_mlp_forward.argtypes = [vp, C.c_int, vp, vp]
+    lib.eu3d_mlp_train_step.argtypes = [vp, C.c_int, vp, vp, C.c_float, C.c_float, fp]
+    lib.eu3d_gemm_bf16.argtypes = [C.c_int, C.c_int, C.c_int, vp, vp, C.c_int, vp]
+    lib.eu3d_mlp_stream.argtypes = [vp]
+    lib.eu3d_mlp_stream.restype = vp
+    lib.eu3d_mlp_set_surrogate.argtypes = [vp, C.POINTER(SurrogateParams)]
+    lib.eu3d_surrogate_features.argtypes = [vp, C.c_longlong, vp, vp, C.c_double, vp]
+    lib.eu3d_surrogate_targets.argtypes = [vp, C.c_longlong, vp, vp, vp]
+    lib.eu3d_surrogate_batch.argtypes = [vp, vp, C.c_longlong, vp, vp, vp, C.c_double, C.c_int,
+                                         C.POINTER(SurrogateStats)]
+    lib.eu3d_set_surrogate.argtypes = [vp, vp]
+    lib.eu3d_get_surrogate_stats.argtypes = [vp, C.POINTER(SurrogateStats)]
     _lib = lib
     return lib
 
@@ -287,8 +323,109 @@ class Context:
         self._chk(self.lib.eu3d_chem_get_schedule(self.h, C.byref(s)), "get_schedule")
         return {"block": s.block, "blocks_per_cu": s.blocks_per_cu, "chunk": s.chunk}
 
+    def set_surrogate(self, mlp):
+        self._chk(self.lib.eu3d_set_surrogate(self.h, mlp.h if mlp is not None else None), "set_surrogate")
+
+    def surrogate_stats(self) -> dict:
+        s = SurrogateStats()
+        self._chk(self.lib.eu3d_get_surrogate_stats(self.h, C.byref(s)), "surrogate_stats")
+        return {k: getattr(s, k) for k, _ in SurrogateStats._fields_}
+
+    def surrogate_batch(self, mlp, n, rho_ptr, T_ptr, E_ptr, dtm, nchem_max=0) -> dict:
+        s = SurrogateStats()
+        rc = self.lib.eu3d_surrogate_batch(self.h, mlp.h, n, C.c_void_p(rho_ptr), C.c_void_p(T_ptr),
+                                           C.c_void_p(E_ptr), dtm, nchem_max, C.byref(s))
+        self._chk(rc, "surrogate_batch")
+        return {k: getattr(s, k) for k, _ in SurrogateStats._fields_}
+
     def device_field(self, name: str) -> int:
         return self.lib.eu3d_device_field(self.h, FIELDS[name])
 
     def stream(self) -> int:
         return self.lib.eu3d_stream(self.h)
+
+
+def _vp(x):
+    """Device pointer of a torch tensor (plumbing only)."""
+    return C.c_void_p(x.data_ptr())
+
+
+class MLP:
+    """The chemistry surrogate network (an eu3d_mlp): 12 -> 4 x 256 GELU -> 9, BF16 MFMA."""
+
+    NIN, NOUT = 32, 16
+
+    def __init__(self, seed: int = 1):
+        self.lib = load_library()
+        h = C.c_void_p()
+        rc = self.lib.eu3d_mlp_create(C.byref(h), seed)
+        if rc != 0:
+            raise Eu3dError(rc, "eu3d_mlp_create failed")
+        self.h = h
+        self.n_params = self.lib.eu3d_mlp_param_count(h)
+        self.params_set = False
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.eu3d_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            raise Eu3dError(rc, what)
+
+    def get_params(self) -> np.ndarray:
+        a = np.empty(self.n_params, dtype=np.float32)
+        self._chk(self.lib.eu3d_mlp_get_params(self.h, a.ctypes.data_as(C.POINTER(C.c_float))), "get_params")
+        return a
+
+    def set_params(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        assert a.size == self.n_params
+        self._chk(self.lib.eu3d_mlp_set_params(self.h, a.ctypes.data_as(C.POINTER(C.c_float))), "set_params")
+
+    def get_grads(self) -> np.ndarray:
+        a = np.empty(self.n_params, dtype=np.float32)
+        self._chk(self.lib.eu3d_mlp_get_grads(self.h, a.ctypes.data_as(C.POINTER(C.c_float))), "get_grads")
+        return a
+
+    def forward(self, feats_bf16, out_f32):
+        """feats: torch int16 [n, 32] holding BF16 bits; out: torch float32 [n, 16]."""
+        n = feats_bf16.shape[0]
+        self._chk(self.lib.eu3d_mlp_forward(self.h, n, _vp(feats_bf16), _vp(out_f32)), "mlp_forward")
+
+    def train_step(self, feats_bf16, targets_f32, lr: float, wd: float = 0.0) -> float:
+        loss = C.c_float()
+        n = feats_bf16.shape[0]
+        self._chk(self.lib.eu3d_mlp_train_step(self.h, n, _vp(feats_bf16), _vp(targets_f32), lr, wd, C.byref(loss)),
+                  "mlp_train_step")
+        return loss.value
+
+    def set_surrogate(self, p: "SurrogateParams"):
+        self._chk(self.lib.eu3d_mlp_set_surrogate(self.h, C.byref(p)), "set_surrogate")
+        self.params_set = True
+
+    def features(self, rho, T, dt, out_bf16):
+        n = T.shape[0]
+        self._chk(self.lib.eu3d_surrogate_features(self.h, n, _vp(rho), _vp(T), dt, _vp(out_bf16)), "features")
+
+    def targets(self, rho0, rho1, out_f32):
+        n = rho0.shape[1]
+        self._chk(self.lib.eu3d_surrogate_targets(self.h, n, _vp(rho0), _vp(rho1), _vp(out_f32)), "targets")
+
+
+def gemm_bf16(A_bits, B_bits, b_kc: bool, C_out):
+    """C = A * B on device through the surrogate's GEMM kernel (A [M x K] BF16 bits, B [N x K]
+    when b_kc else [K x N], C FP32 [M x N])."""
+    lib = load_library()
+    M, K = A_bits.shape
+    N = B_bits.shape[0] if b_kc else B_bits.shape[1]
+    rc = lib.eu3d_gemm_bf16(M, N, K, _vp(A_bits), _vp(B_bits), 1 if b_kc else 0, _vp(C_out))
+    if rc != 0:
+        raise Eu3dError(rc, "gemm_bf16")

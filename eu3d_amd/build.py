@@ -9,7 +9,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libeu3d.so")
 
-HIP_SOURCES = ["eu3d_kernels.hip", "eu3d_capi.hip"]
+HIP_SOURCES = ["eu3d_kernels.hip", "eu3d_capi.hip", "mlp_kernels.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 # Packed FP32 instructions stay off; FP64 code is unaffected by the flag.

@@ -2,6 +2,7 @@
 // wrapper that validates its arguments and enqueues the kernels of eu3d_kernels.hip.
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -86,6 +87,7 @@ int eu3d_create(eu3d_ctx** out, const eu3d_grid* grid, const eu3d_mech* mech, ui
     g.NX = g.ni + 2 * g.bc; g.NY = g.nj + 2 * g.bc; g.NZ = g.nk + 2 * g.bc;
     g.n3 = (size_t)g.NX * g.NY * g.NZ;
     if (g.n3 >= (1ull << 32)) { delete c; return EU3D_ERR_ARG; }
+    if (const char* v = getenv("EU3D_FLUX_XY_TILE")) c->flux_variant = atoi(v);
     c->dx0 = grid->xnode[1] - grid->xnode[0];
     c->dy0 = grid->ynode[1] - grid->ynode[0];
     c->dz0 = grid->znode[1] - grid->znode[0];
@@ -132,6 +134,7 @@ int eu3d_destroy(eu3d_ctx* c) {
     if (c->part) (void)hipFree(c->part);
     if (c->rk_save) (void)hipFree(c->rk_save);
     if (c->batch_active) (void)hipFree(c->batch_active);
+    if (c->sur_list) (void)hipFree(c->sur_list);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return EU3D_OK;
@@ -307,7 +310,9 @@ int eu3d_step(eu3d_ctx* c, const eu3d_step_params* p, int nsteps) {
     if (p->rk != 1 && p->rk != 2) return fail(c, EU3D_ERR_ARG, "rk must be 1 or 2");
     if (p->rk == 2 && p->chem == EU3D_CHEM_IMEX) return fail(c, EU3D_ERR_UNSUPPORTED, "IMEX couples to the EE stage only");
     if (p->diff < 0 || p->diff > 1 || p->limiter < 0 || p->limiter > 4) return fail(c, EU3D_ERR_ARG, "bad scheme");
+    if (p->chem == EU3D_CHEM_SURROGATE && !c->mlp) return fail(c, EU3D_ERR_ARG, "no surrogate attached");
     for (int n = 0; n < nsteps; n++) HIPCHK(c, launch_step(c, p, n == 0), "step");
+    if (p->chem == EU3D_CHEM_SURROGATE && nsteps > 0) HIPCHK(c, surrogate_grid_stats(c), "surrogate stats");
     if (nsteps > 0) {
         HIPCHK(c, launch_derive(c), "step derive");
         HIPCHK(c, launch_check_finite(c), "step finite");

@@ -289,6 +289,70 @@ __device__ __forceinline__ void ausm_flux(int dir, const Face& L, const Face& R,
     F[NS + 3] = VI * (Dx * Hx + 0.5 * Dx * (Ux * Ux + Vx * Vx + Wx * Wx));
 }
 
+// AUSM face flux from a 4-cell stencil with upwind-only species reconstruction. ld(v, m)
+// returns primitive v (0 P, 1 rho, 2 u, 3 v, 4 w, 5 H, 6 gamma, 7.. Y_s) at stencil position
+// m = 0..3 (cells c-1, c, c+1, c+2 of the face between c and c+1). The seven flow variables are
+// reconstructed on both sides; a species' face state only enters the flux on the side the
+// interface velocity selects (src/Flowfield.cpp:1440-1457), so each species is reconstructed
+// once, on that side, by selecting the MUSCL inputs: the value equals the reference's.
+__device__ __forceinline__ double muscl_side(int limit, bool left, double fm, double f0, double f1, double f2) {
+    const double A1 = f1 - f0;
+    const double base = left ? f0 : f1;
+    const double d = left ? (f0 - fm) : (f2 - f1);
+    const double r = A1 / (d + 1e-12);
+    const double b = (limit == 0) ? mm_bracket(r) : (KM * lim(limit, r) + KP * lim(limit, 1.0 / r) * r);
+    const double t = 0.25 * b * d;
+    return left ? base + t : base - t;
+}
+
+template <int dir, class Ld>
+__device__ __forceinline__ void ausm_stencil(const Ld& ld, int order, int limit, bool zero_y, double (&F)[NC]) {
+    double fl[7], fr[7];
+#pragma unroll
+    for (int v = 0; v < 7; v++) muscl(order, limit, ld(v, 0), ld(v, 1), ld(v, 2), ld(v, 3), fl[v], fr[v]);
+    const double PL = fl[0], DL = fl[1], UL = fl[2], VL = fl[3], WL = fl[4], HL = fl[5], GL = fl[6];
+    const double PR = fr[0], DR = fr[1], UR = fr[2], VR = fr[3], WR = fr[4], HR = fr[5], GR = fr[6];
+    const double CL = sqrt(GL * PL / DL), CR = sqrt(GR * PR / DR);
+    const double CI = 0.5 * (CL + CR);
+    const double unL = dir == 0 ? UL : (dir == 1 ? VL : WL);
+    const double unR = dir == 0 ? UR : (dir == 1 ? VR : WR);
+    const double MaL = unL / CI, MaR = unR / CI;
+    double B1, G1, B2, G2;
+    if (fabs(MaL) < 1.0) {
+        const double a = (MaL + 1.0) * (MaL + 1.0), b = MaL * MaL - 1.0;
+        B1 = 0.25 * a + 0.125 * (b * b);
+        G1 = 0.25 * a * (2.0 - MaL) + 3.0 / 16.0 * MaL * (b * b);
+    } else {
+        B1 = 0.5 * (MaL + fabs(MaL));
+        G1 = (MaL > 0) ? 1.0 : -0.0;
+    }
+    if (fabs(MaR) < 1.0) {
+        const double a = (MaR - 1.0) * (MaR - 1.0), b = MaR * MaR - 1.0;
+        B2 = -0.25 * a - 0.125 * (b * b);
+        G2 = 0.25 * a * (2.0 + MaR) - 3.0 / 16.0 * MaR * (b * b);
+    } else {
+        B2 = 0.5 * (MaR - fabs(MaR));
+        G2 = (MaR > 0) ? 0.0 : 1.0;
+    }
+    const double VI = (B1 + B2) * CI;
+    const bool up = VI >= 0.0;
+    const double Dx = up ? DL : DR, Ux = up ? UL : UR, Vx = up ? VL : VR, Wx = up ? WL : WR;
+    const double Hx = up ? HL : HR;
+    const double m = VI * Dx;
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        double y;
+        if (order == 0) y = up ? ld(7 + s, 1) : ld(7 + s, 2);
+        else y = muscl_side(limit, up, ld(7 + s, 0), ld(7 + s, 1), ld(7 + s, 2), ld(7 + s, 3));
+        F[s] = zero_y ? m * 0.0 : m * y;
+    }
+    F[NS] = m * Ux;
+    F[NS + 1] = m * Vx;
+    F[NS + 2] = m * Wx;
+    F[NS + dir] = F[NS + dir] + G1 * PL + G2 * PR;
+    F[NS + 3] = VI * (Dx * Hx + 0.5 * Dx * (Ux * Ux + Vx * Vx + Wx * Wx));
+}
+
 // ------------------------------------------------------------------ kinetics
 // One pass over the reactions of one cell (the common part of Reaction::Trapezoid,
 // TrapezoidPrediction and Diagonalized; src/Reaction.cpp:420-482,499-588,744-809), accumulating

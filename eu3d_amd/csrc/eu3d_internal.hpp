@@ -20,6 +20,9 @@ struct Fields {
     double* f[EU3D_FIELD_COUNT];
 };
 
+#define IDX(g, i, j, k) ((((size_t)(k)) * (size_t)(g).NY + (size_t)(j)) * (size_t)(g).NX + (size_t)(i))
+#define FLD(F, id) ((F).f[id])
+
 // Device-resident scalars of a context.
 struct Scalars {
     double dt, time, dtm;            // dtm: chemistry interval of the current adaptive call
@@ -50,6 +53,11 @@ struct eu3d_ctx {
     unsigned int* batch_active = nullptr;
     long long batch_cap = 0;
     eu3d_chem_schedule sched{256, 4, 1};
+    int flux_variant = 0;                                          // tile shape of the x/y advection pass
+    struct eu3d_mlp* mlp = nullptr;                                // surrogate used by EU3D_CHEM_SURROGATE
+    unsigned* sur_list = nullptr;                                  // fallback list, work list, counters, keep
+    long long sur_cap = 0;
+    eu3d_surrogate_stats sur_stats{};
     eu3d_chem_stats stats{};
     std::string err;
 };
@@ -77,6 +85,13 @@ hipError_t launch_trap_adaptive(eu3d_ctx* c, int nchem_max, bool predict_only);
 hipError_t launch_check_finite(eu3d_ctx* c);
 hipError_t launch_set_scalars(eu3d_ctx* c, double dt, double time);
 hipError_t launch_step(eu3d_ctx* c, const eu3d_step_params* p, bool first);
+hipError_t launch_surrogate_grid(eu3d_ctx* c, int nchem_max);
+hipError_t surrogate_grid_stats(eu3d_ctx* c);
+hipError_t launch_trap_list(eu3d_ctx* c, const unsigned* list, const unsigned* count, unsigned max_count,
+                            unsigned* work, int nchem_max);
+hipError_t launch_batch_list(eu3d_ctx* c, long long n, double* rho, double* T, double* E, double dtm,
+                             const unsigned* list, const unsigned* count, unsigned max_count, unsigned* work,
+                             int* nchem_keep, int nchem_max);
 hipError_t launch_chem_batch(eu3d_ctx* c, long long n, double* rho, double* T, double* E, double dtm,
                              int nchem_max, int* nchem_out);
 }  // namespace eu3d

@@ -8,8 +8,6 @@
 
 using namespace eu3d;
 
-#define IDX(g, i, j, k) ((((size_t)(k)) * (size_t)(g).NY + (size_t)(j)) * (size_t)(g).NX + (size_t)(i))
-#define FLD(F, id) ((F).f[id])
 
 namespace {
 
@@ -412,96 +410,145 @@ __global__ void __launch_bounds__(256) k_explicit(G3 g, Fields F, uint32_t q) {
     }
 }
 
-// ---------------------------------------------------------------- fused flux
-// Flowfield::Advection + TimeAdv::EE in one pass (src/Flowfield.cpp:1096-1128,
-// src/TimeAdv.cpp:29-43). A block of 64 x 8 threads owns a 64 x 8 column of cells and marches
-// through a chunk of FZ planes in z. Per plane every thread computes the x face on its left and
-// the y face below it and shares them through LDS, so each face is evaluated once; the z face
-// above a cell is computed by the cell's own thread and the one below is carried in registers
-// from the previous plane. The first 63 x 7 cells of the column are written (the last lane and
-// row only supply faces). RHS is accumulated in EE's order.
-constexpr int FX = 64, FY = 8, FZ = 16;
+// ---------------------------------------------------------------- advection, two passes
+// The RHS of Flowfield::Advection + TimeAdv::EE in two streaming passes with no face arrays:
+//   k_flux_z (first): one thread per (x, y) column marching up a chunk of z planes with a
+//     4-plane register window (each primitive is read from HBM once, the next plane prefetched);
+//     it stores the z term dt/dz dQ into RHS without reading it.
+//   k_flux_xy (second): one z plane per block. A 64 x 6 thread block stages the primitives of its
+//     tile plus the two-cell halo (67 x 9 cells x 16 variables, 77 KB, two blocks per CU) in LDS
+//     with coalesced loads, computes the x face left of and the y face below every cell from
+//     LDS, exchanges the face fluxes through the same LDS and writes
+//     RHS = (-dt/dx dF - dt/dy dG) - dt/dz dQ for 63 x 5 cells, EE's order of operations.
+constexpr int NV = 16;
+constexpr int ZX = 64, ZY = 4, ZCH = 32;
 
-template <int dir>
-__device__ __forceinline__ void face_flux_at(const G3& g, const Fields& F, size_t c0, int order, int limiter,
-                                             bool zero_y, double (&fl)[NC]) {
-    const size_t st = dir == 0 ? 1 : (dir == 1 ? (size_t)g.NX : (size_t)g.NX * g.NY);
-    Face L, R;
-    auto rec = [&](const double* f, double& l, double& r) {
-        muscl(order, limiter, f[c0 - st], f[c0], f[c0 + st], f[c0 + 2 * st], l, r);
-    };
-    rec(FLD(F, EU3D_FIELD_P), L.P, R.P);
-    rec(FLD(F, EU3D_FIELD_D), L.D, R.D);
-    rec(FLD(F, EU3D_FIELD_U), L.U, R.U);
-    rec(FLD(F, EU3D_FIELD_V), L.V, R.V);
-    rec(FLD(F, EU3D_FIELD_W), L.W, R.W);
-    rec(FLD(F, EU3D_FIELD_H), L.H, R.H);
-    rec(FLD(F, EU3D_FIELD_GAMMA), L.G, R.G);
-    const double* Y = FLD(F, EU3D_FIELD_YI);
-#pragma unroll
-    for (int s = 0; s < NS; s++) rec(Y + s * g.n3, L.Y[s], R.Y[s]);
-    ausm_flux(dir, L, R, zero_y, fl);
+__device__ __forceinline__ const double* prim_ptr(const Fields& F, const G3& g, int v) {
+    switch (v) {
+    case 0: return FLD(F, EU3D_FIELD_P);
+    case 1: return FLD(F, EU3D_FIELD_D);
+    case 2: return FLD(F, EU3D_FIELD_U);
+    case 3: return FLD(F, EU3D_FIELD_V);
+    case 4: return FLD(F, EU3D_FIELD_W);
+    case 5: return FLD(F, EU3D_FIELD_H);
+    case 6: return FLD(F, EU3D_FIELD_GAMMA);
+    default: return FLD(F, EU3D_FIELD_YI) + (size_t)(v - 7) * g.n3;
+    }
 }
 
-__global__ void __launch_bounds__(512) k_flux_fused(G3 g, Fields F, int order, int limiter, uint32_t q,
-                                                    const double* __restrict__ xn, const double* __restrict__ yn,
-                                                    const double* __restrict__ zn, const Scalars* __restrict__ sc) {
-    __shared__ double sf[NC][FX * FY];
-    const int tx = threadIdx.x, ty = threadIdx.y, t = ty * FX + tx;
-    const int x = g.bc + blockIdx.x * (FX - 1) + tx;
-    const int y = g.bc + blockIdx.y * (FY - 1) + ty;
-    const int z0 = g.bc + blockIdx.z * FZ;
-    const int z1 = min(z0 + FZ, g.nk + g.bc);
+template <int XY_X, int XY_Y>
+__global__ void __launch_bounds__(XY_X * XY_Y) k_flux_xy(G3 g, Fields F, int order, int limiter, uint32_t q,
+                                                 const double* __restrict__ xn, const double* __restrict__ yn,
+                                                 const Scalars* __restrict__ sc) {
+    constexpr int XY_TX = XY_X + 3, XY_TY = XY_Y + 3;
+    __shared__ double sp[NV][XY_TY][XY_TX];
+    const int tx = threadIdx.x, ty = threadIdx.y, t = ty * XY_X + tx;
+    const int X0 = g.bc + blockIdx.x * (XY_X - 1), Y0 = g.bc + blockIdx.y * (XY_Y - 1);
+    const int z = g.bc + blockIdx.z;
+    const int x = X0 + tx, y = Y0 + ty;
     const int xe = g.ni + g.bc, ye = g.nj + g.bc;
-    const bool out = tx < FX - 1 && ty < FY - 1 && x < xe && y < ye;
-    const bool xface = x <= xe && y < ye;   // face between x-1 and x
-    const bool yface = x < xe && y <= ye;   // face between y-1 and y
-    const bool f1b = q & Q_F1B;
-    const double dt = sc->dt;
-    double cx = 0, cy = 0;
+    const bool out = tx < XY_X - 1 && ty < XY_Y - 1 && x < xe && y < ye;
+    double* Rh = FLD(F, EU3D_FIELD_RHS);
+    const size_t cc = out ? IDX(g, x, y, z) : 0;
+    double zt[NC];   // z term written by k_flux_z, fetched while the tile is staged
     if (out) {
-        cx = -dt / (xn[x] - xn[x - 1]);
-        cy = dt / (yn[y] - yn[y - 1]);
+#pragma unroll
+        for (int c = 0; c < NC; c++) zt[c] = Rh[c * g.n3 + cc];
     }
-    const size_t sz = (size_t)g.NX * g.NY;
-    double fzb[NC];
-    if (out) face_flux_at<2>(g, F, IDX(g, x, y, z0 - 1), order, limiter, f1b && x >= g.ni, fzb);
+    // stage the tile: column a <-> x = X0 - 2 + a, row b <-> y = Y0 - 2 + b
+    for (int idx = t; idx < XY_TX * XY_TY; idx += XY_X * XY_Y) {
+        const int a = idx % XY_TX, b = idx / XY_TX;
+        const int gx = X0 - 2 + a, gy = Y0 - 2 + b;
+        if (gx < g.NX && gy < g.NY) {
+            const size_t c = IDX(g, gx, gy, z);
+#pragma unroll
+            for (int v = 0; v < NV; v++) sp[v][b][a] = prim_ptr(F, g, v)[c];
+        }
+    }
+    __syncthreads();
+    const bool xface = ty < XY_Y - 1 && x <= xe && y < ye;
+    const bool yface = tx < XY_X - 1 && x < xe && y <= ye;
+    const bool f1b = q & Q_F1B;
+    double fx[NC], fy[NC];
+    if (xface) {   // face between x-1 and x: columns x-2 .. x+1 = a tx .. tx+3, row ty+2
+        auto ld = [&](int v, int m) { return sp[v][ty + 2][tx + m]; };
+        ausm_stencil<0>(ld, order, limiter, f1b && x - 1 >= g.ni, fx);
+    }
+    if (yface) {   // face between y-1 and y: rows ty .. ty+3, column tx+2
+        auto ld = [&](int v, int m) { return sp[v][ty + m][tx + 2]; };
+        ausm_stencil<1>(ld, order, limiter, f1b && x >= g.ni, fy);
+    }
+    __syncthreads();
+    double* sf = &sp[0][0][0];   // reuse the staging area for the face-flux exchange
+    if (xface) {
+#pragma unroll
+        for (int c = 0; c < NC; c++) sf[c * (XY_X * XY_Y) + t] = fx[c];
+    }
+    __syncthreads();
+    double rhs[NC];
+    const double dt = sc->dt;
+    if (out) {
+        const double cx = -dt / (xn[x] - xn[x - 1]);
+#pragma unroll
+        for (int c = 0; c < NC; c++) rhs[c] = cx * (sf[c * (XY_X * XY_Y) + t + 1] - sf[c * (XY_X * XY_Y) + t]);
+    }
+    __syncthreads();
+    if (yface) {
+#pragma unroll
+        for (int c = 0; c < NC; c++) sf[c * (XY_X * XY_Y) + t] = fy[c];
+    }
+    __syncthreads();
+    if (out) {
+        const double cy = dt / (yn[y] - yn[y - 1]);
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            rhs[c] -= cy * (sf[c * (XY_X * XY_Y) + t + XY_X] - sf[c * (XY_X * XY_Y) + t]);
+            Rh[c * g.n3 + cc] = rhs[c] - zt[c];
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) k_flux_z(G3 g, Fields F, int order, int limiter, uint32_t q,
+                                                const double* __restrict__ zn, const Scalars* __restrict__ sc) {
+    const int x = g.bc + blockIdx.x * ZX + threadIdx.x, y = g.bc + blockIdx.y * ZY + threadIdx.y;
+    if (x >= g.ni + g.bc || y >= g.nj + g.bc) return;
+    const int z0 = g.bc + blockIdx.z * ZCH, z1 = min(z0 + ZCH, g.nk + g.bc);
+    const size_t base = IDX(g, x, y, 0), sz = (size_t)g.NX * g.NY;
+    const bool zy = (q & Q_F1B) && x >= g.ni;
+    double w[NV][4], nxt[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+        const double* f = prim_ptr(F, g, v) + base;
+#pragma unroll
+        for (int m = 0; m < 4; m++) w[v][m] = f[(size_t)(z0 - 2 + m) * sz];
+        nxt[v] = f[(size_t)(z0 + 2) * sz];
+    }
+    auto ld = [&](int v, int m) { return w[v][m]; };
+    double fb[NC], ft[NC];
+    ausm_stencil<2>(ld, order, limiter, zy, fb);
+    const double dt = sc->dt;
     double* Rh = FLD(F, EU3D_FIELD_RHS);
     for (int z = z0; z < z1; z++) {
-        double rhs[NC], f[NC];
-        if (xface) {
-            face_flux_at<0>(g, F, IDX(g, x - 1, y, z), order, limiter, f1b && x - 1 >= g.ni, f);
 #pragma unroll
-            for (int c = 0; c < NC; c++) sf[c][t] = f[c];
+        for (int v = 0; v < NV; v++) {
+            w[v][0] = w[v][1];
+            w[v][1] = w[v][2];
+            w[v][2] = w[v][3];
+            w[v][3] = nxt[v];
         }
-        __syncthreads();
-        if (out) {
+        if (z + 1 < z1) {
 #pragma unroll
-            for (int c = 0; c < NC; c++) rhs[c] = cx * (sf[c][t + 1] - sf[c][t]);
+            for (int v = 0; v < NV; v++) nxt[v] = prim_ptr(F, g, v)[base + (size_t)(z + 3) * sz];
         }
-        __syncthreads();
-        if (yface) {
-            face_flux_at<1>(g, F, IDX(g, x, y - 1, z), order, limiter, f1b && x >= g.ni, f);
+        ausm_stencil<2>(ld, order, limiter, zy, ft);
+        const double cz = dt / (zn[z] - zn[z - 1]);
+        const size_t cc = base + (size_t)z * sz;
 #pragma unroll
-            for (int c = 0; c < NC; c++) sf[c][t] = f[c];
+        for (int c = 0; c < NC; c++) {
+            Rh[c * g.n3 + cc] = cz * (ft[c] - fb[c]);
+            fb[c] = ft[c];
         }
-        __syncthreads();
-        if (out) {
-#pragma unroll
-            for (int c = 0; c < NC; c++) rhs[c] -= cy * (sf[c][t + FX] - sf[c][t]);
-            const double cz = dt / (zn[z] - zn[z - 1]);
-            face_flux_at<2>(g, F, IDX(g, x, y, z), order, limiter, f1b && x >= g.ni, f);
-            const size_t cc = IDX(g, x, y, z);
-#pragma unroll
-            for (int c = 0; c < NC; c++) {
-                rhs[c] -= cz * (f[c] - fzb[c]);
-                fzb[c] = f[c];
-                Rh[c * g.n3 + cc] = rhs[c];
-            }
-        }
-        __syncthreads();
     }
-    (void)sz;
 }
 
 // ---------------------------------------------------------------- fused IMEX
@@ -510,7 +557,7 @@ __global__ void __launch_bounds__(512) k_flux_fused(G3 g, Fields F, int order, i
 // registers. The inputs of Diagonalized (Mc, Di, dT/drho_s) are recomputed from the state at
 // the start of the step (CS, T) instead of being read back; the outputs are the new CS, T, E and
 // the primitives the next advection and CFL bound read. Each block writes its CFL partial.
-__global__ void __launch_bounds__(256) k_imex_fused(G3 g, Fields F, uint32_t q, const Scalars* __restrict__ sc,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) k_imex_fused(G3 g, Fields F, uint32_t q, const Scalars* __restrict__ sc,
                                                     double dx, double dy, double dz, double* part) {
     int i, j, k;
     double v = 1e10;
@@ -545,6 +592,7 @@ __global__ void __launch_bounds__(256) k_imex_fused(G3 g, Fields F, uint32_t q, 
         for (int s = 0; s < NS; s++) pt[s] *= tsc;
         KinOut ko;
         kinetics<false, true>(T0, Mc, Dr, pt, 2, q & Q_C4, ko);
+        __builtin_amdgcn_sched_barrier(0);
         const double dt = sc->dt;
         const double* R = FLD(F, EU3D_FIELD_RHS);
 #pragma unroll
@@ -968,6 +1016,85 @@ __global__ void __launch_bounds__(256) k_batch_first(BatchCells acc, int* nchem_
     chem_count(sc, in ? nch : 0, active, list, (unsigned)id);
 }
 
+// Prediction + first sub-step for the cells of a list (the surrogate's fallback cells), grid
+// and batch forms; cells needing more sub-steps are compacted into out_list as above.
+template <bool C3>
+__global__ void __launch_bounds__(256) k_trap_first_list(G3 g, Fields F, uint32_t q, const unsigned* __restrict__ in_list,
+                                                         const unsigned* __restrict__ in_count, Scalars* sc,
+                                                         unsigned* out_list, int nmax) {
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool in = t < *in_count;
+    const size_t c = in ? in_list[t] : 0;
+    int nch = 0;
+    bool active = false;
+    if (in) {
+        ChemCell cell;
+#pragma unroll
+        for (int s = 0; s < NS; s++) cell.Di[s] = FLD(F, EU3D_FIELD_DI)[s * g.n3 + c];
+        cell.U = FLD(F, EU3D_FIELD_U)[c];
+        cell.V = FLD(F, EU3D_FIELD_V)[c];
+        cell.W = FLD(F, EU3D_FIELD_W)[c];
+        cell.E = FLD(F, EU3D_FIELD_E)[c];
+        cell.T = FLD(F, EU3D_FIELD_T)[c];
+        double Mc[NS], Dr[NS], Y[NS];
+        react_units(F, g, c, Mc, Dr);
+#pragma unroll
+        for (int s = 0; s < NS; s++) Y[s] = FLD(F, EU3D_FIELD_YI)[s * g.n3 + c];
+        KinOut ko;
+        kinetics<true, false, C3>(cell.T, Mc, Dr, Dr, (q & Q_C2) ? 1 : 2, q & Q_C4, ko);
+        const double dtm = sc->dtm;
+        nch = chem_nchem(ko, Dr, Y, dtm, nmax);
+        FLD(F, EU3D_FIELD_NCHEM)[c] = (double)nch;
+        if (nch <= 0) atomicOr(&sc->nonfinite, 2);
+        if (nch > 0) {
+            double cs[NC];
+            const Prim o = chem_apply(cell, ko, Dr, dtm / nch, q, cs);
+            GridCells acc{g, F};
+            acc.store((unsigned)c, cs, o, ko);
+            active = nch > 1;
+        }
+    }
+    chem_count(sc, in ? nch : 0, active, out_list, (unsigned)c);
+}
+
+template <bool C3>
+__global__ void __launch_bounds__(256) k_batch_first_list(BatchCells acc, int* nchem_keep, uint32_t q, double dtm, int nmax,
+                                                          const unsigned* __restrict__ in_list,
+                                                          const unsigned* __restrict__ in_count, Scalars* sc,
+                                                          unsigned* out_list) {
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool in = t < *in_count;
+    const long long id = in ? in_list[t] : 0;
+    int nch = 0;
+    bool active = false;
+    if (in) {
+        ChemCell cell;
+#pragma unroll
+        for (int s = 0; s < NS; s++) cell.Di[s] = acc.rho[(size_t)s * acc.n + id];
+        cell.U = cell.V = cell.W = 0.0;
+        cell.E = acc.E[id];
+        cell.T = acc.T[id];
+        double Mc[NS], Dr[NS], Y[NS], D = 0;
+        chem_inputs(cell, Mc, Dr);
+#pragma unroll
+        for (int s = 0; s < NS; s++) D += cell.Di[s];
+#pragma unroll
+        for (int s = 0; s < NS; s++) Y[s] = cell.Di[s] / D;
+        KinOut ko;
+        kinetics<true, false, C3>(cell.T, Mc, Dr, Dr, (q & Q_C2) ? 1 : 2, q & Q_C4, ko);
+        nch = chem_nchem(ko, Dr, Y, dtm, nmax);
+        nchem_keep[id] = nch;
+        if (nch <= 0) atomicOr(&sc->nonfinite, 2);
+        if (nch > 0) {
+            double cs[NC];
+            const Prim o = chem_apply(cell, ko, Dr, dtm / nch, q, cs);
+            acc.store((unsigned)id, cs, o, ko);
+            active = nch > 1;
+        }
+    }
+    chem_count(sc, in ? nch : 0, active, out_list, (unsigned)id);
+}
+
 __global__ void k_chem_reset(Scalars* sc, double* dtm_slot, double dtm, int use_dt) {
     sc->chem_head = 0;
     sc->chem_active = 0;
@@ -1049,11 +1176,29 @@ hipError_t launch_cfl(eu3d_ctx* c, double cfl, double final_time) {
     return hipGetLastError();
 }
 
+template <int X, int Y>
+static void launch_xy(eu3d_ctx* c, int diff, int limiter) {
+    const G3& g = c->g;
+    dim3 gxy((g.ni + X - 2) / (X - 1), (g.nj + Y - 2) / (Y - 1), g.nk);
+    k_flux_xy<X, Y><<<gxy, dim3(X, Y), 0, c->stream>>>(g, c->fl, diff, limiter, c->quirks, c->xnode, c->ynode, c->sc);
+}
+
 hipError_t launch_flux_fused(eu3d_ctx* c, int diff, int limiter) {
     const G3& g = c->g;
-    dim3 grid((g.ni + FX - 2) / (FX - 1), (g.nj + FY - 2) / (FY - 1), (g.nk + FZ - 1) / FZ);
-    k_flux_fused<<<grid, dim3(FX, FY), 0, c->stream>>>(g, c->fl, diff, limiter, c->quirks, c->xnode, c->ynode,
-                                                        c->znode, c->sc);
+    dim3 gz((g.ni + ZX - 1) / ZX, (g.nj + ZY - 1) / ZY, (g.nk + ZCH - 1) / ZCH);
+    k_flux_z<<<gz, dim3(ZX, ZY), 0, c->stream>>>(g, c->fl, diff, limiter, c->quirks, c->znode, c->sc);
+    // tile shape of the x/y pass (threads in x, y); 32 x 8 measured fastest on bubble3d
+    // 384x128x128 among 64x8, 64x6, 64x4, 32x8, 32x4 (the others stay selectable)
+    switch (c->flux_variant) {
+    case 1: launch_xy<64, 8>(c, diff, limiter); break;
+    case 2: launch_xy<64, 4>(c, diff, limiter); break;
+    case 3: launch_xy<64, 6>(c, diff, limiter); break;
+    case 4: launch_xy<32, 4>(c, diff, limiter); break;
+    case 5: launch_xy<16, 16>(c, diff, limiter); break;
+    case 6: launch_xy<32, 12>(c, diff, limiter); break;
+    case 7: launch_xy<16, 8>(c, diff, limiter); break;
+    default: launch_xy<32, 8>(c, diff, limiter);
+    }
     return hipGetLastError();
 }
 
@@ -1164,6 +1309,52 @@ hipError_t launch_chem_batch(eu3d_ctx* c, long long n, double* rho, double* T, d
     return hipGetLastError();
 }
 
+// FP64 adaptive trapezoid on the cells of a device list (count on the device), grid form;
+// the chemistry interval is the context's dt. max_count bounds the launch.
+hipError_t launch_trap_list(eu3d_ctx* c, const unsigned* list, const unsigned* count, unsigned max_count,
+                            unsigned* work, int nchem_max) {
+    const int nmax = nchem_max > 0 ? nchem_max : 10000;
+    k_chem_reset<<<1, 1, 0, c->stream>>>(c->sc, &c->sc->dtm, 0.0, 1);
+    const unsigned nb = (max_count + 255) / 256;
+    const bool c3 = c->quirks & Q_C3;
+    if (nb) {
+        if (c3)
+            k_trap_first_list<true><<<nb, 256, 0, c->stream>>>(G(c), c->fl, c->quirks, list, count, c->sc, work, nmax);
+        else
+            k_trap_first_list<false><<<nb, 256, 0, c->stream>>>(G(c), c->fl, c->quirks, list, count, c->sc, work, nmax);
+    }
+    GridCells acc{c->g, c->fl};
+    const int nbp = c->sched.blocks_per_cu * c->num_cu;
+    if (c3)
+        k_chem_persist<GridCells, true><<<nbp, c->sched.block, 0, c->stream>>>(acc, work, c->sc, c->quirks, &c->sc->dtm);
+    else
+        k_chem_persist<GridCells, false><<<nbp, c->sched.block, 0, c->stream>>>(acc, work, c->sc, c->quirks, &c->sc->dtm);
+    return hipGetLastError();
+}
+
+// Batch form of launch_trap_list: flat cells at rest (rho [ns][n], T, E).
+hipError_t launch_batch_list(eu3d_ctx* c, long long n, double* rho, double* T, double* E, double dtm,
+                             const unsigned* list, const unsigned* count, unsigned max_count, unsigned* work,
+                             int* nchem_keep, int nchem_max) {
+    const int nmax = nchem_max > 0 ? nchem_max : 10000;
+    k_chem_reset<<<1, 1, 0, c->stream>>>(c->sc, &c->sc->dtm, dtm, 0);
+    BatchCells acc{n, rho, T, E, nchem_keep};
+    const unsigned nb = (max_count + 255) / 256;
+    const bool c3 = c->quirks & Q_C3;
+    if (nb) {
+        if (c3)
+            k_batch_first_list<true><<<nb, 256, 0, c->stream>>>(acc, nchem_keep, c->quirks, dtm, nmax, list, count, c->sc, work);
+        else
+            k_batch_first_list<false><<<nb, 256, 0, c->stream>>>(acc, nchem_keep, c->quirks, dtm, nmax, list, count, c->sc, work);
+    }
+    const int nbp = c->sched.blocks_per_cu * c->num_cu;
+    if (c3)
+        k_chem_persist<BatchCells, true><<<nbp, c->sched.block, 0, c->stream>>>(acc, work, c->sc, c->quirks, &c->sc->dtm);
+    else
+        k_chem_persist<BatchCells, false><<<nbp, c->sched.block, 0, c->stream>>>(acc, work, c->sc, c->quirks, &c->sc->dtm);
+    return hipGetLastError();
+}
+
 hipError_t launch_step(eu3d_ctx* c, const eu3d_step_params* p, bool first) {
     hipError_t e;
     const bool imex = p->rk == 1 && p->chem == EU3D_CHEM_IMEX;
@@ -1190,6 +1381,7 @@ hipError_t launch_step(eu3d_ctx* c, const eu3d_step_params* p, bool first) {
             e = launch_explicit(c);
             break;
         case EU3D_CHEM_TRAP_ADAPTIVE: e = launch_trap_adaptive(c, p->nchem_max, false); break;
+        case EU3D_CHEM_SURROGATE: e = launch_surrogate_grid(c, p->nchem_max); break;
         default: e = hipSuccess;
         }
         return e;
@@ -1205,6 +1397,10 @@ hipError_t launch_step(eu3d_ctx* c, const eu3d_step_params* p, bool first) {
     case EU3D_CHEM_TRAP_ADAPTIVE:
         if ((e = launch_update_after_adv(c)) != hipSuccess) return e;
         e = launch_trap_adaptive(c, p->nchem_max, false);
+        break;
+    case EU3D_CHEM_SURROGATE:
+        if ((e = launch_update_after_adv(c)) != hipSuccess) return e;
+        e = launch_surrogate_grid(c, p->nchem_max);
         break;
     default: e = launch_update_after_adv(c);
     }

@@ -1,0 +1,890 @@
+// MLP chemistry surrogate on gfx950: BF16 MFMA GEMMs with FP32 accumulation, fused epilogues,
+// backward pass and Adam; plus the features, apply and fallback kernels that couple it to the
+// FP64 state. This part has no counterpart in the reference (its "DNN" scheme is a stub that
+// only advects, src/Euler.cpp:773-780).
+//
+// Network: x (12 features, padded to 32) -> 4 x [Linear 256 + GELU] -> Linear 9 (padded to 16).
+// Storage: activations row-major [rows x features] in BF16; weights kept as FP32 masters in
+// "out x in" layout (Wt) with BF16 copies in both layouts (Wt and W) so every GEMM reads both
+// operands K-contiguous.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "eu3d_internal.hpp"
+
+namespace {
+
+typedef uint16_t bf16_t;
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+__host__ __device__ __forceinline__ bf16_t f2bf(float f) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    if ((u & 0x7F800000u) == 0x7F800000u) return (bf16_t)((u >> 16) | ((u & 0xFFFF) ? 0x40 : 0));
+    u += 0x7FFFu + ((u >> 16) & 1u);   // round to nearest even
+    return (bf16_t)(u >> 16);
+}
+__host__ __device__ __forceinline__ float bf2f(bf16_t h) {
+    uint32_t u = (uint32_t)h << 16;
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+}
+
+__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+__device__ __forceinline__ float dgelu(float z) {
+    return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * expf(-0.5f * z * z);
+}
+
+// ------------------------------------------------------------------ GEMM
+// C[M x N] = A[M x K] * B[K x N]. AKC: A stored K-contiguous (A[m * lda + k]) else
+// M-contiguous (A[k * lda + m]); BKC: B stored K-contiguous (B[n * ldb + k]) else (B[k * ldb + n]).
+// Block tile 128 x 128 x 32, four waves in 2 x 2, each wave 64 x 64 = 4 x 4 MFMA 16x16x32 tiles.
+// Tiles are staged in LDS as [row][k] (K-contiguous, padded pitch) so a lane's MFMA fragment,
+// 8 consecutive k of one row, is one 16-byte LDS read. Split-K over gridDim.z writes partials.
+constexpr int BM = 128, BN = 128, BK = 32, LDK = BK + 8;
+
+enum Epi { EPI_F32 = 0, EPI_BIAS_F32 = 1, EPI_BIAS_GELU = 2, EPI_DGELU = 3 };
+
+struct GemmArgs {
+    int M, N, K, kchunk;
+    const bf16_t* A;
+    int lda;
+    const bf16_t* B;
+    int ldb;
+    float* Cf;            // EPI_F32 / EPI_BIAS_F32 output (split s at Cf + s * M * ldc)
+    bf16_t* Cb;           // EPI_BIAS_GELU activation / EPI_DGELU gradient output
+    bf16_t* Z;            // EPI_BIAS_GELU: pre-activation store (may be null); EPI_DGELU: read
+    const float* bias;
+    int ldc;
+};
+
+template <bool KC, int ROWS>
+__device__ __forceinline__ void stage(bf16_t (*S)[LDK], const bf16_t* G, int ld, int r0, int nrows, int k0, int kend,
+                                      int tid) {
+    if (KC) {   // G[r * ld + k]; 8-element chunks along k
+        for (int c = tid; c < ROWS * (BK / 8); c += 256) {
+            const int r = c / (BK / 8), kc = (c % (BK / 8)) * 8;
+            const int gr = r0 + r, gk = k0 + kc;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (gr < nrows) {
+                if (gk + 8 <= kend && ((((uintptr_t)(G + (size_t)gr * ld + gk)) & 15) == 0)) {
+                    v = *reinterpret_cast<const uint4*>(G + (size_t)gr * ld + gk);
+                } else {
+                    bf16_t tmp[8];
+                    for (int e = 0; e < 8; e++) tmp[e] = (gk + e < kend) ? G[(size_t)gr * ld + gk + e] : (bf16_t)0;
+                    memcpy(&v, tmp, 16);
+                }
+            }
+            *reinterpret_cast<uint4*>(&S[r][kc]) = v;
+        }
+    } else {    // G[k * ld + r]; 8-element chunks along r, transposed into S[r][k]
+        for (int c = tid; c < BK * (ROWS / 8); c += 256) {
+            const int k = c / (ROWS / 8), rc = (c % (ROWS / 8)) * 8;
+            const int gk = k0 + k, gr = r0 + rc;
+            bf16_t tmp[8];
+            if (gk < kend && gr + 8 <= nrows && ((((uintptr_t)(G + (size_t)gk * ld + gr)) & 15) == 0)) {
+                uint4 v = *reinterpret_cast<const uint4*>(G + (size_t)gk * ld + gr);
+                memcpy(tmp, &v, 16);
+            } else {
+                for (int e = 0; e < 8; e++) tmp[e] = (gk < kend && gr + e < nrows) ? G[(size_t)gk * ld + gr + e] : (bf16_t)0;
+            }
+            for (int e = 0; e < 8; e++) S[rc + e][k] = tmp[e];
+        }
+    }
+}
+
+template <bool AKC, bool BKC, int EPI>
+__global__ void __launch_bounds__(256) k_gemm(GemmArgs a) {
+    __shared__ bf16_t As[BM][LDK];
+    __shared__ bf16_t Bs[BN][LDK];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+    const int kbeg = blockIdx.z * a.kchunk, kend = min(a.K, kbeg + a.kchunk);
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+    const int fr = lane & 15, fk = (lane >> 4) * 8;
+    f32x4_t acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = (f32x4_t){0.f, 0.f, 0.f, 0.f};
+    for (int k0 = kbeg; k0 < kend; k0 += BK) {
+        stage<AKC, BM>(As, a.A, a.lda, m0, a.M, k0, kend, tid);
+        stage<BKC, BN>(Bs, a.B, a.ldb, n0, a.N, k0, kend, tid);
+        __syncthreads();
+        bf16x8_t af[4], bfr[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) af[i] = *reinterpret_cast<const bf16x8_t*>(&As[wm + 16 * i + fr][fk]);
+#pragma unroll
+        for (int j = 0; j < 4; j++) bfr[j] = *reinterpret_cast<const bf16x8_t*>(&Bs[wn + 16 * j + fr][fk]);
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+        __syncthreads();
+    }
+    // D fragment: lane holds rows 4 * (lane / 16) + r, column lane % 16 of each 16 x 16 tile
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int row = m0 + wm + 16 * i + 4 * (lane >> 4) + r;
+                const int col = n0 + wn + 16 * j + (lane & 15);
+                if (row >= a.M || col >= a.N) continue;
+                const float v = acc[i][j][r];
+                const size_t o = (size_t)row * a.ldc + col;
+                if (EPI == EPI_F32) {
+                    a.Cf[(size_t)blockIdx.z * a.M * a.ldc + o] = v;
+                } else if (EPI == EPI_BIAS_F32) {
+                    a.Cf[o] = v + a.bias[col];
+                } else if (EPI == EPI_BIAS_GELU) {
+                    const float z = v + a.bias[col];
+                    const bf16_t zb = f2bf(z);
+                    if (a.Z) a.Z[o] = zb;
+                    a.Cb[o] = f2bf(gelu(bf2f(zb)));
+                } else {   // EPI_DGELU: gradient wrt the pre-activation of the previous layer
+                    a.Cb[o] = f2bf(v * dgelu(bf2f(a.Z[o])));
+                }
+            }
+}
+
+// Sum split-K partials: out[i] = sum_s part[s * n + i]
+__global__ void k_reduce_splits(const float* __restrict__ part, float* __restrict__ out, int n, int splits) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float s = 0.f;
+    for (int k = 0; k < splits; k++) s += part[(size_t)k * n + i];
+    out[i] = s;
+}
+
+// Column sums of a bf16 [rows x ld] matrix (bias gradients), deterministic two-level:
+// block b sums a row range per column into part[b * ncol + col].
+__global__ void k_colsum_bf16(const bf16_t* __restrict__ X, int rows, int ld, int ncol, int rows_per_block,
+                              float* __restrict__ part) {
+    const int col = threadIdx.x + blockIdx.y * blockDim.x;
+    if (col >= ncol) return;
+    const int r0 = blockIdx.x * rows_per_block, r1 = min(rows, r0 + rows_per_block);
+    float s = 0.f;
+    for (int r = r0; r < r1; r++) s += bf2f(X[(size_t)r * ld + col]);
+    part[(size_t)blockIdx.x * ncol + col] = s;
+}
+
+// Loss gradient: dY = 2 (Y - T) / (n * nout) on the nout real outputs, zero on the padding;
+// per-block partial sums of the squared error.
+__global__ void k_mse_grad(const float* __restrict__ Y, const float* __restrict__ T, int n, int ld, int nout,
+                           float scale, bf16_t* __restrict__ dY, float* __restrict__ part) {
+    __shared__ float red[256];
+    float s = 0.f;
+    const size_t total = (size_t)n * ld;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const int col = (int)(t % ld);
+        float g = 0.f;
+        if (col < nout) {
+            const float d = Y[t] - T[t];
+            s += d * d;
+            g = scale * d;
+        }
+        dY[t] = f2bf(g);
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// Adam on FP32 masters (Wt layout [out x in] and bias [out]); refreshes the BF16 copies Wt and
+// W ([in x out]). Padded rows/columns stay zero.
+__global__ void k_adam(float* __restrict__ w, float* __restrict__ m, float* __restrict__ v, const float* __restrict__ g,
+                       int n_out, int n_in, int real_out, int real_in, float lr, float b1, float b2, float eps,
+                       float bc1, float bc2, float wd, bf16_t* __restrict__ wt_b, bf16_t* __restrict__ w_b) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_out * n_in) return;
+    const int o = i / n_in, k = i % n_in;
+    if (o >= real_out || k >= real_in) {
+        w[i] = 0.f;
+        wt_b[i] = 0;
+        if (w_b) w_b[(size_t)k * n_out + o] = 0;
+        return;
+    }
+    const float gi = g[i] + wd * w[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float wi = w[i] - lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+    w[i] = wi;
+    const bf16_t h = f2bf(wi);
+    wt_b[i] = h;
+    if (w_b) w_b[(size_t)k * n_out + o] = h;
+}
+
+__global__ void k_cast_weights(const float* __restrict__ w, int n_out, int n_in, bf16_t* __restrict__ wt_b,
+                               bf16_t* __restrict__ w_b) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_out * n_in) return;
+    const int o = i / n_in, k = i % n_in;
+    const bf16_t h = f2bf(w[i]);
+    wt_b[i] = h;
+    w_b[(size_t)k * n_out + o] = h;
+}
+
+__global__ void k_f32_to_bf16(const float* __restrict__ x, bf16_t* __restrict__ y, size_t n) {
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
+        y[t] = f2bf(x[t]);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ MLP object
+struct eu3d_mlp {
+    int nl = 5;                   // linear layers
+    int din[5], dout[5];          // padded
+    int rin[5], rout[5];          // real
+    float *w[5], *b[5];           // FP32 masters: w[l] is [dout x din] (Wt), b[l] [dout]
+    float *mw[5], *vw[5], *mb[5], *vb[5];
+    float *gw[5], *gb[5];         // gradients
+    bf16_t *wt[5], *wn[5];        // BF16 copies: Wt [dout x din], W [din x dout]
+    int cap = 0;                  // workspace rows
+    bf16_t* act[5] = {};          // act[0] = input [cap x 32], act[l] = output of layer l (l=1..4)
+    bf16_t* zs[5] = {};           // pre-activations of layers 1..4
+    bf16_t *dz_a = nullptr, *dz_b = nullptr;
+    float* yout = nullptr;        // [cap x 16]
+    float* part = nullptr;        // split-K / reduction partials
+    size_t part_cap = 0;
+    int adam_t = 0;
+    hipStream_t stream = nullptr;
+    eu3d_surrogate_params sp{};
+    bool sp_set = false;
+    std::string err;
+};
+
+namespace {
+
+constexpr int NIN_REAL = 12, NIN = 32, NH = 256, NOUT_REAL = 9, NOUT = 16;
+
+template <bool AKC, bool BKC, int EPI>
+hipError_t gemm(hipStream_t s, const GemmArgs& a0, int splits = 1) {
+    GemmArgs a = a0;
+    a.kchunk = ((a.K + splits - 1) / splits + BK - 1) / BK * BK;
+    const int sp = (a.K + a.kchunk - 1) / a.kchunk;
+    dim3 grid((a.N + BN - 1) / BN, (a.M + BM - 1) / BM, sp);
+    k_gemm<AKC, BKC, EPI><<<grid, 256, 0, s>>>(a);
+    return hipGetLastError();
+}
+
+int splits_for(int K) { return std::max(1, std::min(64, K / 2048)); }
+
+hipError_t ensure_ws(eu3d_mlp* m, int n) {
+    if (n <= m->cap) return hipSuccess;
+    for (int l = 0; l < 5; l++) {
+        if (m->act[l]) (void)hipFree(m->act[l]);
+        if (m->zs[l]) (void)hipFree(m->zs[l]);
+        m->act[l] = m->zs[l] = nullptr;
+    }
+    if (m->dz_a) (void)hipFree(m->dz_a);
+    if (m->dz_b) (void)hipFree(m->dz_b);
+    if (m->yout) (void)hipFree(m->yout);
+    hipError_t e;
+    if ((e = hipMalloc(&m->act[0], (size_t)n * NIN * 2)) != hipSuccess) return e;
+    for (int l = 1; l < 5; l++) {
+        if ((e = hipMalloc(&m->act[l], (size_t)n * NH * 2)) != hipSuccess) return e;
+        if ((e = hipMalloc(&m->zs[l], (size_t)n * NH * 2)) != hipSuccess) return e;
+    }
+    if ((e = hipMalloc(&m->dz_a, (size_t)n * NH * 2)) != hipSuccess) return e;
+    if ((e = hipMalloc(&m->dz_b, (size_t)n * NH * 2)) != hipSuccess) return e;
+    if ((e = hipMalloc(&m->yout, (size_t)n * NOUT * 4)) != hipSuccess) return e;
+    m->cap = n;
+    return hipSuccess;
+}
+
+hipError_t ensure_part(eu3d_mlp* m, size_t nfloats) {
+    if (nfloats <= m->part_cap) return hipSuccess;
+    if (m->part) (void)hipFree(m->part);
+    m->part = nullptr;
+    hipError_t e = hipMalloc(&m->part, nfloats * 4);
+    if (e == hipSuccess) m->part_cap = nfloats;
+    return e;
+}
+
+// Forward through all layers from act[0] (n rows); output FP32 [n x 16] in m->yout.
+hipError_t forward(eu3d_mlp* m, int n, bool save_z, hipStream_t st = nullptr) {
+    hipError_t e;
+    hipStream_t s0 = st ? st : m->stream;
+    for (int l = 0; l < 4; l++) {
+        GemmArgs a{};
+        a.M = n; a.N = m->dout[l]; a.K = m->din[l];
+        a.A = m->act[l]; a.lda = m->din[l];
+        a.B = m->wt[l]; a.ldb = m->din[l];
+        a.Cb = m->act[l + 1]; a.Z = save_z ? m->zs[l + 1] : nullptr; a.bias = m->b[l]; a.ldc = m->dout[l];
+        if ((e = gemm<true, true, EPI_BIAS_GELU>(s0, a)) != hipSuccess) return e;
+    }
+    GemmArgs a{};
+    a.M = n; a.N = m->dout[4]; a.K = m->din[4];
+    a.A = m->act[4]; a.lda = m->din[4];
+    a.B = m->wt[4]; a.ldb = m->din[4];
+    a.Cf = m->yout; a.bias = m->b[4]; a.ldc = m->dout[4];
+    return gemm<true, true, EPI_BIAS_F32>(s0, a);
+}
+
+}  // namespace
+
+namespace {
+using namespace eu3d;
+
+constexpr int NEL = 3;   // H, O, N atoms per species (mechanism order H, O, H2O, OH, O2, H2, H2O2, HO2, N2)
+__device__ __forceinline__ int atoms(int e, int s) {
+    const int H[NS] = {1, 0, 2, 1, 0, 2, 2, 1, 0}, O[NS] = {0, 1, 1, 1, 2, 0, 2, 2, 0}, N[NS] = {0, 0, 0, 0, 0, 0, 0, 0, 2};
+    return e == 0 ? H[s] : (e == 1 ? O[s] : N[s]);
+}
+
+// Surrogate features of one cell: T, ln(p / 1 atm), ln(max(Y_k, 1e-20)), ln(dt), normalised.
+__device__ __forceinline__ bool sur_features(const eu3d_surrogate_params& sp, const double (&Y)[NS], double T, double D,
+                                             double dt, float (&f)[NIN_REAL]) {
+    double sy = 0;
+#pragma unroll
+    for (int s = 0; s < NS; s++) sy += Y[s] / MW[s];
+    const double p = T * D * (R_UNIV * 1000 * sy);
+    double raw[NIN_REAL];
+    raw[0] = T;
+    raw[1] = log(p / 101325.0);
+#pragma unroll
+    for (int s = 0; s < NS; s++) raw[2 + s] = log(fmax(Y[s], 1e-20));
+    raw[11] = log(dt);
+    bool in = true;
+#pragma unroll
+    for (int i = 0; i < NIN_REAL; i++) {
+        f[i] = (float)((raw[i] - sp.feat_mean[i]) / sp.feat_std[i]);
+        in = in && f[i] >= sp.hull_lo[i] && f[i] <= sp.hull_hi[i];
+    }
+    return in;
+}
+
+__device__ __forceinline__ void store_feats(bf16_t* out, size_t r, const float (&f)[NIN_REAL]) {
+    bf16_t* o = out + r * NIN;
+#pragma unroll
+    for (int i = 0; i < NIN; i++) o[i] = i < NIN_REAL ? f2bf(f[i]) : (bf16_t)0;
+}
+
+// Batch cells at rest: features of (rho_s, T) with interval dt. raw != null also receives the
+// un-normalised features in FP32 (for the normalisation statistics).
+__global__ void k_sur_feat_batch(eu3d_surrogate_params sp, long long n, const double* __restrict__ rho,
+                                 const double* __restrict__ T, double dt, bf16_t* __restrict__ feats) {
+    const long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (id >= n) return;
+    double Y[NS], D = 0;
+#pragma unroll
+    for (int s = 0; s < NS; s++) D += rho[(size_t)s * n + id];
+#pragma unroll
+    for (int s = 0; s < NS; s++) Y[s] = rho[(size_t)s * n + id] / D;
+    float f[NIN_REAL];
+    sur_features(sp, Y, T[id], D, dt, f);
+    store_feats(feats, (size_t)id, f);
+}
+
+// Training targets: normalised mass-fraction increments between two batch states.
+__global__ void k_sur_targets(eu3d_surrogate_params sp, long long n, const double* __restrict__ rho0,
+                              const double* __restrict__ rho1, float* __restrict__ tgt) {
+    const long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (id >= n) return;
+    double D0 = 0, D1 = 0;
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        D0 += rho0[(size_t)s * n + id];
+        D1 += rho1[(size_t)s * n + id];
+    }
+#pragma unroll
+    for (int s = 0; s < NOUT; s++) {
+        float v = 0.f;
+        if (s < NS) {
+            const double dy = rho1[(size_t)s * n + id] / D1 - rho0[(size_t)s * n + id] / D0;
+            v = (float)((dy - sp.out_mean[s]) / sp.out_std[s]);
+        }
+        tgt[(size_t)id * NOUT + s] = v;
+    }
+}
+
+// Apply the network's increment to one cell and run the acceptance checks: training hull,
+// mass fractions not below -y_neg_tol, H/O/N element conservation within elem_tol of the
+// cell's element content, and |dT| <= dT_max after the energy Newton at constant internal
+// energy. On acceptance returns the new primitive state and the new partial densities in cs.
+struct SurCell {
+    double Di[NS];
+    double U, V, W, E, T;
+};
+
+__device__ __forceinline__ int sur_apply(const eu3d_surrogate_params& sp, const SurCell& c, const float* out, double dt,
+                                         uint32_t q, double (&cs)[NC], Prim& o) {
+    double D = 0, Y[NS];
+#pragma unroll
+    for (int s = 0; s < NS; s++) D += c.Di[s];
+#pragma unroll
+    for (int s = 0; s < NS; s++) Y[s] = c.Di[s] / D;
+    float f[NIN_REAL];
+    if (!sur_features(sp, Y, c.T, D, dt, f)) return 1;
+    double dY[NS], Yn[NS], ymin = 1.0;
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        dY[s] = (double)out[s] * sp.out_std[s] + sp.out_mean[s];
+        Yn[s] = Y[s] + dY[s];
+        ymin = fmin(ymin, Yn[s]);
+    }
+    if (!(ymin >= -sp.y_neg_tol)) return 2;
+#pragma unroll
+    for (int e = 0; e < NEL; e++) {
+        double have = 0, change = 0;
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+            const int a = atoms(e, s);
+            if (a) {
+                have += a * Y[s] / MW[s];
+                change += a * dY[s] / MW[s];
+            }
+        }
+        if (have > 0 && !(fabs(change) <= sp.elem_tol * have)) return 3;
+    }
+    double sum = 0;
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        Yn[s] = fmax(Yn[s], 0.0);
+        sum += Yn[s];
+    }
+#pragma unroll
+    for (int s = 0; s < NS; s++) cs[s] = D * (Yn[s] / sum);
+    double Dn = 0;
+#pragma unroll
+    for (int s = 0; s < NS; s++) Dn += cs[s];
+    cs[NS] = c.U * Dn;
+    cs[NS + 1] = c.V * Dn;
+    cs[NS + 2] = c.W * Dn;
+    cs[NS + 3] = Dn * c.E + 0.5 * (c.U * c.U + c.V * c.V + c.W * c.W) * Dn;
+    o = prim_from_cs(cs, c.T, c.E, q);
+    if (!(fabs(o.T - c.T) <= sp.dT_max)) return 4;
+    return 0;
+}
+
+__device__ __forceinline__ void sur_count(unsigned* cnt, unsigned* list, unsigned id, int why) {
+    const bool fb = why != 0;
+    const unsigned long long m = __ballot(fb);
+    const int lane = __lane_id();
+    const int leader = __ffsll((unsigned long long)__ballot(1)) - 1;
+    unsigned base = 0;
+    if (lane == leader && m) base = atomicAdd(&cnt[0], (unsigned)__popcll(m));
+    base = __shfl(base, leader);
+    if (fb) {
+        list[base + __popcll(m & ((1ull << lane) - 1ull))] = id;
+        atomicAdd(&cnt[why], 1u);
+    }
+}
+
+// Batch apply: cells at rest with internal energy E; cells colder than T_cold keep their state.
+__global__ void __launch_bounds__(256) k_sur_apply_batch(eu3d_surrogate_params sp, long long n, double* rho, double* T,
+                                                         double* E, const float* __restrict__ out, double dt, uint32_t q,
+                                                         unsigned* cnt, unsigned* list) {
+    const long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    int why = 0;
+    if (id < n && T[id] >= sp.T_cold) {
+        SurCell c;
+#pragma unroll
+        for (int s = 0; s < NS; s++) c.Di[s] = rho[(size_t)s * n + id];
+        c.U = c.V = c.W = 0.0;
+        c.E = E[id];
+        c.T = T[id];
+        double cs[NC];
+        Prim o;
+        why = sur_apply(sp, c, out + (size_t)id * NOUT, dt, q, cs, o);
+        if (why == 0) {
+#pragma unroll
+            for (int s = 0; s < NS; s++) rho[(size_t)s * n + id] = cs[s];
+            T[id] = o.T;
+            E[id] = o.E;
+        }
+    }
+    sur_count(cnt, list, (unsigned)id, why);
+}
+
+// Grid: interior cell (i, j, k) is feature row ((k - bc) * nj + (j - bc)) * ni + (i - bc).
+__global__ void __launch_bounds__(256) k_sur_feat_grid(eu3d_surrogate_params sp, G3 g, Fields F, const Scalars* sc,
+                                                       bf16_t* __restrict__ feats) {
+    const int i = g.bc + blockIdx.x * 64 + threadIdx.x, j = g.bc + blockIdx.y * 4 + threadIdx.y, k = g.bc + blockIdx.z;
+    if (i >= g.ni + g.bc || j >= g.nj + g.bc) return;
+    const size_t c = IDX(g, i, j, k);
+    const size_t r = ((size_t)(k - g.bc) * g.nj + (j - g.bc)) * g.ni + (i - g.bc);
+    double Y[NS], D = 0;
+#pragma unroll
+    for (int s = 0; s < NS; s++) D += FLD(F, EU3D_FIELD_CS)[s * g.n3 + c];
+#pragma unroll
+    for (int s = 0; s < NS; s++) Y[s] = FLD(F, EU3D_FIELD_CS)[s * g.n3 + c] / D;
+    float f[NIN_REAL];
+    sur_features(sp, Y, FLD(F, EU3D_FIELD_T)[c], D, sc->dt, f);
+    store_feats(feats, r, f);
+}
+
+__global__ void __launch_bounds__(256) k_sur_apply_grid(eu3d_surrogate_params sp, G3 g, Fields F, const Scalars* sc,
+                                                        const float* __restrict__ out, uint32_t q, unsigned* cnt,
+                                                        unsigned* list) {
+    const int i = g.bc + blockIdx.x * 64 + threadIdx.x, j = g.bc + blockIdx.y * 4 + threadIdx.y, k = g.bc + blockIdx.z;
+    const bool in = i < g.ni + g.bc && j < g.nj + g.bc;
+    const size_t c = in ? IDX(g, i, j, k) : 0;
+    int why = 0;
+    if (in && FLD(F, EU3D_FIELD_T)[c] >= sp.T_cold) {
+        const size_t r = ((size_t)(k - g.bc) * g.nj + (j - g.bc)) * g.ni + (i - g.bc);
+        SurCell cl;
+#pragma unroll
+        for (int s = 0; s < NS; s++) cl.Di[s] = FLD(F, EU3D_FIELD_CS)[s * g.n3 + c];
+        cl.U = FLD(F, EU3D_FIELD_U)[c];
+        cl.V = FLD(F, EU3D_FIELD_V)[c];
+        cl.W = FLD(F, EU3D_FIELD_W)[c];
+        cl.E = FLD(F, EU3D_FIELD_E)[c];
+        cl.T = FLD(F, EU3D_FIELD_T)[c];
+        double cs[NC];
+        Prim o;
+        why = sur_apply(sp, cl, out + r * NOUT, sc->dt, q, cs, o);
+        if (why == 0) {
+            double* CS = FLD(F, EU3D_FIELD_CS);
+#pragma unroll
+            for (int s = 0; s < NC; s++) CS[s * g.n3 + c] = cs[s];
+            FLD(F, EU3D_FIELD_D)[c] = o.D;
+            FLD(F, EU3D_FIELD_T)[c] = o.T;
+            FLD(F, EU3D_FIELD_U)[c] = o.U;
+            FLD(F, EU3D_FIELD_V)[c] = o.V;
+            FLD(F, EU3D_FIELD_W)[c] = o.W;
+            FLD(F, EU3D_FIELD_E)[c] = o.E;
+            FLD(F, EU3D_FIELD_P)[c] = o.P;
+            FLD(F, EU3D_FIELD_H)[c] = o.H;
+            FLD(F, EU3D_FIELD_GAMMA)[c] = o.Gamma;
+            FLD(F, EU3D_FIELD_C)[c] = o.C;
+#pragma unroll
+            for (int s = 0; s < NS; s++) {
+                FLD(F, EU3D_FIELD_DI)[s * g.n3 + c] = cs[s];
+                FLD(F, EU3D_FIELD_MC)[s * g.n3 + c] = cs[s] / MW[s] * 1000;
+                FLD(F, EU3D_FIELD_YI)[s * g.n3 + c] = o.Y[s];
+            }
+        }
+    }
+    sur_count(cnt, list, (unsigned)c, why);
+}
+
+__global__ void k_zero_u32(unsigned* p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int eu3d_mlp_create(eu3d_mlp** out, uint64_t seed) {
+    if (!out) return EU3D_ERR_ARG;
+    eu3d_mlp* m = new eu3d_mlp();
+    const int din[5] = {NIN, NH, NH, NH, NH}, dout[5] = {NH, NH, NH, NH, NOUT};
+    const int rin[5] = {NIN_REAL, NH, NH, NH, NH}, rout[5] = {NH, NH, NH, NH, NOUT_REAL};
+    if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess) { delete m; return EU3D_ERR_HIP; }
+    uint64_t st = seed * 6364136223846793005ull + 1442695040888963407ull;
+    auto rnd = [&]() {   // uniform (-1, 1) from a 64-bit LCG
+        st = st * 6364136223846793005ull + 1442695040888963407ull;
+        return ((double)(st >> 11) / (double)(1ull << 53)) * 2.0 - 1.0;
+    };
+    for (int l = 0; l < 5; l++) {
+        m->din[l] = din[l]; m->dout[l] = dout[l]; m->rin[l] = rin[l]; m->rout[l] = rout[l];
+        const size_t nw = (size_t)dout[l] * din[l];
+        std::vector<float> hw(nw, 0.f), hb(dout[l], 0.f);
+        const double lim = std::sqrt(6.0 / (rin[l] + rout[l]));   // Glorot uniform
+        for (int o = 0; o < rout[l]; o++)
+            for (int k = 0; k < rin[l]; k++) hw[(size_t)o * din[l] + k] = (float)(lim * rnd());
+        float** bufs[] = {&m->w[l], &m->mw[l], &m->vw[l], &m->gw[l]};
+        for (float** p : bufs) {
+            if (hipMalloc(p, nw * 4) != hipSuccess) return EU3D_ERR_HIP;
+            (void)hipMemset(*p, 0, nw * 4);
+        }
+        float** bb[] = {&m->b[l], &m->mb[l], &m->vb[l], &m->gb[l]};
+        for (float** p : bb) {
+            if (hipMalloc(p, dout[l] * 4) != hipSuccess) return EU3D_ERR_HIP;
+            (void)hipMemset(*p, 0, dout[l] * 4);
+        }
+        if (hipMalloc(&m->wt[l], nw * 2) != hipSuccess || hipMalloc(&m->wn[l], nw * 2) != hipSuccess) return EU3D_ERR_HIP;
+        (void)hipMemcpy(m->w[l], hw.data(), nw * 4, hipMemcpyHostToDevice);
+        k_cast_weights<<<(unsigned)((nw + 255) / 256), 256, 0, m->stream>>>(m->w[l], dout[l], din[l], m->wt[l], m->wn[l]);
+    }
+    if (hipStreamSynchronize(m->stream) != hipSuccess) return EU3D_ERR_HIP;
+    *out = m;
+    return EU3D_OK;
+}
+
+int eu3d_mlp_destroy(eu3d_mlp* m) {
+    if (!m) return EU3D_ERR_ARG;
+    (void)hipStreamSynchronize(m->stream);
+    for (int l = 0; l < 5; l++) {
+        for (float* p : {m->w[l], m->mw[l], m->vw[l], m->gw[l], m->b[l], m->mb[l], m->vb[l], m->gb[l]})
+            if (p) (void)hipFree(p);
+        if (m->wt[l]) (void)hipFree(m->wt[l]);
+        if (m->wn[l]) (void)hipFree(m->wn[l]);
+        if (m->act[l]) (void)hipFree(m->act[l]);
+        if (m->zs[l]) (void)hipFree(m->zs[l]);
+    }
+    for (void* p : {(void*)m->dz_a, (void*)m->dz_b, (void*)m->yout, (void*)m->part})
+        if (p) (void)hipFree(p);
+    (void)hipStreamDestroy(m->stream);
+    delete m;
+    return EU3D_OK;
+}
+
+// Parameters in host FP32, layer by layer: W^T [dout x din] (padded) then bias [dout].
+size_t eu3d_mlp_param_count(const eu3d_mlp* m) {
+    size_t n = 0;
+    for (int l = 0; l < 5; l++) n += (size_t)m->dout[l] * m->din[l] + m->dout[l];
+    return n;
+}
+
+static int xfer_params(eu3d_mlp* m, float* host, bool to_host, bool grads) {
+    size_t off = 0;
+    for (int l = 0; l < 5; l++) {
+        const size_t nw = (size_t)m->dout[l] * m->din[l];
+        float* dw = grads ? m->gw[l] : m->w[l];
+        float* db = grads ? m->gb[l] : m->b[l];
+        if (to_host) {
+            if (hipMemcpy(host + off, dw, nw * 4, hipMemcpyDeviceToHost) != hipSuccess) return EU3D_ERR_HIP;
+            if (hipMemcpy(host + off + nw, db, m->dout[l] * 4, hipMemcpyDeviceToHost) != hipSuccess) return EU3D_ERR_HIP;
+        } else {
+            if (hipMemcpy(dw, host + off, nw * 4, hipMemcpyHostToDevice) != hipSuccess) return EU3D_ERR_HIP;
+            if (hipMemcpy(db, host + off + nw, m->dout[l] * 4, hipMemcpyHostToDevice) != hipSuccess) return EU3D_ERR_HIP;
+            k_cast_weights<<<(unsigned)((nw + 255) / 256), 256, 0, m->stream>>>(m->w[l], m->dout[l], m->din[l], m->wt[l],
+                                                                           m->wn[l]);
+        }
+        off += nw + m->dout[l];
+    }
+    return hipStreamSynchronize(m->stream) == hipSuccess ? EU3D_OK : EU3D_ERR_HIP;
+}
+
+int eu3d_mlp_get_params(eu3d_mlp* m, float* host) { return m && host ? xfer_params(m, host, true, false) : EU3D_ERR_ARG; }
+int eu3d_mlp_set_params(eu3d_mlp* m, const float* host) {
+    return m && host ? xfer_params(m, const_cast<float*>(host), false, false) : EU3D_ERR_ARG;
+}
+int eu3d_mlp_get_grads(eu3d_mlp* m, float* host) { return m && host ? xfer_params(m, host, true, true) : EU3D_ERR_ARG; }
+
+// Inference on n rows of BF16 features (device, [n x 32], zero padded) -> FP32 outputs
+// (device, [n x 16]; columns 9..15 are padding).
+int eu3d_mlp_forward(eu3d_mlp* m, int n, const void* feats_bf16, float* out_f32) {
+    if (!m || n <= 0 || !feats_bf16 || !out_f32) return EU3D_ERR_ARG;
+    if (ensure_ws(m, n) != hipSuccess) return EU3D_ERR_HIP;
+    if (hipMemcpyAsync(m->act[0], feats_bf16, (size_t)n * NIN * 2, hipMemcpyDeviceToDevice, m->stream) != hipSuccess)
+        return EU3D_ERR_HIP;
+    if (forward(m, n, false) != hipSuccess) return EU3D_ERR_HIP;
+    if (hipMemcpyAsync(out_f32, m->yout, (size_t)n * NOUT * 4, hipMemcpyDeviceToDevice, m->stream) != hipSuccess)
+        return EU3D_ERR_HIP;
+    return hipStreamSynchronize(m->stream) == hipSuccess ? EU3D_OK : EU3D_ERR_HIP;
+}
+
+// One optimisation step on n rows: BF16 features [n x 32], FP32 targets [n x 16] (device).
+// Computes the MSE loss over the 9 real outputs, the gradients of every parameter (kept, see
+// eu3d_mlp_get_grads) and, when lr > 0, applies Adam (beta 0.9, 0.999, eps 1e-8) with weight
+// decay wd. loss_out receives the mean squared error of the forward pass.
+int eu3d_mlp_train_step(eu3d_mlp* m, int n, const void* feats_bf16, const float* targets, float lr, float wd,
+                        float* loss_out) {
+    if (!m || n <= 0 || !feats_bf16 || !targets) return EU3D_ERR_ARG;
+    hipStream_t s = m->stream;
+    if (ensure_ws(m, n) != hipSuccess) return EU3D_ERR_HIP;
+    const int sp = splits_for(n);
+    if (ensure_part(m, std::max<size_t>((size_t)sp * NH * NH, (size_t)((n + 255) / 256 + 1) * NH)) != hipSuccess)
+        return EU3D_ERR_HIP;
+    if (hipMemcpyAsync(m->act[0], feats_bf16, (size_t)n * NIN * 2, hipMemcpyDeviceToDevice, s) != hipSuccess)
+        return EU3D_ERR_HIP;
+    if (forward(m, n, true) != hipSuccess) return EU3D_ERR_HIP;
+    // loss and dY (bf16, [n x 16]) into dz_a
+    const int lb = 1024;
+    k_mse_grad<<<lb, 256, 0, s>>>(m->yout, targets, n, NOUT, NOUT_REAL, 2.0f / ((float)n * NOUT_REAL), m->dz_a, m->part);
+    std::vector<float> hp(lb);
+    if (hipMemcpyAsync(hp.data(), m->part, lb * 4, hipMemcpyDeviceToHost, s) != hipSuccess) return EU3D_ERR_HIP;
+    if (hipStreamSynchronize(s) != hipSuccess) return EU3D_ERR_HIP;
+    double loss = 0;
+    for (float v : hp) loss += v;
+    if (loss_out) *loss_out = (float)(loss / ((double)n * NOUT_REAL));
+    // backward: dz holds dL/dZ of layer l (bf16 [n x dout[l]])
+    bf16_t* dz = m->dz_a;
+    bf16_t* dz_next = m->dz_b;
+    for (int l = 4; l >= 0; l--) {
+        // dW^T [dout x din] = dZ^T [dout x n] * A_l [n x din], split over n
+        GemmArgs g{};
+        g.M = m->dout[l]; g.N = m->din[l]; g.K = n;
+        g.A = dz; g.lda = m->dout[l];          // element (m = o, k = b) at dz[b * dout + o]
+        g.B = m->act[l]; g.ldb = m->din[l];    // element (n = i, k = b) at act[b * din + i]
+        g.Cf = m->part; g.ldc = m->din[l];
+        if (gemm<false, false, EPI_F32>(s, g, sp) != hipSuccess) return EU3D_ERR_HIP;
+        const int nw = m->dout[l] * m->din[l];
+        const int kch = ((n + sp - 1) / sp + BK - 1) / BK * BK, nsp = (n + kch - 1) / kch;
+        k_reduce_splits<<<(nw + 255) / 256, 256, 0, s>>>(m->part, m->gw[l], nw, nsp);
+        // db = column sums of dZ
+        const int rpb = 4096, nb = (n + rpb - 1) / rpb;
+        float* cpart = m->part + (size_t)sp * NH * NH - (size_t)nb * m->dout[l];
+        k_colsum_bf16<<<dim3(nb, (m->dout[l] + 255) / 256), 256, 0, s>>>(dz, n, m->dout[l], m->dout[l], rpb, cpart);
+        k_reduce_splits<<<(m->dout[l] + 255) / 256, 256, 0, s>>>(cpart, m->gb[l], m->dout[l], nb);
+        if (l > 0) {
+            // dZ_{l-1} = (dZ_l [n x dout] * W_l^T) .* GELU'(Z_{l-1}); B operand element
+            // (n = i, k = o) = W[i][o] = wn[i * dout + o]
+            GemmArgs d{};
+            d.M = n; d.N = m->din[l]; d.K = m->dout[l];
+            d.A = dz; d.lda = m->dout[l];
+            d.B = m->wn[l]; d.ldb = m->dout[l];
+            d.Cb = dz_next; d.Z = m->zs[l]; d.ldc = m->din[l];
+            if (gemm<true, true, EPI_DGELU>(s, d) != hipSuccess) return EU3D_ERR_HIP;
+            std::swap(dz, dz_next);
+        }
+    }
+    if (lr > 0) {
+        m->adam_t++;
+        const float b1 = 0.9f, b2 = 0.999f;
+        const float bc1 = 1.f - std::pow(b1, (float)m->adam_t), bc2 = 1.f - std::pow(b2, (float)m->adam_t);
+        for (int l = 0; l < 5; l++) {
+            const int nw = m->dout[l] * m->din[l];
+            k_adam<<<(nw + 255) / 256, 256, 0, s>>>(m->w[l], m->mw[l], m->vw[l], m->gw[l], m->dout[l], m->din[l],
+                                                    m->rout[l], m->rin[l], lr, b1, b2, 1e-8f, bc1, bc2, wd, m->wt[l],
+                                                    m->wn[l]);
+            k_adam<<<(m->dout[l] + 255) / 256, 256, 0, s>>>(m->b[l], m->mb[l], m->vb[l], m->gb[l], m->dout[l], 1,
+                                                            m->rout[l], 1, lr, b1, b2, 1e-8f, bc1, bc2, 0.f,
+                                                            reinterpret_cast<bf16_t*>(m->part), nullptr);
+        }
+    }
+    return hipStreamSynchronize(s) == hipSuccess ? EU3D_OK : EU3D_ERR_HIP;
+}
+
+// General BF16 GEMM through the same kernel (exposed for its tests): C[M x N] (FP32,
+// row-major) = A[M x K] (K-contiguous) * B where B is given as [N x K] (K-contiguous) when
+// b_kc != 0, else as [K x N].
+int eu3d_gemm_bf16(int M, int N, int K, const void* A, const void* B, int b_kc, float* C) {
+    if (M <= 0 || N <= 0 || K <= 0 || !A || !B || !C) return EU3D_ERR_ARG;
+    GemmArgs a{};
+    a.M = M; a.N = N; a.K = K;
+    a.A = static_cast<const bf16_t*>(A); a.lda = K;
+    a.B = static_cast<const bf16_t*>(B); a.ldb = b_kc ? K : N;
+    a.Cf = C; a.ldc = N;
+    hipError_t e = b_kc ? gemm<true, true, EPI_F32>(nullptr, a) : gemm<true, false, EPI_F32>(nullptr, a);
+    if (e != hipSuccess) return EU3D_ERR_HIP;
+    return hipDeviceSynchronize() == hipSuccess ? EU3D_OK : EU3D_ERR_HIP;
+}
+
+void* eu3d_mlp_stream(eu3d_mlp* m) { return m ? (void*)m->stream : nullptr; }
+
+int eu3d_mlp_set_surrogate(eu3d_mlp* m, const eu3d_surrogate_params* p) {
+    if (!m || !p) return EU3D_ERR_ARG;
+    for (int i = 0; i < NIN_REAL; i++)
+        if (!(p->feat_std[i] > 0)) return EU3D_ERR_ARG;
+    for (int i = 0; i < NOUT_REAL; i++)
+        if (!(p->out_std[i] > 0)) return EU3D_ERR_ARG;
+    m->sp = *p;
+    m->sp_set = true;
+    return EU3D_OK;
+}
+
+// Normalised BF16 features [n x 32] of batch cells (rho_s [ns][n], T [n]) for an interval dt.
+int eu3d_surrogate_features(eu3d_mlp* m, long long n, const double* rho, const double* T, double dt, void* feats) {
+    if (!m || !m->sp_set || n <= 0 || !rho || !T || !feats) return EU3D_ERR_ARG;
+    k_sur_feat_batch<<<(unsigned)((n + 255) / 256), 256, 0, m->stream>>>(m->sp, n, rho, T, dt, (bf16_t*)feats);
+    return hipStreamSynchronize(m->stream) == hipSuccess ? EU3D_OK : EU3D_ERR_HIP;
+}
+
+// Normalised training targets [n x 16] (FP32) from the states before and after the FP64 step.
+int eu3d_surrogate_targets(eu3d_mlp* m, long long n, const double* rho0, const double* rho1, float* targets) {
+    if (!m || !m->sp_set || n <= 0 || !rho0 || !rho1 || !targets) return EU3D_ERR_ARG;
+    k_sur_targets<<<(unsigned)((n + 255) / 256), 256, 0, m->stream>>>(m->sp, n, rho0, rho1, targets);
+    return hipStreamSynchronize(m->stream) == hipSuccess ? EU3D_OK : EU3D_ERR_HIP;
+}
+
+static int sur_ensure(eu3d_ctx* c, long long n) {
+    if (c->sur_cap >= n) return EU3D_OK;
+    if (c->sur_list) (void)hipFree(c->sur_list);
+    c->sur_list = nullptr;
+    if (hipMalloc(&c->sur_list, (size_t)n * 2 * sizeof(unsigned) + 64 * sizeof(unsigned) + (size_t)n * sizeof(int)) != hipSuccess)
+        return EU3D_ERR_HIP;
+    c->sur_cap = n;
+    return EU3D_OK;
+}
+
+static int sur_stats(eu3d_ctx* c, eu3d_surrogate_stats* st, long long n) {
+    unsigned h[8];
+    if (hipMemcpyAsync(h, c->sur_list + 2 * c->sur_cap, sizeof(h), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
+        return EU3D_ERR_HIP;
+    if (st) {
+        st->cells = n;
+        st->fallback = h[0];
+        st->hull = h[1];
+        st->negative = h[2];
+        st->element = h[3];
+        st->dT = h[4];
+    }
+    return EU3D_OK;
+}
+
+// Surrogate chemistry on batch cells at rest, with the FP64 adaptive trapezoid on every cell the
+// checks reject. Device pointers as eu3d_chem_batch.
+int eu3d_surrogate_batch(eu3d_ctx* c, eu3d_mlp* m, long long n, double* rho, double* T, double* E, double dtm,
+                         int nchem_max, eu3d_surrogate_stats* st) {
+    if (!c || !m || !m->sp_set || n <= 0 || n >= (1ll << 31) || !rho || !T || !E || !(dtm > 0)) return EU3D_ERR_ARG;
+    if (ensure_ws(m, (int)n) != hipSuccess || sur_ensure(c, n) != EU3D_OK) return EU3D_ERR_HIP;
+    unsigned* list = c->sur_list;
+    unsigned* work = c->sur_list + c->sur_cap;
+    unsigned* cnt = c->sur_list + 2 * c->sur_cap;
+    int* keep = reinterpret_cast<int*>(cnt + 64);
+    const unsigned nb = (unsigned)((n + 255) / 256);
+    k_zero_u32<<<1, 64, 0, c->stream>>>(cnt, 64);
+    k_sur_feat_batch<<<nb, 256, 0, c->stream>>>(m->sp, n, rho, T, dtm, m->act[0]);
+    if (forward(m, (int)n, false, c->stream) != hipSuccess) return EU3D_ERR_HIP;
+    k_sur_apply_batch<<<nb, 256, 0, c->stream>>>(m->sp, n, rho, T, E, m->yout, dtm, c->quirks, cnt, list);
+    if (eu3d::launch_batch_list(c, n, rho, T, E, dtm, list, cnt, (unsigned)n, work, keep, nchem_max) != hipSuccess)
+        return EU3D_ERR_HIP;
+    return sur_stats(c, st, n);
+}
+
+int eu3d_set_surrogate(eu3d_ctx* c, eu3d_mlp* m) {
+    if (!c) return EU3D_ERR_ARG;
+    if (m && !m->sp_set) return EU3D_ERR_ARG;
+    c->mlp = m;
+    return EU3D_OK;
+}
+
+int eu3d_get_surrogate_stats(eu3d_ctx* c, eu3d_surrogate_stats* st) {
+    if (!c || !st) return EU3D_ERR_ARG;
+    *st = c->sur_stats;
+    return EU3D_OK;
+}
+
+}  // extern "C"
+
+namespace eu3d {
+// Surrogate chemistry over the interior of a grid context (the chemistry half of a step with
+// EU3D_CHEM_SURROGATE): features, network, apply with checks, FP64 fallback on the rejects.
+hipError_t launch_surrogate_grid(eu3d_ctx* c, int nchem_max) {
+    eu3d_mlp* m = c->mlp;
+    const long long n = (long long)c->g.ni * c->g.nj * c->g.nk;
+    if (!m || ensure_ws(m, (int)n) != hipSuccess || sur_ensure(c, n) != EU3D_OK) return hipErrorInvalidValue;
+    unsigned* list = c->sur_list;
+    unsigned* work = c->sur_list + c->sur_cap;
+    unsigned* cnt = c->sur_list + 2 * c->sur_cap;
+    const dim3 grid((c->g.ni + 63) / 64, (c->g.nj + 3) / 4, c->g.nk), blk(64, 4);
+    k_zero_u32<<<1, 64, 0, c->stream>>>(cnt, 64);
+    k_sur_feat_grid<<<grid, blk, 0, c->stream>>>(m->sp, c->g, c->fl, c->sc, m->act[0]);
+    hipError_t e = forward(m, (int)n, false, c->stream);
+    if (e != hipSuccess) return e;
+    k_sur_apply_grid<<<grid, blk, 0, c->stream>>>(m->sp, c->g, c->fl, c->sc, m->yout, c->quirks, cnt, list);
+    return launch_trap_list(c, list, cnt, (unsigned)n, work, nchem_max);
+}
+
+hipError_t surrogate_grid_stats(eu3d_ctx* c) {
+    const long long n = (long long)c->g.ni * c->g.nj * c->g.nk;
+    return sur_stats(c, &c->sur_stats, n) == EU3D_OK ? hipSuccess : hipErrorUnknown;
+}
+}  // namespace eu3d
+
+extern "C" {
+
+}  // extern "C"

@@ -80,6 +80,9 @@ enum eu3d_field {
 #define EU3D_CHEM_IMEX 1          /* Euler::IMEX, src/Euler.cpp:748-760 */
 #define EU3D_CHEM_TRAPEZOID 2     /* Euler::Trapezoid(0), src/Euler.cpp:369-379 */
 #define EU3D_CHEM_TRAP_ADAPTIVE 3 /* Euler::Trapezoid(1) cell loop, src/Euler.cpp:383,526-556 */
+#define EU3D_CHEM_SURROGATE 4     /* MLP surrogate with FP64 fallback (the reference's "DNN" slot,
+                                     src/Euler.cpp:129-132,773-780, is a stub; this has no reference
+                                     counterpart) */
 
 typedef struct eu3d_ctx eu3d_ctx;
 
@@ -188,6 +191,43 @@ int eu3d_chem_batch(eu3d_ctx *ctx, long long n, double *rho_s, double *T, double
                     int nchem_max, int *nchem_out, eu3d_chem_stats *stats);
 int eu3d_chem_set_schedule(eu3d_ctx *ctx, const eu3d_chem_schedule *s);
 int eu3d_chem_get_schedule(eu3d_ctx *ctx, eu3d_chem_schedule *s);
+
+/* ---- MLP chemistry surrogate (no reference counterpart: the reference's "DNN" scheme only
+ * advects, src/Euler.cpp:773-780). 12 features (T, ln p/atm, 9 ln max(Y_k, 1e-20), ln dt),
+ * 4 hidden layers of 256 with GELU, 9 outputs (normalised dY_k). BF16 weights and activations
+ * on MFMA with FP32 accumulation; FP32 master weights and Adam. Device pointers throughout. */
+typedef struct eu3d_mlp eu3d_mlp;
+typedef struct {
+    float feat_mean[12], feat_std[12];   /* feature normalisation */
+    float out_mean[9], out_std[9];       /* output (dY_k) normalisation */
+    float hull_lo[12], hull_hi[12];      /* training hull, in normalised features */
+    double dT_max;                       /* largest accepted |dT| of one step [K] */
+    double elem_tol;                     /* H/O/N conservation bound, relative to the cell's content */
+    double y_neg_tol;                    /* most negative accepted mass fraction */
+    double T_cold;                       /* cells below this temperature keep their state */
+} eu3d_surrogate_params;
+typedef struct {
+    long long cells, fallback, hull, negative, element, dT;
+} eu3d_surrogate_stats;
+int eu3d_mlp_create(eu3d_mlp **out, uint64_t seed);
+int eu3d_mlp_destroy(eu3d_mlp *m);
+size_t eu3d_mlp_param_count(const eu3d_mlp *m);
+int eu3d_mlp_get_params(eu3d_mlp *m, float *host);      /* per layer: W^T [dout x din] then bias */
+int eu3d_mlp_set_params(eu3d_mlp *m, const float *host);
+int eu3d_mlp_get_grads(eu3d_mlp *m, float *host);
+int eu3d_mlp_forward(eu3d_mlp *m, int n, const void *feats_bf16, float *out_f32);
+int eu3d_mlp_train_step(eu3d_mlp *m, int n, const void *feats_bf16, const float *targets, float lr, float wd,
+                        float *loss_out);
+int eu3d_gemm_bf16(int M, int N, int K, const void *A, const void *B, int b_kc, float *C);
+void *eu3d_mlp_stream(eu3d_mlp *m);
+int eu3d_mlp_set_surrogate(eu3d_mlp *m, const eu3d_surrogate_params *p);
+int eu3d_surrogate_features(eu3d_mlp *m, long long n, const double *rho, const double *T, double dt, void *feats);
+int eu3d_surrogate_targets(eu3d_mlp *m, long long n, const double *rho0, const double *rho1, float *targets);
+int eu3d_surrogate_batch(eu3d_ctx *ctx, eu3d_mlp *m, long long n, double *rho_s, double *T, double *E, double dtm,
+                         int nchem_max, eu3d_surrogate_stats *stats);
+/* Attach a trained surrogate to a context for eu3d_step with EU3D_CHEM_SURROGATE (NULL detaches). */
+int eu3d_set_surrogate(eu3d_ctx *ctx, eu3d_mlp *m);
+int eu3d_get_surrogate_stats(eu3d_ctx *ctx, eu3d_surrogate_stats *stats);
 
 /* Device memory of a field (SoA, x fastest; component c at offset c * n3) for zero-copy use. */
 double *eu3d_device_field(eu3d_ctx *ctx, int field);

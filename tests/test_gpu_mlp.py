@@ -1,0 +1,233 @@
+"""GPU tests of the surrogate network and the batched chemistry, through the C ABI."""
+import numpy as np
+import pytest
+
+from tests.common import say
+
+pytestmark = pytest.mark.gpu
+
+
+def _t():
+    import torch
+    return torch
+
+
+def bf16_bits(a):
+    torch = _t()
+    return torch.tensor(a, dtype=torch.float32).to(torch.bfloat16).view(torch.int16).cuda()
+
+
+def bf16_vals(a):
+    from oracle.mlp import bf16_round
+    return bf16_round(a)
+
+
+@pytest.mark.parametrize("M,N,K,b_kc", [(128, 128, 32, True), (256, 384, 96, True), (200, 130, 70, True),
+                                        (33, 17, 300, True), (256, 256, 256, False), (150, 90, 45, False)])
+def test_gemm_matches_fp64(M, N, K, b_kc):
+    """Checks the MFMA fragment layouts and the LDS staging (both operand storages)."""
+    torch = _t()
+    say("case gemm", M, N, K, b_kc)
+    rng = np.random.default_rng(M + N + K)
+    A = rng.normal(size=(M, K))
+    B = rng.normal(size=(N, K) if b_kc else (K, N))
+    C = torch.empty((M, N), dtype=torch.float32, device="cuda")
+    from eu3d_amd.api import gemm_bf16
+    gemm_bf16(bf16_bits(A), bf16_bits(B), b_kc, C)
+    torch.cuda.synchronize()
+    Ab, Bb = bf16_vals(A), bf16_vals(B)
+    ref = Ab @ (Bb.T if b_kc else Bb)
+    err = np.abs(C.cpu().numpy() - ref).max() / np.abs(ref).max()
+    assert err < 1e-5, err
+
+
+def _random_params(mlp, seed=0):
+    from tests.test_mlp_oracle import random_params
+    p = random_params(seed).astype(np.float32)
+    mlp.set_params(p)
+    return mlp.get_params().astype(np.float64)
+
+
+def test_forward_matches_fp64():
+    torch = _t()
+    from eu3d_amd.api import MLP
+    from oracle import mlp as om
+    say("case mlp forward")
+    m = MLP(seed=3)
+    p = _random_params(m, 1)
+    rng = np.random.default_rng(2)
+    n = 3000
+    X = np.zeros((n, 32))
+    X[:, :12] = rng.normal(size=(n, 12))
+    out = torch.empty((n, 16), dtype=torch.float32, device="cuda")
+    m.forward(bf16_bits(X), out)
+    torch.cuda.synchronize()
+    Y = out.cpu().numpy()
+    Yb, _ = om.forward(p, X, bf16=True)
+    Yf, _ = om.forward(p, X, bf16=False)
+    sc = np.abs(Yf[:, :9]).max()
+    assert np.abs(Y[:, :9] - Yb[:, :9]).max() <= 2e-2 * sc
+    assert np.abs(Y[:, :9] - Yf[:, :9]).max() <= 6e-2 * sc
+    assert np.abs(Y[:, 9:]).max() == 0.0   # padding outputs stay zero
+
+
+def test_gradients_match_fp64():
+    torch = _t()
+    from eu3d_amd.api import MLP
+    from oracle import mlp as om
+    say("case mlp grads")
+    m = MLP(seed=4)
+    p = _random_params(m, 5)
+    rng = np.random.default_rng(6)
+    n = 4096
+    X = np.zeros((n, 32))
+    X[:, :12] = rng.normal(size=(n, 12))
+    T = np.zeros((n, 16))
+    T[:, :9] = rng.normal(size=(n, 9))
+    loss = m.train_step(bf16_bits(X), torch.tensor(T, dtype=torch.float32, device="cuda"), 0.0)
+    g = m.get_grads().astype(np.float64)
+    lref, gref = om.loss_and_grads(p, bf16_vals(X), T)
+    assert abs(loss - lref) <= 2e-2 * lref
+    off = 0
+    for din, dout in zip(om.DIN, om.DOUT):
+        for sz in (din * dout, dout):
+            a, b = g[off:off + sz], gref[off:off + sz]
+            off += sz
+            cos = a @ b / (np.linalg.norm(a) * np.linalg.norm(b) + 1e-30)
+            rel = np.linalg.norm(a - b) / (np.linalg.norm(b) + 1e-30)
+            assert cos > 0.995 and rel < 0.1, (din, dout, sz, cos, rel)
+
+
+def test_training_reduces_loss():
+    torch = _t()
+    from eu3d_amd.api import MLP
+    say("case mlp training")
+    m = MLP(seed=7)
+    rng = np.random.default_rng(8)
+    n = 32768
+    X = np.zeros((n, 32))
+    X[:, :12] = rng.uniform(-1, 1, (n, 12))
+    Wt = rng.normal(size=(12, 9))
+    T = np.zeros((n, 16))
+    T[:, :9] = np.sin(X[:, :12] @ Wt)
+    xb = bf16_bits(X)
+    tt = torch.tensor(T, dtype=torch.float32, device="cuda")
+    first = m.train_step(xb, tt, 1e-3)
+    for _ in range(300):
+        last = m.train_step(xb, tt, 1e-3)
+    assert last < 0.1 * first, (first, last)
+
+
+def _states(n, seed=11):  # device copies of configs.chem_states
+    torch = _t()
+    from eu3d_amd import configs
+    rho, T, E, _ = configs.chem_states(n, seed=seed)
+    return (torch.tensor(rho.T.copy(), device="cuda"), torch.tensor(T, device="cuda"),
+            torch.tensor(E, device="cuda"), rho, T, E)
+
+
+def _oracle_batch(rho, T, E, dtm):
+    """The oracle's Euler::Trapezoid(type 1) cell loop on states at rest, laid out along x."""
+    from eu3d_amd import configs
+    from oracle import OracleFlow
+    n = T.shape[0]
+    x = configs.make_nodes(n, 1e-4)
+    y = configs.make_nodes(1, 1e-4)
+    o = OracleFlow(x, y, y, quirks=0)
+    sh = o.shape3
+    cs = np.zeros(sh + (13,))
+    cs[..., 0:9] = rho[0]            # ghosts: any valid state
+    cs[..., 12] = rho[0].sum() * E[0]
+    cs[2:-2, 2, 2, :9] = rho
+    cs[2:-2, 2, 2, 12] = rho.sum(1) * E
+    To = np.full(sh, T[0])
+    To[2:-2, 2, 2] = T
+    o.set("CS", cs)
+    o.set("T", To)
+    o.update_after_cs()
+    T_in = o.get("T")[2:-2, 2, 2].copy()
+    E_in = o.get("E")[2:-2, 2, 2].copy()
+    o.set_time(dtm)
+    o.trapezoid_adaptive(10000)
+    return T_in, E_in, o.get("CS")[2:-2, 2, 2, :9], o.get("T")[2:-2, 2, 2], o.get("NCHEM")[2:-2, 2, 2]
+
+
+@pytest.mark.parametrize("dtm", [1e-8, 7e-6])
+def test_chem_batch_matches_oracle(dtm):
+    """Batched FP64 chemistry (compaction + persistent kernel) equals the oracle's
+    Euler::Trapezoid(type 1) cell loop on the same states (7e-6 s: up to 4 sub-steps)."""
+    torch = _t()
+    from eu3d_amd import configs
+    from eu3d_amd.api import Context
+    say("case chem batch", dtm)
+    n = 3000
+    rho, T, E, _ = configs.chem_states(n, seed=11)
+    T_in, E_in, ro, To, nch_o = _oracle_batch(rho, T, E, dtm)
+    r = torch.tensor(rho.T.copy(), device="cuda")
+    t = torch.tensor(T_in, device="cuda")
+    e = torch.tensor(E_in, device="cuda")
+    x = configs.make_nodes(8, 1e-4)
+    g = Context(x, x, x, quirks=0)
+    nch = torch.empty(n, dtype=torch.int32, device="cuda")
+    st = g.chem_batch(n, r.data_ptr(), t.data_ptr(), e.data_ptr(), dtm, 10000, nch.data_ptr())
+    torch.cuda.synchronize()
+    assert (nch.cpu().numpy() == nch_o).all()
+    assert st["active_cells"] == int((nch_o > 1).sum()) and st["substeps_total"] == int(nch_o.sum())
+    rg = r.cpu().numpy().T
+    assert np.abs(rg - ro).max() / ro.sum(1).max() < 1e-10
+    assert np.abs(t.cpu().numpy() - To).max() < 1e-10 * 3000
+    say("  active", st["active_cells"], "substeps", st["substeps_total"], "max nchem", st["nchem_max_seen"])
+
+
+def test_surrogate_full_fallback_equals_fp64():
+    """With every cell rejected (dT_max = -1) the surrogate path returns the FP64 result bit for bit."""
+    torch = _t()
+    from eu3d_amd import configs, surrogate
+    from eu3d_amd.api import MLP, Context
+    say("case surrogate fallback")
+    n = 20000
+    r, t, e, *_ = _states(n, 21)
+    x = configs.make_nodes(8, 1e-4)
+    g = Context(x, x, x, quirks=0)
+    m = MLP(seed=1)
+    p = surrogate.fit_params(r, r * 1.0001, t, 1e-8)
+    p.dT_max = -1.0
+    p.T_cold = 0.0
+    m.set_surrogate(p)
+    r2, t2, e2 = r.clone(), t.clone(), e.clone()
+    st = g.surrogate_batch(m, n, r.data_ptr(), t.data_ptr(), e.data_ptr(), 1e-8, 10000)
+    g.chem_batch(n, r2.data_ptr(), t2.data_ptr(), e2.data_ptr(), 1e-8, 10000)
+    torch.cuda.synchronize()
+    assert st["fallback"] == n
+    assert float((r - r2).abs().max()) <= 1e-13 * float(r2.abs().max())
+    assert float((t - t2).abs().max()) <= 1e-12 * 3000
+    say("  bitwise equal:", bool(torch.equal(r, r2) and torch.equal(t, t2)))
+
+
+def test_surrogate_trained_small():
+    """Harvest FP64 increments, train briefly, and apply with fallback: accepted cells stay within
+    the |dT| bound and the loss falls."""
+    torch = _t()
+    from eu3d_amd import configs, surrogate
+    from eu3d_amd.api import MLP, Context
+    say("case surrogate trained")
+    x = configs.make_nodes(8, 1e-4)
+    g = Context(x, x, x, quirks=0)
+    H = surrogate.Harvest()
+    surrogate.harvest_states(g, H, 1 << 16, [5e-9, 1e-8, 2e-8], seed=3)
+    r0, r1, t0, dt = H.cat()
+    m = MLP(seed=2)
+    p = surrogate.fit_params(r0, r1, t0, dt)
+    m.set_surrogate(p)
+    X, Y = surrogate.make_dataset(m, r0, r1, t0, dt)
+    hist = surrogate.train(m, X, Y, epochs=4, batch=8192, lr=2e-3)
+    assert hist[-1] < 0.5 * hist[0]
+    n = 1 << 14
+    r, t, e, *_ = _states(n, 99)
+    rf, tf, ef = r.clone(), t.clone(), e.clone()
+    st = g.surrogate_batch(m, n, r.data_ptr(), t.data_ptr(), e.data_ptr(), 1e-8, 10000)
+    g.chem_batch(n, rf.data_ptr(), tf.data_ptr(), ef.data_ptr(), 1e-8, 10000)
+    torch.cuda.synchronize()
+    dT = (t - tf).abs()
+    say("  fallback", st, "max |dT err|", float(dT.max()), "median", float(dT.median()))
+    assert float(dT.max()) <= 2 * p.dT_max + 1e-6
