@@ -6,9 +6,10 @@ synthetic initial condition). A step is one full time step: CFL dt, ghost fill, 
 advection of all three directions with the explicit update, and the IMEX chemistry with the
 temperature Newton. Inputs are resident in HBM before the timed region.
 
-Prints one JSON line. `roofline` is for the dominant kernel (the fused advection pass),
-timed live with HIP events on the context's stream; `cpu_baseline` times the CPU restatement
-of the reference arithmetic (the oracle) on a bounded sample of the same workload.
+Prints one JSON line. `roofline` is for the dominant kernel of the step, k_imex_fused (the fused
+Diagonalized + Update_IMEX + Update_after_CS cell kernel), timed live with HIP events on the
+context's stream; `kernels` gives the same for the two advection passes. `cpu_baseline` times the
+CPU restatement of the reference arithmetic (the oracle) on a bounded sample of the same workload.
 """
 import argparse
 import json
@@ -24,9 +25,19 @@ sys.path.insert(0, ROOT)
 METRIC = "cell-updates/s (flux+chem), plain FP64 vs MLP-chem surrogate at stated error"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
-# algorithmic bytes of the fused advection pass per interior cell: read the 16 reconstructed
-# primitives (P, rho, u, v, w, H, gamma, Y_1..9) once, write the 13 RHS components once
-FLUX_BYTES_PER_CELL = (16 + 13) * 8
+# Algorithmic bytes per interior cell of each kernel of the plain IMEX step (FP64):
+#  k_imex_fused (Diagonalized + Update_IMEX + Update_after_CS of one cell): read the 13 conserved
+#    values, the 13 RHS values and the temperature guess, write the 13 conserved values and T
+#  k_flux_z / k_flux_xy (the two advection passes): read the 16 reconstructed primitives
+#    (P, rho, u, v, w, H, gamma, Y_1..9) and write 13 values (the z term, then the RHS, which
+#    the x/y pass also reads)
+KERNEL_BYTES_PER_CELL = {
+    "imex_fused": (13 + 13 + 1 + 13 + 1) * 8,
+    "flux_z": (16 + 13) * 8,
+    "flux_xy": (16 + 13 + 13) * 8,
+}
+KERNEL_NAMES = {"imex_fused": "k_imex_fused", "flux_z": "k_flux_z", "flux_xy": "k_flux_xy<32,8>"}
+PROFILE = "imex_fused_bubble3d_384x128x128_pmc.json"
 
 
 def parse():
@@ -36,7 +47,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--grid", type=str, default="384x128x128")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-grid", type=str, default="192x96x96")
+    ap.add_argument("--cpu-grid", type=str, default="384x128x128")
     ap.add_argument("--cpu-steps", type=int, default=3)
     return ap.parse_args()
 
@@ -75,22 +86,21 @@ def cpu_baseline(grid, steps):
                       f"after 1 warm-up step, {el:.1f} s"}
 
 
-def read_traffic():
-    """HBM bytes per launch of the fused advection pass from the committed rocprofv3 --pmc
-    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), or None."""
-    p = os.path.join(ROOT, "profiles", "flux_fused_bubble3d_384x128x128_pmc.json")
+def read_profile():
+    """HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration) and
+    FP64 FLOPs per cell of k_imex_fused from the committed rocprofv3 --pmc summary, or {}."""
+    p = os.path.join(ROOT, "profiles", PROFILE)
     if not os.path.exists(p):
-        return None
+        return {}
     with open(p) as f:
-        d = json.load(f)
-    return d.get("hbm_bytes_per_launch")
+        return json.load(f).get("kernels", {}).get("k_imex_fused", {})
 
 
 def main():
     a = parse()
     if a.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it "
-                                   "(see DESIGN.md for how the path would shard)", "n_gpus": a.gpus}))
+        print(json.dumps({"error": "one GPU only: runs on several GPUs are not part of this implementation",
+                          "n_gpus": a.gpus}))
         sys.exit(2)
     import torch
     from eu3d_amd import configs
@@ -118,18 +128,25 @@ def main():
     ms = t0.elapsed_time(t1) / a.steps
     value = wl.ncells / (ms * 1e-3)
     rc = g.check_finite()
-    # dominant kernel: the fused advection pass, timed alone on the context stream
-    reps = 10
-    g.advection_fused(1, 0)
-    g.sync()
-    k0 = ev()
-    for _ in range(reps):
-        g.advection_fused(1, 0)
-    k1 = ev()
-    g.sync()
-    kern_s = k0.elapsed_time(k1) * 1e-3 / reps
-    alg = FLUX_BYTES_PER_CELL * wl.ncells
-    achieved = alg / kern_s / 1e9
+    # per-kernel launch durations, HIP events on the context stream around each launch
+    kern = {}
+    for k in ("imex_fused", "flux_z", "flux_xy"):
+        kms = g.time_kernel(k, reps=10)
+        gbs = KERNEL_BYTES_PER_CELL[k] * wl.ncells / (kms * 1e-3) / 1e9
+        kern[KERNEL_NAMES[k]] = {"avg_launch_ms": kms, "algorithmic_bytes_per_cell": KERNEL_BYTES_PER_CELL[k],
+                                 "achieved_GBs": gbs, "frac_hbm": gbs / HBM_PEAK_GBS}
+    dom = kern["k_imex_fused"]
+    prof = read_profile()
+    alg = KERNEL_BYTES_PER_CELL["imex_fused"] * wl.ncells
+    achieved = alg / (dom["avg_launch_ms"] * 1e-3) / 1e9
+    roof = {"kernel": "k_imex_fused (Diagonalized + Update_IMEX + Update_after_CS, one cell per lane)",
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": prof.get("hbm_bytes_per_launch"),
+            "algorithmic_bytes_per_launch": alg, "avg_launch_ms": dom["avg_launch_ms"]}
+    if prof.get("fp64_flop_per_cell"):
+        tf = prof["fp64_flop_per_cell"] * wl.ncells / (dom["avg_launch_ms"] * 1e-3) / 1e12
+        roof["fp64"] = {"flop_per_cell": prof["fp64_flop_per_cell"], "achieved_TFLOPs": tf,
+                        "peak_TFLOPs": FP64_PEAK_TFLOPS, "frac": tf / FP64_PEAK_TFLOPS}
     res = {
         "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
@@ -137,10 +154,8 @@ def main():
         "config": {"workload": f"bubble3d {ni}x{nj}x{nk} (3-D Mach-2 shock / H2-air bubble, IMEX, "
                                f"FP64, MUSCL-minmod + AUSM, CFL 0.5)", "cells": wl.ncells,
                    "chemistry": "IMEX", "quirks": "FIXED"},
-        "roofline": {"kernel": "k_flux_fused (advection, 3 directions + EE)", "bound": "hbm",
-                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(),
-                     "algorithmic_bytes_per_launch": alg, "avg_launch_ms": kern_s * 1e3},
+        "roofline": roof,
+        "kernels": kern,
         "finite": rc == 0, "wall_s_timed": wall,
     }
     if not a.no_cpu:

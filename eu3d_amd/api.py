@@ -39,7 +39,7 @@ EXPORTED = [
     "eu3d_diagonalized", "eu3d_update_imex", "eu3d_trapezoid", "eu3d_trapezoid_prediction",
     "eu3d_trapezoid_adaptive", "eu3d_check_finite", "eu3d_step", "eu3d_get_chem_stats",
     "eu3d_chem_batch", "eu3d_chem_set_schedule", "eu3d_chem_get_schedule", "eu3d_device_field",
-    "eu3d_cells_with_ghosts", "eu3d_stream",
+    "eu3d_cells_with_ghosts", "eu3d_stream", "eu3d_time_kernel",
     "eu3d_mlp_create", "eu3d_mlp_destroy", "eu3d_mlp_param_count", "eu3d_mlp_get_params", "eu3d_mlp_set_params",
     "eu3d_mlp_get_grads", "eu3d_mlp_forward", "eu3d_mlp_train_step", "eu3d_gemm_bf16", "eu3d_mlp_stream",
     "eu3d_mlp_set_surrogate", "eu3d_surrogate_features", "eu3d_surrogate_targets", "eu3d_surrogate_batch",
@@ -140,6 +140,7 @@ def load_library(path: str = LIB_PATH):
     lib.eu3d_cells_with_ghosts.restype = C.c_size_t
     lib.eu3d_stream.argtypes = [vp]
     lib.eu3d_stream.restype = vp
+    lib.eu3d_time_kernel.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, dp]
     fp = C.POINTER(C.c_float)
     lib.eu3d_mlp_create.argtypes = [C.POINTER(vp), C.c_uint64]
     lib.eu3d_mlp_destroy.argtypes = [vp]
@@ -307,6 +308,7 @@ class Context:
     def chem_batch(self, n: int, rho_ptr: int, T_ptr: int, E_ptr: int, dtm: float, nchem_max: int = 0,
                    nchem_ptr: int | None = None) -> dict:
         """Batched chemistry on device pointers (rho_s SoA [ns][n], T [n], E [n])."""
+        _order_after_producer()
         s = ChemStats()
         rc = self.lib.eu3d_chem_batch(self.h, n, C.c_void_p(rho_ptr), C.c_void_p(T_ptr), C.c_void_p(E_ptr),
                                       dtm, nchem_max, C.c_void_p(nchem_ptr) if nchem_ptr else None, C.byref(s))
@@ -332,6 +334,7 @@ class Context:
         return {k: getattr(s, k) for k, _ in SurrogateStats._fields_}
 
     def surrogate_batch(self, mlp, n, rho_ptr, T_ptr, E_ptr, dtm, nchem_max=0) -> dict:
+        _order_after_producer()
         s = SurrogateStats()
         rc = self.lib.eu3d_surrogate_batch(self.h, mlp.h, n, C.c_void_p(rho_ptr), C.c_void_p(T_ptr),
                                            C.c_void_p(E_ptr), dtm, nchem_max, C.byref(s))
@@ -344,10 +347,27 @@ class Context:
     def stream(self) -> int:
         return self.lib.eu3d_stream(self.h)
 
+    def time_kernel(self, kernel: str, reps: int = 10, diff: int = 1, limiter: int = 0) -> float:
+        """Mean launch duration [ms] of one step kernel ("flux_z", "flux_xy", "imex_fused"),
+        HIP events on the context stream."""
+        ms = C.c_double()
+        k = {"flux_z": 0, "flux_xy": 1, "imex_fused": 2}[kernel]
+        self._chk(self.lib.eu3d_time_kernel(self.h, k, reps, diff, limiter, C.byref(ms)), "time_kernel")
+        return ms.value
+
 
 def _vp(x):
     """Device pointer of a torch tensor (plumbing only)."""
     return C.c_void_p(x.data_ptr())
+
+
+def _order_after_producer():
+    """Wait for the work torch queued on its current stream (the producer of the device buffers
+    handed to the library); the library's own streams are ordered after the null stream only."""
+    import sys
+    torch = sys.modules.get("torch")
+    if torch is not None and torch.cuda.is_initialized():
+        torch.cuda.current_stream().synchronize()
 
 
 class MLP:
@@ -397,10 +417,12 @@ class MLP:
 
     def forward(self, feats_bf16, out_f32):
         """feats: torch int16 [n, 32] holding BF16 bits; out: torch float32 [n, 16]."""
+        _order_after_producer()
         n = feats_bf16.shape[0]
         self._chk(self.lib.eu3d_mlp_forward(self.h, n, _vp(feats_bf16), _vp(out_f32)), "mlp_forward")
 
     def train_step(self, feats_bf16, targets_f32, lr: float, wd: float = 0.0) -> float:
+        _order_after_producer()
         loss = C.c_float()
         n = feats_bf16.shape[0]
         self._chk(self.lib.eu3d_mlp_train_step(self.h, n, _vp(feats_bf16), _vp(targets_f32), lr, wd, C.byref(loss)),
@@ -412,10 +434,12 @@ class MLP:
         self.params_set = True
 
     def features(self, rho, T, dt, out_bf16):
+        _order_after_producer()
         n = T.shape[0]
         self._chk(self.lib.eu3d_surrogate_features(self.h, n, _vp(rho), _vp(T), dt, _vp(out_bf16)), "features")
 
     def targets(self, rho0, rho1, out_f32):
+        _order_after_producer()
         n = rho0.shape[1]
         self._chk(self.lib.eu3d_surrogate_targets(self.h, n, _vp(rho0), _vp(rho1), _vp(out_f32)), "targets")
 
@@ -423,6 +447,7 @@ class MLP:
 def gemm_bf16(A_bits, B_bits, b_kc: bool, C_out):
     """C = A * B on device through the surrogate's GEMM kernel (A [M x K] BF16 bits, B [N x K]
     when b_kc else [K x N], C FP32 [M x N])."""
+    _order_after_producer()
     lib = load_library()
     M, K = A_bits.shape
     N = B_bits.shape[0] if b_kc else B_bits.shape[1]

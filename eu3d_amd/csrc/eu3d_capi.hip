@@ -96,7 +96,9 @@ int eu3d_create(eu3d_ctx** out, const eu3d_grid* grid, const eu3d_mech* mech, ui
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, c->device) == hipSuccess && prop.multiProcessorCount > 0)
         c->num_cu = prop.multiProcessorCount;
-    if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) { delete c; return EU3D_ERR_HIP; }
+    // a blocking stream: work the caller issued to the null stream (where it produced the device
+    // buffers it hands over) completes before the context's kernels read them
+    if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamDefault)) != hipSuccess) { delete c; return EU3D_ERR_HIP; }
     for (int f = 0; f < EU3D_FIELD_COUNT; f++) {
         const size_t bytes = g.n3 * ncomp(f) * sizeof(double);
         if ((e = hipMalloc(&c->fl.f[f], bytes)) != hipSuccess) { eu3d_destroy(c); return EU3D_ERR_HIP; }
@@ -355,6 +357,61 @@ int eu3d_chem_get_schedule(eu3d_ctx* c, eu3d_chem_schedule* s) {
 double* eu3d_device_field(eu3d_ctx* c, int field) {
     if (!c || field < 0 || field >= EU3D_FIELD_COUNT) return nullptr;
     return c->fl.f[field];
+}
+
+// Mean duration of one kernel of the step, timed with HIP events on the context stream around
+// each of `reps` launches on identical inputs (the state the kernel writes is restored between
+// launches, outside the timed region). A profiling hook with no reference counterpart.
+int eu3d_time_kernel(eu3d_ctx* c, int kernel, int reps, int diff, int limiter, double* avg_ms) {
+    if (!c || reps < 1 || !avg_ms || kernel < 0 || kernel > EU3D_KERNEL_IMEX_FUSED || diff < 0 || diff > 1 ||
+        limiter < 0 || limiter > 4)
+        return fail(c, EU3D_ERR_ARG, "eu3d_time_kernel: bad arguments");
+    const size_t n3 = c->g.n3;
+    double* save = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto cleanup = [&]() {
+        if (save) (void)hipFree(save);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    };
+    const bool imex = kernel == EU3D_KERNEL_IMEX_FUSED;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess ||
+        (imex && hipMalloc(&save, n3 * (NC + 2) * sizeof(double)) != hipSuccess)) {
+        cleanup();
+        return fail(c, EU3D_ERR_HIP, "eu3d_time_kernel: allocation");
+    }
+    // k_imex_fused reads CS, T, E (and the RHS, which it leaves alone) and overwrites CS, T, E
+    double* cs = c->fl.f[EU3D_FIELD_CS];
+    double* T = c->fl.f[EU3D_FIELD_T];
+    double* E = c->fl.f[EU3D_FIELD_E];
+    const size_t b = n3 * sizeof(double);
+    auto copy3 = [&](bool to_save) -> hipError_t {
+        hipError_t e;
+        if ((e = hipMemcpyAsync(to_save ? save : cs, to_save ? cs : save, NC * b, hipMemcpyDeviceToDevice, c->stream)) != hipSuccess) return e;
+        if ((e = hipMemcpyAsync(to_save ? save + NC * n3 : T, to_save ? T : save + NC * n3, b, hipMemcpyDeviceToDevice, c->stream)) != hipSuccess) return e;
+        return hipMemcpyAsync(to_save ? save + (NC + 1) * n3 : E, to_save ? E : save + (NC + 1) * n3, b, hipMemcpyDeviceToDevice, c->stream);
+    };
+    double total = 0;
+    hipError_t e = imex ? copy3(true) : hipSuccess;
+    for (int r = 0; r < reps && e == hipSuccess; r++) {
+        if ((e = hipEventRecord(e0, c->stream)) != hipSuccess) break;
+        switch (kernel) {
+        case EU3D_KERNEL_FLUX_Z: e = launch_flux_z(c, diff, limiter); break;
+        case EU3D_KERNEL_FLUX_XY: e = launch_flux_xy(c, diff, limiter); break;
+        default: e = launch_imex_fused(c); break;
+        }
+        if (e != hipSuccess || (e = hipEventRecord(e1, c->stream)) != hipSuccess || (e = hipEventSynchronize(e1)) != hipSuccess)
+            break;
+        float ms = 0;
+        if ((e = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess) break;
+        total += ms;
+        if (imex) e = copy3(false);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    cleanup();
+    if (e != hipSuccess) return hip_fail(c, e, "eu3d_time_kernel");
+    *avg_ms = total / reps;
+    return EU3D_OK;
 }
 
 size_t eu3d_cells_with_ghosts(const eu3d_ctx* c) { return c ? c->g.n3 : 0; }

@@ -28,6 +28,8 @@ enum : uint32_t {
 };
 
 __host__ __device__ constexpr double RI(int s) { return R_UNIV / MW[s]; }   // src/Reaction.cpp:353-354
+// 1 / Mw_s, folded at compile time: divisions by the molar masses become multiplications
+__host__ __device__ constexpr double RMW(int s) { return 1.0 / MW[s]; }
 
 // ------------------------------------------------------------------ NASA-9 thermo
 struct TPow {
@@ -149,23 +151,23 @@ __device__ __forceinline__ Prim prim_from_cs(const double (&cs)[NC], double T_ol
     o.D = D;
     const double rD = 1.0 / D;
 #pragma unroll
-    for (int s = 0; s < NS; s++) o.Y[s] = cs[s] / D;
+    for (int s = 0; s < NS; s++) o.Y[s] = cs[s] * rD;
     o.U = cs[NS] * rD;
     o.V = cs[NS + 1] * rD;
     o.W = cs[NS + 2] * rD;
-    o.E = (cs[NS + 3] - 0.5 * D * (o.U * o.U + o.V * o.V + o.W * o.W)) / D;
+    o.E = (cs[NS + 3] - 0.5 * D * (o.U * o.U + o.V * o.V + o.W * o.W)) * rD;
     const MixFit m = mix_fit(o.Y, q & Q_C4);
     o.T = newton_T(m, T_old, (q & Q_F3) ? E_old : o.E, iters);
     double sy = 0;
 #pragma unroll
-    for (int s = 0; s < NS; s++) sy += o.Y[s] / MW[s];
+    for (int s = 0; s < NS; s++) sy += o.Y[s] * RMW(s);
     o.Wav = 1.0 / sy;
     o.Rgas = R_UNIV * 1000 / o.Wav;
     o.P = o.T * D * o.Rgas;
-    o.H = o.P / D + o.E;
+    o.H = o.P * rD + o.E;
     o.Cp = mix_cp(m, o.T) * 1000;
     o.Gamma = o.Cp / (o.Cp - R_UNIV * sy * 1000);
-    o.C = sqrt(o.Gamma * o.P / D);
+    o.C = sqrt(o.Gamma * o.P * rD);
     return o;
 }
 
@@ -175,12 +177,12 @@ __device__ __forceinline__ void partial_T(const Prim& o, uint32_t q, double (&pt
     const bool lo = o.T < 1000;
     const double CV = o.Cp - o.Rgas;
     const double ke = 0.5 * (o.U * o.U + o.V * o.V + o.W * o.W);
-    const double den = CV * o.D;
+    const double rden = 1.0 / (CV * o.D);
     const double hs = (q & Q_C7) ? 1.0 : 1e3;
 #pragma unroll
     for (int s = 0; s < NS; s++) {
         const double hi = RI(s) * h_R(s, lo, q & Q_C4, p) * hs;
-        pt[s] = (ke - hi + RI(s) * 1e3 * o.T) / den;
+        pt[s] = (ke - hi + RI(s) * 1e3 * o.T) * rden;
     }
 }
 
@@ -383,10 +385,10 @@ __device__ __forceinline__ void kinetics(double T, const double (&Mc)[NS], const
         if (TRAP) { o.P[s] = 0.0; o.Q[s] = 0.0; }
         if (DIAG) o.md[s] = 0.0;
     }
-    double rDr[NS];
+    double rDr[NS];   // 1 / Dr, or 0 where Dr = 0 (those species contribute no Jacobian term)
     if (DIAG) {
 #pragma unroll
-        for (int s = 0; s < NS; s++) rDr[s] = 1.0 / Dr[s];
+        for (int s = 0; s < NS; s++) rDr[s] = (Dr[s] != 0) ? 1.0 / Dr[s] : 0.0;
     }
     const double RT = R_UNIV * T;
     constexpr double P0R = P0_PA * 1e-6;
@@ -443,12 +445,9 @@ __device__ __forceinline__ void kinetics(double T, const double (&Mc)[NS], const
                 }
             }
             if (DIAG && dn != 0) {
-                double w1 = 0.0;
-                if (Dr[s] != 0) {
-                    const double a = nf ? (kf * (double)nf * rf) * rDr[s] : 0.0;
-                    const double b = nb ? (kb * (double)nb * rb) * rDr[s] : 0.0;
-                    w1 = a - b;
-                }
+                const double a = nf ? (kf * (double)nf * rf) * rDr[s] : 0.0;
+                const double b = nb ? (kb * (double)nb * rb) * rDr[s] : 0.0;
+                const double w1 = a - b;
                 double term = (w1 + dRdT * pts[s]) * tb;
                 if (TB[s][r] != 0.0) term += rr * (TB[s][r] / MW[s]);
                 o.md[s] += (double)dn * term;

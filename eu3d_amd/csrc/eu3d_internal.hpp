@@ -74,6 +74,8 @@ hipError_t launch_advection(eu3d_ctx* c, int diff, int limiter);
 hipError_t launch_cfl(eu3d_ctx* c, double cfl, double final_time);
 hipError_t launch_cfl_finish(eu3d_ctx* c, double cfl, double final_time);
 hipError_t launch_flux_fused(eu3d_ctx* c, int diff, int limiter);
+hipError_t launch_flux_z(eu3d_ctx* c, int diff, int limiter);
+hipError_t launch_flux_xy(eu3d_ctx* c, int diff, int limiter);
 hipError_t launch_imex_fused(eu3d_ctx* c);
 hipError_t launch_derive(eu3d_ctx* c);
 hipError_t launch_update_after_adv(eu3d_ctx* c);

@@ -48,7 +48,7 @@ __device__ __forceinline__ void store_prims(const Fields& F, const G3& g, size_t
 #pragma unroll
     for (int s = 0; s < NS; s++) {
         FLD(F, EU3D_FIELD_DI)[s * g.n3 + c] = cs[s];
-        FLD(F, EU3D_FIELD_MC)[s * g.n3 + c] = cs[s] / MW[s] * 1000;
+        FLD(F, EU3D_FIELD_MC)[s * g.n3 + c] = cs[s] * RMW(s) * 1000;
         FLD(F, EU3D_FIELD_YI)[s * g.n3 + c] = o.Y[s];
     }
 }
@@ -122,7 +122,7 @@ __global__ void k_field_initial(G3 g, Fields F, uint32_t q) {
     double sy = 0, cp = 0, h = 0, e = 0;
 #pragma unroll
     for (int s = 0; s < NS; s++) {
-        sy += Y[s] / MW[s];
+        sy += Y[s] * RMW(s);
         const double hi = RI(s) * h_R(s, lo, q & Q_C4, p);
         cp += RI(s) * cp_R(s, lo, p) * Y[s];
         h += Y[s] * hi;
@@ -144,7 +144,7 @@ __global__ void k_field_initial(G3 g, Fields F, uint32_t q) {
     for (int s = 0; s < NS; s++) {
         const double di = Y[s] * D;
         FLD(F, EU3D_FIELD_DI)[s * g.n3 + c] = di;
-        FLD(F, EU3D_FIELD_MC)[s * g.n3 + c] = di / MW[s] * 1000;
+        FLD(F, EU3D_FIELD_MC)[s * g.n3 + c] = di * RMW(s) * 1000;
         CS[s * g.n3 + c] = D * Y[s];
     }
     CS[NS * g.n3 + c] = D * U;
@@ -563,36 +563,46 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
     double v = 1e10;
     if (interior_ijk(g, i, j, k)) {
         const size_t c = IDX(g, i, j, k);
+        const double* CS0 = FLD(F, EU3D_FIELD_CS);
+        const double T0 = FLD(F, EU3D_FIELD_T)[c], E0 = FLD(F, EU3D_FIELD_E)[c];
+        KinOut ko;
+        {
+            double rho[NS], D = 0;
+#pragma unroll
+            for (int s = 0; s < NS; s++) {
+                rho[s] = CS0[s * g.n3 + c];
+                D += rho[s];
+            }
+            Prim o;
+            o.D = D;
+            const double rD = 1.0 / D;
+            o.U = CS0[NS * g.n3 + c] * rD;
+            o.V = CS0[(NS + 1) * g.n3 + c] * rD;
+            o.W = CS0[(NS + 2) * g.n3 + c] * rD;
+            o.T = T0;
+            double Y[NS], Mc[NS], Dr[NS], sy = 0;
+#pragma unroll
+            for (int s = 0; s < NS; s++) {
+                Y[s] = rho[s] * rD;
+                sy += Y[s] * RMW(s);
+                Mc[s] = (rho[s] * RMW(s) * 1000) * 1e-6;
+                Dr[s] = rho[s] * 1e-3;
+            }
+            o.Rgas = R_UNIV * 1000 * sy;
+            o.Cp = mix_cp(mix_fit(Y, q & Q_C4), T0) * 1000;
+            double pt[NS];
+            partial_T(o, q, pt);
+            const double tsc = (q & Q_C6) ? 1.0 : 1e3;
+#pragma unroll
+            for (int s = 0; s < NS; s++) pt[s] *= tsc;
+            kinetics<false, true>(T0, Mc, Dr, pt, 2, q & Q_C4, ko);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        // the conserved state is read again here rather than held in registers through the
+        // kinetics (its second read mostly hits L2)
+        asm volatile("" ::: "memory");
         double cs[NC];
         load_cs(F, g, c, cs);
-        const double T0 = FLD(F, EU3D_FIELD_T)[c], E0 = FLD(F, EU3D_FIELD_E)[c];
-        double D = 0;
-#pragma unroll
-        for (int s = 0; s < NS; s++) D += cs[s];
-        Prim o;
-        o.D = D;
-        o.U = cs[NS] / D;
-        o.V = cs[NS + 1] / D;
-        o.W = cs[NS + 2] / D;
-        o.T = T0;
-        double Y[NS], Mc[NS], Dr[NS], sy = 0;
-#pragma unroll
-        for (int s = 0; s < NS; s++) {
-            Y[s] = cs[s] / D;
-            sy += Y[s] / MW[s];
-            Mc[s] = (cs[s] / MW[s] * 1000) * 1e-6;
-            Dr[s] = cs[s] * 1e-3;
-        }
-        o.Rgas = R_UNIV * 1000 / (1.0 / sy);
-        o.Cp = mix_cp(mix_fit(Y, q & Q_C4), T0) * 1000;
-        double pt[NS];
-        partial_T(o, q, pt);
-        const double tsc = (q & Q_C6) ? 1.0 : 1e3;
-#pragma unroll
-        for (int s = 0; s < NS; s++) pt[s] *= tsc;
-        KinOut ko;
-        kinetics<false, true>(T0, Mc, Dr, pt, 2, q & Q_C4, ko);
-        __builtin_amdgcn_sched_barrier(0);
         const double dt = sc->dt;
         const double* R = FLD(F, EU3D_FIELD_RHS);
 #pragma unroll
@@ -634,9 +644,9 @@ __global__ void __launch_bounds__(256) k_derive(G3 g, Fields F, uint32_t q) {
 #pragma unroll
     for (int s = 0; s < NS; s++) {
         Y[s] = cs[s] / D;
-        sy += Y[s] / MW[s];
+        sy += Y[s] * RMW(s);
         FLD(F, EU3D_FIELD_DI)[s * g.n3 + c] = cs[s];
-        FLD(F, EU3D_FIELD_MC)[s * g.n3 + c] = cs[s] / MW[s] * 1000;
+        FLD(F, EU3D_FIELD_MC)[s * g.n3 + c] = cs[s] * RMW(s) * 1000;
     }
     Prim o;
     o.D = D;
@@ -769,7 +779,7 @@ struct ChemCell {
 __device__ __forceinline__ void chem_inputs(const ChemCell& c, double (&Mc)[NS], double (&Dr)[NS]) {
 #pragma unroll
     for (int s = 0; s < NS; s++) {
-        Mc[s] = (c.Di[s] / MW[s] * 1000) * 1e-6;
+        Mc[s] = (c.Di[s] * RMW(s) * 1000) * 1e-6;
         Dr[s] = c.Di[s] * 1e-3;
     }
 }
@@ -1183,10 +1193,14 @@ static void launch_xy(eu3d_ctx* c, int diff, int limiter) {
     k_flux_xy<X, Y><<<gxy, dim3(X, Y), 0, c->stream>>>(g, c->fl, diff, limiter, c->quirks, c->xnode, c->ynode, c->sc);
 }
 
-hipError_t launch_flux_fused(eu3d_ctx* c, int diff, int limiter) {
+hipError_t launch_flux_z(eu3d_ctx* c, int diff, int limiter) {
     const G3& g = c->g;
     dim3 gz((g.ni + ZX - 1) / ZX, (g.nj + ZY - 1) / ZY, (g.nk + ZCH - 1) / ZCH);
     k_flux_z<<<gz, dim3(ZX, ZY), 0, c->stream>>>(g, c->fl, diff, limiter, c->quirks, c->znode, c->sc);
+    return hipGetLastError();
+}
+
+hipError_t launch_flux_xy(eu3d_ctx* c, int diff, int limiter) {
     // tile shape of the x/y pass (threads in x, y); 32 x 8 measured fastest on bubble3d
     // 384x128x128 among 64x8, 64x6, 64x4, 32x8, 32x4 (the others stay selectable)
     switch (c->flux_variant) {
@@ -1200,6 +1214,11 @@ hipError_t launch_flux_fused(eu3d_ctx* c, int diff, int limiter) {
     default: launch_xy<32, 8>(c, diff, limiter);
     }
     return hipGetLastError();
+}
+
+hipError_t launch_flux_fused(eu3d_ctx* c, int diff, int limiter) {
+    hipError_t e = launch_flux_z(c, diff, limiter);
+    return e != hipSuccess ? e : launch_flux_xy(c, diff, limiter);
 }
 
 hipError_t launch_imex_fused(eu3d_ctx* c) {

@@ -105,7 +105,7 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs a) {
     __shared__ bf16_t As[BM][LDK];
     __shared__ bf16_t Bs[BN][LDK];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+    const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;   // rows on x: no 65535 limit on M
     const int kbeg = blockIdx.z * a.kchunk, kend = min(a.K, kbeg + a.kchunk);
     const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
     const int fr = lane & 15, fk = (lane >> 4) * 8;
@@ -213,7 +213,7 @@ __global__ void k_adam(float* __restrict__ w, float* __restrict__ m, float* __re
     const int o = i / n_in, k = i % n_in;
     if (o >= real_out || k >= real_in) {
         w[i] = 0.f;
-        wt_b[i] = 0;
+        if (wt_b) wt_b[i] = 0;
         if (w_b) w_b[(size_t)k * n_out + o] = 0;
         return;
     }
@@ -225,7 +225,7 @@ __global__ void k_adam(float* __restrict__ w, float* __restrict__ m, float* __re
     const float wi = w[i] - lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
     w[i] = wi;
     const bf16_t h = f2bf(wi);
-    wt_b[i] = h;
+    if (wt_b) wt_b[i] = h;
     if (w_b) w_b[(size_t)k * n_out + o] = h;
 }
 
@@ -278,7 +278,7 @@ hipError_t gemm(hipStream_t s, const GemmArgs& a0, int splits = 1) {
     GemmArgs a = a0;
     a.kchunk = ((a.K + splits - 1) / splits + BK - 1) / BK * BK;
     const int sp = (a.K + a.kchunk - 1) / a.kchunk;
-    dim3 grid((a.N + BN - 1) / BN, (a.M + BM - 1) / BM, sp);
+    dim3 grid((a.M + BM - 1) / BM, (a.N + BN - 1) / BN, sp);
     k_gemm<AKC, BKC, EPI><<<grid, 256, 0, s>>>(a);
     return hipGetLastError();
 }
@@ -589,7 +589,8 @@ int eu3d_mlp_create(eu3d_mlp** out, uint64_t seed) {
     eu3d_mlp* m = new eu3d_mlp();
     const int din[5] = {NIN, NH, NH, NH, NH}, dout[5] = {NH, NH, NH, NH, NOUT};
     const int rin[5] = {NIN_REAL, NH, NH, NH, NH}, rout[5] = {NH, NH, NH, NH, NOUT_REAL};
-    if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess) { delete m; return EU3D_ERR_HIP; }
+    // blocking stream: ordered after the caller's null-stream work on the buffers it passes in
+    if (hipStreamCreateWithFlags(&m->stream, hipStreamDefault) != hipSuccess) { delete m; return EU3D_ERR_HIP; }
     uint64_t st = seed * 6364136223846793005ull + 1442695040888963407ull;
     auto rnd = [&]() {   // uniform (-1, 1) from a 64-bit LCG
         st = st * 6364136223846793005ull + 1442695040888963407ull;
@@ -694,9 +695,9 @@ int eu3d_mlp_train_step(eu3d_mlp* m, int n, const void* feats_bf16, const float*
     if (!m || n <= 0 || !feats_bf16 || !targets) return EU3D_ERR_ARG;
     hipStream_t s = m->stream;
     if (ensure_ws(m, n) != hipSuccess) return EU3D_ERR_HIP;
-    const int sp = splits_for(n);
-    if (ensure_part(m, std::max<size_t>((size_t)sp * NH * NH, (size_t)((n + 255) / 256 + 1) * NH)) != hipSuccess)
-        return EU3D_ERR_HIP;
+    // partials: split-K dW [sp x 256 x 256] followed by the bias column sums [nb x 256]
+    const int sp = splits_for(n), rpb = 4096, nb = (n + rpb - 1) / rpb;
+    if (ensure_part(m, (size_t)sp * NH * NH + (size_t)nb * NH) != hipSuccess) return EU3D_ERR_HIP;
     if (hipMemcpyAsync(m->act[0], feats_bf16, (size_t)n * NIN * 2, hipMemcpyDeviceToDevice, s) != hipSuccess)
         return EU3D_ERR_HIP;
     if (forward(m, n, true) != hipSuccess) return EU3D_ERR_HIP;
@@ -724,8 +725,7 @@ int eu3d_mlp_train_step(eu3d_mlp* m, int n, const void* feats_bf16, const float*
         const int kch = ((n + sp - 1) / sp + BK - 1) / BK * BK, nsp = (n + kch - 1) / kch;
         k_reduce_splits<<<(nw + 255) / 256, 256, 0, s>>>(m->part, m->gw[l], nw, nsp);
         // db = column sums of dZ
-        const int rpb = 4096, nb = (n + rpb - 1) / rpb;
-        float* cpart = m->part + (size_t)sp * NH * NH - (size_t)nb * m->dout[l];
+        float* cpart = m->part + (size_t)sp * NH * NH;
         k_colsum_bf16<<<dim3(nb, (m->dout[l] + 255) / 256), 256, 0, s>>>(dz, n, m->dout[l], m->dout[l], rpb, cpart);
         k_reduce_splits<<<(m->dout[l] + 255) / 256, 256, 0, s>>>(cpart, m->gb[l], m->dout[l], nb);
         if (l > 0) {
@@ -750,8 +750,8 @@ int eu3d_mlp_train_step(eu3d_mlp* m, int n, const void* feats_bf16, const float*
                                                     m->rout[l], m->rin[l], lr, b1, b2, 1e-8f, bc1, bc2, wd, m->wt[l],
                                                     m->wn[l]);
             k_adam<<<(m->dout[l] + 255) / 256, 256, 0, s>>>(m->b[l], m->mb[l], m->vb[l], m->gb[l], m->dout[l], 1,
-                                                            m->rout[l], 1, lr, b1, b2, 1e-8f, bc1, bc2, 0.f,
-                                                            reinterpret_cast<bf16_t*>(m->part), nullptr);
+                                                            m->rout[l], 1, lr, b1, b2, 1e-8f, bc1, bc2, 0.f, nullptr,
+                                                            nullptr);
         }
     }
     return hipStreamSynchronize(s) == hipSuccess ? EU3D_OK : EU3D_ERR_HIP;

@@ -94,6 +94,23 @@ def harvest_states(ctx: Context, H: Harvest, n: int, dts, seed: int = 0, T_range
         H.add(r0, r, t0, dt)
 
 
+def harvest_trajectories(ctx: Context, H: Harvest, n: int, dts, nsteps: int, seed: int = 0, keep_every: int = 1):
+    """FP64 chemistry trajectories: seeded states (configs.chem_states) advanced nsteps times by
+    each dt, every keep_every-th (before, after) pair harvested, so that radical pools and the
+    ignition transients enter the data."""
+    torch = _torch()
+    for i, dt in enumerate(dts):
+        rho, T, E, _ = configs.chem_states(n, seed=seed + 7919 * i)
+        r = torch.tensor(rho.T.copy(), device="cuda")
+        t = torch.tensor(T, device="cuda")
+        e = torch.tensor(E, device="cuda")
+        for s in range(nsteps):
+            r0, t0 = r.clone(), t.clone()
+            fp64_chem(ctx, r, t, e, dt)
+            if s % keep_every == 0:
+                H.add(r0, r.clone(), t0, dt)
+
+
 def harvest_run(H: Harvest, wl, nsteps: int, every: int, cfl=0.5, ramp=(0.05, 20), min_T=600.0, max_cells=1 << 20):
     """Snapshots of a plain FP64 run of a workload: every `every` steps the interior state after
     the flow update is taken as a chemistry input and advanced by the step's dt with the FP64
@@ -131,7 +148,10 @@ def fit_params(rho0, rho1, T0, dt, hull_margin=0.25, dT_max=40.0, elem_tol=2e-3,
     f = raw_features(rho0, T0, dt)
     dY = (rho1 / rho1.sum(0) - rho0 / rho0.sum(0)).T
     p = SurrogateParams()
-    fm, fs = f.mean(0), f.std(0).clamp(min=1e-6)
+    # the kernels normalise with the FP32 copies of these statistics: round first, so the hull
+    # below is taken on the features exactly as the kernels compute them
+    fm = f.mean(0).float().double()
+    fs = f.std(0).clamp(min=1e-3).float().double()
     om, osd = dY.mean(0), dY.std(0).clamp(min=1e-12)
     fn = (f - fm) / fs
     lo, hi = fn.min(0).values, fn.max(0).values

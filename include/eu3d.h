@@ -13,7 +13,10 @@
  *
  * Return value: 0 on success, a negative EU3D_ERR_* code on failure; eu3d_last_error() gives
  * the message. All work is ordered on the context's HIP stream. Entry points that return data
- * to the host synchronise the stream; the others return once the work is queued.
+ * to the host synchronise the stream; the others return once the work is queued. The context
+ * and network streams are blocking streams: they start after work the caller queued on the null
+ * stream. A caller that produces the device buffers it passes in on another non-blocking
+ * stream synchronises that stream first.
  */
 #ifndef EU3D_H
 #define EU3D_H
@@ -233,6 +236,16 @@ int eu3d_get_surrogate_stats(eu3d_ctx *ctx, eu3d_surrogate_stats *stats);
 double *eu3d_device_field(eu3d_ctx *ctx, int field);
 size_t eu3d_cells_with_ghosts(const eu3d_ctx *ctx);
 void *eu3d_stream(eu3d_ctx *ctx);
+
+/* Profiling hook (no reference counterpart): mean duration in ms of one kernel of the plain step,
+ * timed with HIP events on the context stream around each of `reps` launches on identical
+ * inputs. FLUX_Z / FLUX_XY are the two passes of eu3d_advection_fused; IMEX_FUSED is the fused
+ * Diagonalized + Update_IMEX + Update_after_CS cell kernel of eu3d_step (its CS, T, E inputs
+ * are restored between launches, outside the timed region). */
+#define EU3D_KERNEL_FLUX_Z 0
+#define EU3D_KERNEL_FLUX_XY 1
+#define EU3D_KERNEL_IMEX_FUSED 2
+int eu3d_time_kernel(eu3d_ctx *ctx, int kernel, int reps, int diff, int limiter, double *avg_ms);
 
 #ifdef __cplusplus
 }
