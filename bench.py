@@ -82,8 +82,8 @@ def cpu_baseline(grid, steps):
     o.step(steps, **kw)
     el = time.perf_counter() - t0
     return {"value": wl.ncells * steps / el, "unit": "cell-updates/s", "cores": cores, "kind": "port",
-            "sample": f"bubble3d {ni}x{nj}x{nk} (same generator, bubble radius scaled), {steps} IMEX steps "
-                      f"after 1 warm-up step, {el:.1f} s"}
+            "sample": f"bubble3d {ni}x{nj}x{nk} (same generator; bubble radius 25 cells scaled with ni/384), "
+                      f"{steps} IMEX steps after 1 warm-up step, {el:.1f} s"}
 
 
 def read_profile():

@@ -415,11 +415,12 @@ __global__ void __launch_bounds__(256) k_explicit(G3 g, Fields F, uint32_t q) {
 //   k_flux_z (first): one thread per (x, y) column marching up a chunk of z planes with a
 //     4-plane register window (each primitive is read from HBM once, the next plane prefetched);
 //     it stores the z term dt/dz dQ into RHS without reading it.
-//   k_flux_xy (second): one z plane per block. A 64 x 6 thread block stages the primitives of its
-//     tile plus the two-cell halo (67 x 9 cells x 16 variables, 77 KB, two blocks per CU) in LDS
-//     with coalesced loads, computes the x face left of and the y face below every cell from
-//     LDS, exchanges the face fluxes through the same LDS and writes
-//     RHS = (-dt/dx dF - dt/dy dG) - dt/dz dQ for 63 x 5 cells, EE's order of operations.
+//   k_flux_xy (second): one z plane per block. An X x Y thread block (32 x 8 by default) stages
+//     the primitives of its tile plus the two-cell halo ((X+3) x (Y+3) cells x 16 variables,
+//     48 KB at 32 x 8, three blocks per CU) in LDS with coalesced loads, computes the x face left
+//     of and the y face below every cell from LDS, exchanges the face fluxes through the same LDS
+//     and writes RHS = (-dt/dx dF - dt/dy dG) - dt/dz dQ for (X-1) x (Y-1) cells, EE's order of
+//     operations.
 constexpr int NV = 16;
 constexpr int ZX = 64, ZY = 4, ZCH = 32;
 
