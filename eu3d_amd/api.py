@@ -43,7 +43,7 @@ EXPORTED = [
     "eu3d_mlp_create", "eu3d_mlp_destroy", "eu3d_mlp_param_count", "eu3d_mlp_get_params", "eu3d_mlp_set_params",
     "eu3d_mlp_get_grads", "eu3d_mlp_forward", "eu3d_mlp_train_step", "eu3d_gemm_bf16", "eu3d_mlp_stream",
     "eu3d_mlp_set_surrogate", "eu3d_surrogate_features", "eu3d_surrogate_targets", "eu3d_surrogate_batch",
-    "eu3d_set_surrogate", "eu3d_get_surrogate_stats",
+    "eu3d_set_surrogate", "eu3d_get_surrogate_stats", "eu3d_surrogate_chem",
 ]
 CHEM_SURROGATE = 4
 

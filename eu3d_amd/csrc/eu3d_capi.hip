@@ -314,6 +314,7 @@ int eu3d_step(eu3d_ctx* c, const eu3d_step_params* p, int nsteps) {
     if (p->diff < 0 || p->diff > 1 || p->limiter < 0 || p->limiter > 4) return fail(c, EU3D_ERR_ARG, "bad scheme");
     if (p->chem == EU3D_CHEM_SURROGATE && !c->mlp) return fail(c, EU3D_ERR_ARG, "no surrogate attached");
     for (int n = 0; n < nsteps; n++) HIPCHK(c, launch_step(c, p, n == 0), "step");
+    if (p->chem == EU3D_CHEM_TRAP_ADAPTIVE) c->stats.cells = (long long)c->g.ni * c->g.nj * c->g.nk;
     if (p->chem == EU3D_CHEM_SURROGATE && nsteps > 0) HIPCHK(c, surrogate_grid_stats(c), "surrogate stats");
     if (nsteps > 0) {
         HIPCHK(c, launch_derive(c), "step derive");
@@ -324,8 +325,9 @@ int eu3d_step(eu3d_ctx* c, const eu3d_step_params* p, int nsteps) {
 
 int eu3d_get_chem_stats(eu3d_ctx* c, eu3d_chem_stats* out) {
     if (!c || !out) return EU3D_ERR_ARG;
+    const int rc = chem_status(c);   // counters of the last chemistry launch, read back now
     *out = c->stats;
-    return EU3D_OK;
+    return rc;
 }
 
 int eu3d_chem_batch(eu3d_ctx* c, long long n, double* rho_s, double* T, double* E, double dtm, int nchem_max,

@@ -845,6 +845,17 @@ int eu3d_surrogate_batch(eu3d_ctx* c, eu3d_mlp* m, long long n, double* rho, dou
     return sur_stats(c, st, n);
 }
 
+// Surrogate chemistry over the grid's interior with the FP64 fallback, on the context's dt (the
+// chemistry half of a step with EU3D_CHEM_SURROGATE, after eu3d_update_after_adv).
+int eu3d_surrogate_chem(eu3d_ctx* c, int nchem_max, eu3d_surrogate_stats* st) {
+    if (!c || !c->mlp) return EU3D_ERR_ARG;
+    if (eu3d::launch_surrogate_grid(c, nchem_max) != hipSuccess) return EU3D_ERR_HIP;
+    const long long n = (long long)c->g.ni * c->g.nj * c->g.nk;
+    const int rc = sur_stats(c, &c->sur_stats, n);
+    if (st) *st = c->sur_stats;
+    return rc;
+}
+
 int eu3d_set_surrogate(eu3d_ctx* c, eu3d_mlp* m) {
     if (!c) return EU3D_ERR_ARG;
     if (m && !m->sp_set) return EU3D_ERR_ARG;

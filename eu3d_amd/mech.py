@@ -155,6 +155,37 @@ def parse_mechanism(mech_path: str, thermo_path: str) -> Mechanism:
                      mw=mw, mr_init=mr)
 
 
+def write_text(mech: Mechanism, mech_path: str, thermo_path: str) -> None:
+    """Write a mechanism + thermo pair in the line layout parse_mechanism (and the reference
+    reader) expects; every number is written so that it reads back exactly."""
+    ns, nr = mech.ns, mech.nr
+    tb_rows = [r for r in range(nr) if np.any(mech.react_tb[:, r] != 0)]
+    L = ["! mechanism: species, reactions, third-body rows", "NS\tNR\tTB", f"{ns}\t{nr}\t{len(tb_rows)}", "!",
+         "\t".join(mech.species), "!", "! stoichiometry: forward row, backward row per reaction",
+         "\t".join(mech.species)]
+    for r in range(nr):
+        L.append("\t".join(f"{int(v)}." for v in mech.stoi_f[:, r]))
+        L.append("\t".join(f"{int(v)}." for v in mech.stoi_b[:, r]))
+    L += ["!", "! reaction A b E[cal/mol]"]
+    L += [f"{r + 1}\t{float(mech.af[r])!r}\t{float(mech.bf[r])!r}\t{float(mech.eaf[r])!r}" for r in range(nr)]
+    L += ["!", "! third-body efficiencies", "!"]
+    L += [f"{r + 1}\t" + "\t".join(f"{int(v)}." for v in mech.react_tb[:, r]) for r in tb_rows]
+    L += ["!", "! initial molar ratios"]
+    mr = mech.mr_init if len(mech.mr_init) == ns else np.zeros(ns)
+    L.append("\t".join(repr(float(v)) for v in mr))
+    with open(mech_path, "w") as f:
+        f.write("\n".join(L) + "\n")
+    T = ["! NASA-9 fits: per species a1..a7 b1 b2 (low range), the same (high range), Mw"] + ["!"] * 13
+    T.append(mech.species[0])
+    for j in range(ns):
+        c = [*mech.coeff0[:, j], *mech.coeff1[:, j], mech.mw[j]]
+        rows = [c[0:5], c[5:9], c[9:14], c[14:18], c[18:19]]
+        T += ["  ".join(repr(float(v)) for v in row) for row in rows]
+        T.append(mech.species[j + 1] if j + 1 < ns else "END")
+    with open(thermo_path, "w") as f:
+        f.write("\n".join(T) + "\n")
+
+
 def load_default() -> Mechanism:
     """The packed 9-species / 19-reaction H2-air set (parsed from input/H2_JA919.txt and
     input/thermo.dat with :func:`parse_mechanism`)."""

@@ -230,6 +230,10 @@ int eu3d_surrogate_batch(eu3d_ctx *ctx, eu3d_mlp *m, long long n, double *rho_s,
                          int nchem_max, eu3d_surrogate_stats *stats);
 /* Attach a trained surrogate to a context for eu3d_step with EU3D_CHEM_SURROGATE (NULL detaches). */
 int eu3d_set_surrogate(eu3d_ctx *ctx, eu3d_mlp *m);
+/* The chemistry half of a surrogate step on the grid's interior (after eu3d_update_after_adv),
+ * over the context's dt, with the FP64 adaptive trapezoid on every rejected cell. Takes the
+ * place of the reference's DNN chemistry slot (src/Euler.cpp:773-780). */
+int eu3d_surrogate_chem(eu3d_ctx *ctx, int nchem_max, eu3d_surrogate_stats *stats);
 int eu3d_get_surrogate_stats(eu3d_ctx *ctx, eu3d_surrogate_stats *stats);
 
 /* Device memory of a field (SoA, x fastest; component c at offset c * n3) for zero-copy use. */

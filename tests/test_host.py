@@ -1,0 +1,213 @@
+"""The C++ host layer (eu3d_amd/host): deck, grid and mechanism readers on the CPU, and the
+deck-driven driver (the reference's Euler flow over the C ABI) against the oracle on the GPU."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from tests.common import compare, say
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+RUN = os.path.join(ROOT, "eu3d_amd", "eu3d_run")
+REF_INPUT = "/root/reference/input"
+
+
+def _need_driver():
+    if not os.path.exists(RUN):
+        pytest.skip("eu3d_run not built (run __graft_entry__.build())")
+
+
+def _mech_files(tmp_path):
+    from eu3d_amd import mech
+    m, t = str(tmp_path / "mech.txt"), str(tmp_path / "thermo.dat")
+    mech.write_text(mech.load_default(), m, t)
+    return m, t
+
+
+def _decks(tmp_path, grid, reaction="IMEX", cfl=0.05, extra=""):
+    m, t = _mech_files(tmp_path)
+    g = tmp_path / "grid.txt"
+    g.write_text("#grid\n" + "\n".join(f"{k}\t{v}" for k, v in grid.items()) + "\n")
+    c = tmp_path / "ctrl.txt"
+    c.write_text(f"""#The grid file
+gridname\t{g}
+
+#chemical reaction kinetic model
+reaction-model\t{m}
+Thermofile\t{t}
+
+#The final time
+Final-Time\t1.0
+cfl\t{cfl}
+dt\t3e-8
+count\t1
+difference\tMUSCL_2
+time-adv\tEE
+reaction\t{reaction}
+{extra}""")
+    return str(c)
+
+
+def _print_decks(ctrl):
+    r = subprocess.run([RUN, ctrl, "--print-decks"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    return json.loads(r.stdout)
+
+
+GRID_1D = {"total_ni": 160, "xa": 0.0, "xb": 159 * 5e-5, "total_nj": 2, "ya": 0.0, "yb": 5e-5,
+           "total_nk": 2, "za": 0.0, "zb": 5e-5, "bc": 2}
+
+
+def test_deck_reader_semantics(tmp_path):
+    """Values are kept as 6-significant-digit text, tokens carry over to a line with fewer tokens,
+    '#' lines are skipped, and a non-numeric value reads as 0 through getdoubleParameter."""
+    _need_driver()
+    ctrl = _decks(tmp_path, GRID_1D, extra="precise\t0.123456789\nlonely\n#comment\tline\nword\tabc\n")
+    d = _print_decks(ctrl)
+    deck, control = d["deck"], d["control"]
+    assert deck["precise"] == "0.123457"
+    assert deck["Final-Time"] == "1" and deck["cfl"] == "0.05" and deck["dt"] == "3e-08"
+    assert deck["lonely"] == "0.123457"          # value carried over from the line before
+    assert "#comment" not in deck and deck["word"] == "abc"
+    assert control["cfl"] == 0.05 and control["dt"] == 0.0   # dt is read only when cfl == 0
+    assert control["Reaction_Sch"] == 1 and control["Diff_Sch"] == 1 and control["TimeAdv_Sch"] == 0
+
+
+def test_mesh_follows_reference_rule(tmp_path):
+    """Mesh::Initial + Boundary: nodes xa + (i - bc) dx, ghosts mirrored through the end nodes."""
+    _need_driver()
+    grid = {"total_ni": 241, "xa": 0.0, "xb": 0.17, "total_nj": 81, "ya": 0.0, "yb": 0.08,
+            "total_nk": 2, "za": 0.0, "zb": 0.02, "bc": 2}
+    d = _print_decks(_decks(tmp_path, grid))["mesh"]
+    assert (d["ni"], d["nj"], d["nk"], d["bc"]) == (241, 81, 2, 2)
+    for key, n, a, b in (("xnode", 241, 0.0, 0.17), ("ynode", 81, 0.0, 0.08), ("znode", 2, 0.0, 0.02)):
+        dx = (b - a) / (n - 1)
+        x = np.zeros(n + 4)
+        x[2:n + 2] = a + np.arange(n) * dx
+        x[0], x[1] = 2 * x[2] - x[4], 2 * x[2] - x[3]
+        x[n + 2], x[n + 3] = 2 * x[n + 1] - x[n], 2 * x[n + 1] - x[n - 1]
+        assert np.array_equal(np.array(d[key]), x), key
+
+
+def test_mechanism_reader_matches(tmp_path):
+    """The C++ ReactionRead gives the packed mechanism exactly, from the text files written by
+    mech.write_text and, where the reference tree is present, from its own input files."""
+    _need_driver()
+    from eu3d_amd import mech
+    ref = mech.load_default()
+    cases = [_decks(tmp_path, GRID_1D)]
+    if os.path.exists(os.path.join(REF_INPUT, "H2_JA919.txt")):
+        p = tmp_path / "ref"
+        p.mkdir()
+        g = tmp_path / "grid.txt"
+        c = p / "ctrl.txt"
+        c.write_text(f"gridname\t{g}\nreaction-model\t{REF_INPUT}/H2_JA919.txt\nThermofile\t{REF_INPUT}/thermo.dat\n"
+                     "Final-Time\t1\ncfl\t0.5\ncount\t1\ndifference\tMUSCL_2\ntime-adv\tEE\nreaction\tIMEX\n")
+        cases.append(str(c))
+    for ctrl in cases:
+        m = _print_decks(ctrl)["mech"]
+        assert (m["NS"], m["NR"], m["TB"]) == (ref.ns, ref.nr, ref.tb)
+        assert m["species"] == list(ref.species)
+        for k, want in (("stoi_f", ref.stoi_f), ("stoi_b", ref.stoi_b), ("react_tb", ref.react_tb), ("af", ref.af),
+                        ("bf", ref.bf), ("eaf", ref.eaf), ("coeff0", ref.coeff0), ("coeff1", ref.coeff1),
+                        ("mw", ref.mw)):
+            assert np.array_equal(np.array(m[k]).reshape(want.shape), want), (ctrl, k)
+
+
+def _ic_file(tmp_path, prims):
+    p = tmp_path / "ic.bin"
+    with open(p, "wb") as f:
+        for k in ("T", "P", "U", "V", "W", "YI"):
+            np.ascontiguousarray(prims[k], dtype=np.float64).tofile(f)
+    return str(p)
+
+
+def _zn_prims(ni, n_driver=10):
+    """The 1-D detonation IC on the deck's 2 x 2 transverse nodes (uniform in y and z)."""
+    from eu3d_amd import configs
+    wl = configs.zn1d(ni=ni, n_driver=n_driver)
+    out = {}
+    for k, v in wl.prims.items():
+        out[k] = np.repeat(np.repeat(v[:, 2:3, 2:3], 6, axis=1), 6, axis=2) if v.ndim == 3 else \
+            np.repeat(np.repeat(v[:, 2:3, 2:3, :], 6, axis=1), 6, axis=2)
+    return out
+
+
+def _run(ctrl, ic, args, tmp_path):
+    dump = str(tmp_path / "dump.bin")
+    cmd = [RUN, ctrl, "--ic", ic, "--out", str(tmp_path), "--dump", dump, *args]
+    say("run", " ".join(cmd[1:]))
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return dump
+
+
+def _oracle(decks, prims, nsteps, **step):
+    import oracle
+    m = decks["mesh"]
+    o = oracle.OracleFlow(np.array(m["xnode"]), np.array(m["ynode"]), np.array(m["znode"]), quirks=0)
+    for k, v in prims.items():
+        o.set(k, v)
+    o.field_initial()
+    o.step(nsteps, **step)
+    return o
+
+
+def _check_dump(dump, o, tol=1e-10):
+    n3 = o.get("T").size
+    raw = np.fromfile(dump, dtype=np.float64)
+    cs, T = raw[:n3 * 13].reshape(o.get("CS").shape), raw[n3 * 13:].reshape(o.get("T").shape)
+
+    class G:   # the driver's result seen through the comparison helper's interface
+        def get(self, k):
+            return {"CS": cs, "T": T}[k]
+    compare(G(), o, ["CS", "T"], tol=tol)
+
+
+@pytest.mark.gpu
+def test_driver_stepwise_trapezoid_adaptive(tmp_path):
+    """Euler::Computing method by method (CFL, ghosts, advection, Update_after_Adv, adaptive
+    trapezoid) on the 1-D detonation tube equals the oracle's composition after 30 steps, and
+    Output writes the Tecplot block file."""
+    _need_driver()
+    ctrl = _decks(tmp_path, GRID_1D, reaction="Trapezoid_Adaptive", cfl=0.05)
+    decks = _print_decks(ctrl)
+    prims = _zn_prims(160)
+    dump = _run(ctrl, _ic_file(tmp_path, prims), ["--bc", "2,1,1,1,1,1", "--steps", "30"], tmp_path)
+    o = _oracle(decks, prims, 30, cfl=0.05, bc_kinds=(2, 1, 1, 1, 1, 1), rk=1, chem=3)
+    _check_dump(dump, o)
+    tec = tmp_path / "SeriesTrapezoid_Adaptive0_0.dat"
+    lines = tec.read_text().splitlines()
+    assert lines[0].startswith("Variables = X,Y,Z,T,D,P,U,V,Ma,Y1")
+    assert lines[1] == "ZONE I=160\tJ=2\tK=2" and lines[2] == "datapacking=block"
+    assert len(lines) == 3 + 18 * 160 * 2 * 2
+
+
+@pytest.mark.gpu
+def test_driver_fused_rk2(tmp_path):
+    """The same tube run as one eu3d_step call with SSP-RK2 stages equals the oracle's RK2 step."""
+    _need_driver()
+    ctrl = _decks(tmp_path, GRID_1D, reaction="Trapezoid_Adaptive", cfl=0.05)
+    decks = _print_decks(ctrl)
+    prims = _zn_prims(160)
+    dump = _run(ctrl, _ic_file(tmp_path, prims), ["--bc", "2,1,1,1,1,1", "--steps", "30", "--fused", "2"], tmp_path)
+    o = _oracle(decks, prims, 30, cfl=0.05, bc_kinds=(2, 1, 1, 1, 1, 1), rk=2, chem=3)
+    _check_dump(dump, o)
+
+
+@pytest.mark.gpu
+def test_driver_stepwise_imex_bubble(tmp_path):
+    """Euler::Computing with IMEX (Diagonalized + Update_IMEX) on a small shock-bubble case."""
+    _need_driver()
+    from tests.common import small_bubble
+    n = (24, 12, 10)
+    grid = {"total_ni": n[0], "xa": 0.0, "xb": (n[0] - 1) * 1e-4, "total_nj": n[1], "ya": 0.0,
+            "yb": (n[1] - 1) * 1e-4, "total_nk": n[2], "za": 0.0, "zb": (n[2] - 1) * 1e-4, "bc": 2}
+    ctrl = _decks(tmp_path, grid, reaction="IMEX", cfl=0.3)
+    decks = _print_decks(ctrl)
+    wl = small_bubble()
+    dump = _run(ctrl, _ic_file(tmp_path, wl.prims), ["--bc", "1,1,2,2,2,2", "--steps", "5"], tmp_path)
+    o = _oracle(decks, wl.prims, 5, cfl=0.3, bc_kinds=(1, 1, 2, 2, 2, 2), rk=1, chem=1)
+    _check_dump(dump, o)
