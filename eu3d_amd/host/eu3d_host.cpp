@@ -79,10 +79,14 @@ void Mesh::Initial(int total_ni, double xa, double xb, int total_nj, double ya, 
         n = total;   // one block holds every node
         node.assign(n + 2 * bc, 0.0);
         for (int i = bc; i < n + bc; i++) node[i] = a + (i - bc) * d;
-        // ghost nodes mirrored through the first / last interior node
+        // ghost nodes mirrored through the first / last interior node. An axis with fewer than
+        // bc + 1 nodes (the 1-D decks' total_nj = 2) has no node to mirror: the reference then
+        // reads a ghost entry not yet written (src/Mesh.cpp:114-118), which makes the CFL
+        // spacing y[1] - y[0] negative; those ghosts extend the spacing d linearly instead.
         for (int i = 0; i < bc; i++) {
-            node[i] = 2 * node[bc] - node[2 * bc - i];
-            node[n + bc + i] = 2 * node[n + bc - 1] - node[n + bc - 2 - i];
+            node[i] = (2 * bc - i < n + bc) ? 2 * node[bc] - node[2 * bc - i] : node[bc] - (bc - i) * d;
+            node[n + bc + i] = (n + bc - 2 - i >= bc) ? 2 * node[n + bc - 1] - node[n + bc - 2 - i]
+                                                      : node[n + bc - 1] + (i + 1) * d;
         }
     };
     axis(total_ni, xa, xb, ni, dx, xnode);
@@ -100,22 +104,8 @@ void Mesh::MeshProcess(const std::string& gridfile) {
 }
 
 void Mesh::Uniform(int ni_, int nj_, int nk_, double d, int bc_) {
-    bc = bc_;
-    auto axis = [&](int n, std::vector<double>& node) {
-        node.assign(n + 2 * bc, 0.0);
-        for (int i = bc; i < n + bc; i++) node[i] = (i - bc) * d;
-        for (int i = 0; i < bc; i++) {
-            // a single-node axis has nothing to mirror through: extend by d
-            node[i] = (2 * bc - i < n + bc) ? 2 * node[bc] - node[2 * bc - i] : node[bc] - (bc - i) * d;
-            node[n + bc + i] = (n + bc - 2 - i >= bc) ? 2 * node[n + bc - 1] - node[n + bc - 2 - i]
-                                                      : node[n + bc - 1] + (i + 1) * d;
-        }
-    };
-    ni = ni_, nj = nj_, nk = nk_;
+    Initial(ni_, 0.0, (ni_ - 1) * d, nj_, 0.0, (nj_ - 1) * d, nk_, 0.0, (nk_ - 1) * d, bc_);
     dx = dy = dz = d;
-    axis(ni, xnode);
-    axis(nj, ynode);
-    axis(nk, znode);
 }
 
 // ------------------------------------------------------------------ Mechanism

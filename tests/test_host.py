@@ -86,9 +86,16 @@ def test_mesh_follows_reference_rule(tmp_path):
         dx = (b - a) / (n - 1)
         x = np.zeros(n + 4)
         x[2:n + 2] = a + np.arange(n) * dx
-        x[0], x[1] = 2 * x[2] - x[4], 2 * x[2] - x[3]
-        x[n + 2], x[n + 3] = 2 * x[n + 1] - x[n], 2 * x[n + 1] - x[n - 1]
+        if n >= 3:
+            x[0], x[1] = 2 * x[2] - x[4], 2 * x[2] - x[3]
+            x[n + 2], x[n + 3] = 2 * x[n + 1] - x[n], 2 * x[n + 1] - x[n - 1]
+        else:   # two nodes: nothing to mirror through, the spacing is extended
+            x[0], x[1] = x[2] - 2 * dx, 2 * x[2] - x[3]
+            x[n + 2], x[n + 3] = 2 * x[n + 1] - x[n], x[n + 1] + 2 * dx
         assert np.array_equal(np.array(d[key]), x), key
+    # the 1-D decks' two-node axes: ghosts extend the spacing (nothing to mirror through)
+    d1 = _print_decks(_decks(tmp_path, GRID_1D))["mesh"]
+    assert np.allclose(np.array(d1["ynode"]), np.array([-2, -1, 0, 1, 2, 3]) * 5e-5, rtol=0, atol=1e-20)
 
 
 def test_mechanism_reader_matches(tmp_path):
