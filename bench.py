@@ -47,6 +47,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--grid", type=str, default="384x128x128")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the batched-chemistry and surrogate-network sections")
     ap.add_argument("--cpu-grid", type=str, default="384x128x128")
     ap.add_argument("--cpu-steps", type=int, default=3)
     return ap.parse_args()
@@ -84,6 +86,71 @@ def cpu_baseline(grid, steps):
     return {"value": wl.ncells * steps / el, "unit": "cell-updates/s", "cores": cores, "kind": "port",
             "sample": f"bubble3d {ni}x{nj}x{nk} (same generator; bubble radius 25 cells scaled with ni/384), "
                       f"{steps} IMEX steps after 1 warm-up step, {el:.1f} s"}
+
+
+def bench_batched_chemistry(ctx, n=1 << 22, dtm=1e-8, seed=2024):
+    """Batched FP64 chemistry: 2^22 seeded cells, one adaptive trapezoid step of dt = 1e-8 s,
+    with the persistent kernel's schedule from the autotuner. Reported for both burnt
+    compositions (complete combustion, and the relaxed equilibrium line with its radical pool)."""
+    import torch
+    from eu3d_amd import autotune, chemistry
+    out = {}
+    for burnt in ("complete", "equilibrium"):
+        (r0, t0, e0), _ = chemistry.states(ctx, n, seed=seed, burnt=burnt)
+        sched = autotune.load_or_tune(ctx, (r0, t0, e0), dtm)
+        s = torch.cuda.ExternalStream(ctx.stream())
+        times = []
+        for _ in range(3):
+            r, t, e = r0.clone(), t0.clone(), e0.clone()
+            torch.cuda.synchronize()
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(s)
+            st = ctx.chem_batch(n, r.data_ptr(), t.data_ptr(), e.data_ptr(), dtm, 10000)
+            b.record(s)
+            b.synchronize()
+            times.append(a.elapsed_time(b))
+        ms = min(times)
+        out[burnt] = {"cells": n, "dt": dtm, "ms": ms, "cells_per_s": n / (ms * 1e-3),
+                      "substeps_per_s": st["substeps_total"] / (ms * 1e-3), "active_cells": st["active_cells"],
+                      "substeps_total": st["substeps_total"], "nchem_max": st["nchem_max_seen"],
+                      "schedule": {k: sched[k] for k in ("block", "blocks_per_cu", "waves")},
+                      "schedule_from_cache": sched.get("cached", False)}
+        del r0, t0, e0
+    return out
+
+
+def bench_surrogate_network(n=1 << 20, reps=10):
+    """Surrogate network throughput at batch 2^20: inference (5 BF16 MFMA GEMMs with fused
+    bias/GELU epilogues) and one training step (forward, backward, Adam), HIP events on the
+    network's stream. FLOPs per cell / sample: 403,968 (inference), 1,205,760 (training)."""
+    import torch
+    from eu3d_amd.api import MLP
+    m = MLP(seed=3)
+    x = (torch.randn(n, MLP.NIN, device="cuda") * (torch.arange(MLP.NIN, device="cuda") < 12)).to(torch.bfloat16)
+    x = x.view(torch.int16)
+    y = torch.randn(n, MLP.NOUT, device="cuda")
+    out = torch.empty(n, MLP.NOUT, device="cuda")
+    s = torch.cuda.ExternalStream(m.stream())
+    m.forward(x, out)
+    m.train_step(x, y, 1e-4)
+
+    def timed(fn):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(s)
+        for _ in range(reps):
+            fn()
+        b.record(s)
+        b.synchronize()
+        return a.elapsed_time(b) / reps
+    fwd = timed(lambda: m.forward(x, out))
+    trn = timed(lambda: m.train_step(x, y, 1e-4))
+    fi, ft = 403968.0 * n, 1205760.0 * n
+    return {"batch": n, "inference_ms": fwd, "inference_cells_per_s": n / (fwd * 1e-3),
+            "inference_TFLOPs": fi / (fwd * 1e-3) / 1e12, "inference_frac_bf16_peak": fi / (fwd * 1e-3) / 2.5e15,
+            "train_step_ms": trn, "train_samples_per_s": n / (trn * 1e-3),
+            "train_TFLOPs": ft / (trn * 1e-3) / 1e12, "train_frac_bf16_peak": ft / (trn * 1e-3) / 2.5e15,
+            "note": "inference includes the copy of the features in and the outputs out; training includes "
+                    "the loss read-back"}
 
 
 def read_profile():
@@ -158,6 +225,9 @@ def main():
         "kernels": kern,
         "finite": rc == 0, "wall_s_timed": wall,
     }
+    if not a.no_extra:
+        res["batched_chemistry"] = bench_batched_chemistry(g)
+        res["surrogate_network"] = bench_surrogate_network()
     if not a.no_cpu:
         res["cpu_baseline"] = cpu_baseline(a.cpu_grid, a.cpu_steps)
     print(json.dumps(res))

@@ -81,7 +81,7 @@ class ChemStats(C.Structure):
 
 
 class ChemSchedule(C.Structure):
-    _fields_ = [("block", C.c_int), ("blocks_per_cu", C.c_int), ("chunk", C.c_int)]
+    _fields_ = [("block", C.c_int), ("blocks_per_cu", C.c_int), ("waves", C.c_int)]
 
 
 class SurrogateParams(C.Structure):
@@ -316,14 +316,16 @@ class Context:
         return {"cells": s.cells, "active_cells": s.active_cells, "substeps_total": s.substeps_total,
                 "nchem_max_seen": s.nchem_max_seen}
 
-    def set_schedule(self, block: int, blocks_per_cu: int, chunk: int = 1):
-        s = ChemSchedule(block, blocks_per_cu, chunk)
+    def set_schedule(self, block: int, blocks_per_cu: int, waves: int = 2):
+        """Persistent chemistry kernel schedule: threads per block, blocks per CU and the
+        occupancy target (waves per SIMD: 1, 2 or 4) of its register allocation."""
+        s = ChemSchedule(block, blocks_per_cu, waves)
         self._chk(self.lib.eu3d_chem_set_schedule(self.h, C.byref(s)), "set_schedule")
 
     def get_schedule(self):
         s = ChemSchedule()
         self._chk(self.lib.eu3d_chem_get_schedule(self.h, C.byref(s)), "get_schedule")
-        return {"block": s.block, "blocks_per_cu": s.blocks_per_cu, "chunk": s.chunk}
+        return {"block": s.block, "blocks_per_cu": s.blocks_per_cu, "waves": s.waves}
 
     def set_surrogate(self, mlp):
         self._chk(self.lib.eu3d_set_surrogate(self.h, mlp.h if mlp is not None else None), "set_surrogate")
@@ -428,6 +430,9 @@ class MLP:
         self._chk(self.lib.eu3d_mlp_train_step(self.h, n, _vp(feats_bf16), _vp(targets_f32), lr, wd, C.byref(loss)),
                   "mlp_train_step")
         return loss.value
+
+    def stream(self) -> int:
+        return self.lib.eu3d_mlp_stream(self.h)
 
     def set_surrogate(self, p: "SurrogateParams"):
         self._chk(self.lib.eu3d_mlp_set_surrogate(self.h, C.byref(p)), "set_surrogate")

@@ -346,7 +346,7 @@ int eu3d_chem_set_schedule(eu3d_ctx* c, const eu3d_chem_schedule* s) {
     if (s->block != 64 && s->block != 128 && s->block != 256) return fail(c, EU3D_ERR_ARG, "block must be 64/128/256");
     if (s->blocks_per_cu < 1 || s->blocks_per_cu > 64) return fail(c, EU3D_ERR_ARG, "blocks_per_cu in [1,64]");
     c->sched = *s;
-    if (c->sched.chunk < 1) c->sched.chunk = 1;
+    if (c->sched.waves != 1 && c->sched.waves != 2 && c->sched.waves != 4) c->sched.waves = 2;
     return EU3D_OK;
 }
 

@@ -52,7 +52,7 @@ struct eu3d_ctx {
     // batched chemistry scratch
     unsigned int* batch_active = nullptr;
     long long batch_cap = 0;
-    eu3d_chem_schedule sched{256, 4, 1};
+    eu3d_chem_schedule sched{256, 4, 2};
     int flux_variant = 0;                                          // tile shape of the x/y advection pass
     struct eu3d_mlp* mlp = nullptr;                                // surrogate used by EU3D_CHEM_SURROGATE
     unsigned* sur_list = nullptr;                                  // fallback list, work list, counters, keep

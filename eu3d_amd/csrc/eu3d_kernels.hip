@@ -942,9 +942,11 @@ __global__ void __launch_bounds__(256) k_trap_first(G3 g, Fields F, uint32_t q, 
 // wave-aggregated atomic queue and runs that cell's remaining sub-steps, taking a new cell as
 // soon as one finishes. All lanes execute the same sub-step code on different cells, so cells of
 // very different sub-step counts do not serialise a wavefront.
-template <class Acc, bool C3>
-__global__ void __launch_bounds__(256) k_chem_persist(Acc acc, const unsigned* __restrict__ list, Scalars* sc,
-                                                      uint32_t q, const double* __restrict__ dtm_p) {
+// W is the occupancy target (waves per SIMD) the register allocation is built for: the schedule
+// chooses among 1 (up to 512 registers, no spills), 2 and 4.
+template <class Acc, bool C3, int W>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W)))
+k_chem_persist(Acc acc, const unsigned* __restrict__ list, Scalars* sc, uint32_t q, const double* __restrict__ dtm_p) {
     const unsigned long long n_active = sc->chem_active;
     const double dtm = *dtm_p;
     const int lane = __lane_id();
@@ -1112,6 +1114,27 @@ __global__ void k_chem_reset(Scalars* sc, double* dtm_slot, double dtm, int use_
     sc->chem_substeps = 0;
     sc->chem_nmax = 0;
     *dtm_slot = use_dt ? sc->dt : dtm;
+}
+
+// Persistent-kernel launch with the context's schedule (block, blocks per CU, occupancy target).
+template <class Acc>
+void launch_persist(eu3d_ctx* c, const Acc& acc, const unsigned* list, const double* dtm_slot) {
+    const int nb = c->sched.blocks_per_cu * c->num_cu, blk = c->sched.block;
+    const bool c3 = c->quirks & Q_C3;
+    hipStream_t s = c->stream;
+    switch (c->sched.waves) {
+    case 1:
+        if (c3) k_chem_persist<Acc, true, 1><<<nb, blk, 0, s>>>(acc, list, c->sc, c->quirks, dtm_slot);
+        else k_chem_persist<Acc, false, 1><<<nb, blk, 0, s>>>(acc, list, c->sc, c->quirks, dtm_slot);
+        break;
+    case 4:
+        if (c3) k_chem_persist<Acc, true, 4><<<nb, blk, 0, s>>>(acc, list, c->sc, c->quirks, dtm_slot);
+        else k_chem_persist<Acc, false, 4><<<nb, blk, 0, s>>>(acc, list, c->sc, c->quirks, dtm_slot);
+        break;
+    default:
+        if (c3) k_chem_persist<Acc, true, 2><<<nb, blk, 0, s>>>(acc, list, c->sc, c->quirks, dtm_slot);
+        else k_chem_persist<Acc, false, 2><<<nb, blk, 0, s>>>(acc, list, c->sc, c->quirks, dtm_slot);
+    }
 }
 
 inline G3 G(const eu3d_ctx* c) { return c->g; }
@@ -1291,12 +1314,7 @@ hipError_t launch_trap_adaptive(eu3d_ctx* c, int nchem_max, bool predict_only) {
                                                                             c->active, nmax, predict_only ? 1 : 0);
     if (!predict_only) {
         GridCells acc{c->g, c->fl};
-        const int blk = c->sched.block;
-        const int nb = c->sched.blocks_per_cu * c->num_cu;
-        if (c3)
-            k_chem_persist<GridCells, true><<<nb, blk, 0, c->stream>>>(acc, c->active, c->sc, c->quirks, slot);
-        else
-            k_chem_persist<GridCells, false><<<nb, blk, 0, c->stream>>>(acc, c->active, c->sc, c->quirks, slot);
+        launch_persist(c, acc, c->active, slot);
     }
     return hipGetLastError();
 }
@@ -1317,15 +1335,13 @@ hipError_t launch_chem_batch(eu3d_ctx* c, long long n, double* rho, double* T, d
     double* slot = &c->sc->dtm;
     k_chem_reset<<<1, 1, 0, c->stream>>>(c->sc, slot, dtm, 0);
     BatchCells acc{n, rho, T, E, keep};
-    const int nb = c->sched.blocks_per_cu * c->num_cu;
     const unsigned gb = (unsigned)((n + 255) / 256);
     if (c->quirks & Q_C3) {
         k_batch_first<true><<<gb, 256, 0, c->stream>>>(acc, nchem_out, keep, c->quirks, dtm, nmax, c->sc, list);
-        k_chem_persist<BatchCells, true><<<nb, c->sched.block, 0, c->stream>>>(acc, list, c->sc, c->quirks, slot);
     } else {
         k_batch_first<false><<<gb, 256, 0, c->stream>>>(acc, nchem_out, keep, c->quirks, dtm, nmax, c->sc, list);
-        k_chem_persist<BatchCells, false><<<nb, c->sched.block, 0, c->stream>>>(acc, list, c->sc, c->quirks, slot);
     }
+    launch_persist(c, acc, list, slot);
     return hipGetLastError();
 }
 
@@ -1344,11 +1360,7 @@ hipError_t launch_trap_list(eu3d_ctx* c, const unsigned* list, const unsigned* c
             k_trap_first_list<false><<<nb, 256, 0, c->stream>>>(G(c), c->fl, c->quirks, list, count, c->sc, work, nmax);
     }
     GridCells acc{c->g, c->fl};
-    const int nbp = c->sched.blocks_per_cu * c->num_cu;
-    if (c3)
-        k_chem_persist<GridCells, true><<<nbp, c->sched.block, 0, c->stream>>>(acc, work, c->sc, c->quirks, &c->sc->dtm);
-    else
-        k_chem_persist<GridCells, false><<<nbp, c->sched.block, 0, c->stream>>>(acc, work, c->sc, c->quirks, &c->sc->dtm);
+    launch_persist(c, acc, work, &c->sc->dtm);
     return hipGetLastError();
 }
 
@@ -1367,11 +1379,7 @@ hipError_t launch_batch_list(eu3d_ctx* c, long long n, double* rho, double* T, d
         else
             k_batch_first_list<false><<<nb, 256, 0, c->stream>>>(acc, nchem_keep, c->quirks, dtm, nmax, list, count, c->sc, work);
     }
-    const int nbp = c->sched.blocks_per_cu * c->num_cu;
-    if (c3)
-        k_chem_persist<BatchCells, true><<<nbp, c->sched.block, 0, c->stream>>>(acc, work, c->sc, c->quirks, &c->sc->dtm);
-    else
-        k_chem_persist<BatchCells, false><<<nbp, c->sched.block, 0, c->stream>>>(acc, work, c->sc, c->quirks, &c->sc->dtm);
+    launch_persist(c, acc, work, &c->sc->dtm);
     return hipGetLastError();
 }
 

@@ -188,7 +188,8 @@ int eu3d_get_chem_stats(eu3d_ctx *ctx, eu3d_chem_stats *out);
 typedef struct {
     int block;          /* threads per block */
     int blocks_per_cu;  /* persistent grid = blocks_per_cu * CU count */
-    int chunk;          /* cells taken per queue pop (per wave) */
+    int waves;          /* occupancy target of the persistent kernel's register allocation:
+                           1, 2 or 4 waves per SIMD (other values select 2) */
 } eu3d_chem_schedule;
 int eu3d_chem_batch(eu3d_ctx *ctx, long long n, double *rho_s, double *T, double *E, double dtm,
                     int nchem_max, int *nchem_out, eu3d_chem_stats *stats);

@@ -239,3 +239,65 @@ def test_surrogate_trained_small():
     say("  stats", st, "max |dT err|", float(dT.max()), "median", float(dT.median()))
     assert st["fallback"] < 0.5 * n
     assert float(dT.max()) <= 2 * p.dT_max + 1e-6
+
+
+def test_schedules_agree():
+    """Every persistent-kernel schedule (block, blocks per CU, occupancy variant) gives the same
+    batched chemistry: identical sub-step counts, states equal to rounding."""
+    torch = _t()
+    from eu3d_amd import configs
+    from eu3d_amd.api import Context
+    say("case schedules")
+    n = 20000
+    rho, T, E, _ = configs.chem_states(n, seed=5)
+    x = configs.make_nodes(8, 1e-4)
+    g = Context(x, x, x, quirks=0)
+    ref = None
+    for sched in ((256, 4, 2), (64, 1, 1), (128, 8, 4), (256, 16, 1)):
+        g.set_schedule(*sched)
+        r, t, e = (torch.tensor(a, device="cuda") for a in (rho.T.copy(), T, E))
+        nch = torch.empty(n, dtype=torch.int32, device="cuda")
+        st = g.chem_batch(n, r.data_ptr(), t.data_ptr(), e.data_ptr(), 7e-6, 10000, nch.data_ptr())
+        torch.cuda.synchronize()
+        assert g.get_schedule() == dict(zip(("block", "blocks_per_cu", "waves"), sched))
+        res = (r.cpu().numpy(), t.cpu().numpy(), nch.cpu().numpy())
+        say("  ", sched, st)
+        if ref is None:
+            ref = res
+            assert st["active_cells"] > 0
+            continue
+        assert (res[2] == ref[2]).all()
+        # a few cells of this seed leave the finite range at dt = 7e-6 in the reference
+        # arithmetic too: the non-finite pattern must coincide, the rest agree to rounding
+        for a, b in ((res[0], ref[0]), (res[1], ref[1])):
+            assert (np.isfinite(a) == np.isfinite(b)).all()
+        fin = np.isfinite(ref[1])
+        assert np.abs(res[0][:, fin] - ref[0][:, fin]).max() <= 1e-12 * np.abs(ref[0][:, fin]).max()
+        assert np.abs(res[1][fin] - ref[1][fin]).max() <= 1e-12 * 3000
+
+
+def test_autotune_caches(tmp_path):
+    """The tuner times candidate schedules on the batch, leaves the batch unchanged, keeps the
+    fastest and answers the next call for the same load from its cache."""
+    torch = _t()
+    from eu3d_amd import autotune, configs
+    from eu3d_amd.api import Context
+    say("case autotune")
+    n = 1 << 14
+    rho, T, E, _ = configs.chem_states(n, seed=6)
+    x = configs.make_nodes(8, 1e-4)
+    g = Context(x, x, x, quirks=0)
+    inputs = tuple(torch.tensor(a, device="cuda") for a in (rho.T.copy(), T, E))
+    keep = [a.clone() for a in inputs]
+    cache = str(tmp_path / "sched.json")
+    cands = [(64, 2, 1), (256, 4, 2), (128, 8, 4)]
+    orig = autotune.tune
+    first = autotune.load_or_tune(g, inputs, 7e-6, cache=cache)
+    assert not first["cached"] and first["candidates"] == len(autotune.BLOCKS) * len(autotune.BLOCKS_PER_CU) * len(autotune.WAVES)
+    assert all(torch.equal(a, b) for a, b in zip(inputs, keep))
+    second = autotune.load_or_tune(g, inputs, 7e-6, cache=cache)
+    assert second["cached"] and {k: second[k] for k in ("block", "blocks_per_cu", "waves")} == \
+        {k: first[k] for k in ("block", "blocks_per_cu", "waves")}
+    best, table = orig(g, inputs, 7e-6, candidates=cands)
+    assert len(table) == 3 and best["ms"] == min(table.values())
+    say("  tuned", first, "table", table)
