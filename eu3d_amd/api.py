@@ -41,7 +41,7 @@ EXPORTED = [
     "eu3d_chem_batch", "eu3d_chem_set_schedule", "eu3d_chem_get_schedule", "eu3d_device_field",
     "eu3d_cells_with_ghosts", "eu3d_stream", "eu3d_time_kernel",
     "eu3d_mlp_create", "eu3d_mlp_destroy", "eu3d_mlp_param_count", "eu3d_mlp_get_params", "eu3d_mlp_set_params",
-    "eu3d_mlp_get_grads", "eu3d_mlp_forward", "eu3d_mlp_train_step", "eu3d_gemm_bf16", "eu3d_mlp_stream",
+    "eu3d_mlp_get_grads", "eu3d_mlp_forward", "eu3d_mlp_forward_layers", "eu3d_mlp_train_step", "eu3d_gemm_bf16", "eu3d_mlp_stream",
     "eu3d_mlp_set_surrogate", "eu3d_surrogate_features", "eu3d_surrogate_targets", "eu3d_surrogate_batch",
     "eu3d_set_surrogate", "eu3d_get_surrogate_stats", "eu3d_surrogate_chem",
 ]
@@ -149,6 +149,7 @@ def load_library(path: str = LIB_PATH):
     for n in ("eu3d_mlp_get_params", "eu3d_mlp_set_params", "eu3d_mlp_get_grads"):
         getattr(lib, n).argtypes = [vp, fp]
     lib.eu3d_mlp_forward.argtypes = [vp, C.c_int, vp, vp]
+    lib.eu3d_mlp_forward_layers.argtypes = [vp, C.c_int, vp, vp]
     lib.eu3d_mlp_train_step.argtypes = [vp, C.c_int, vp, vp, C.c_float, C.c_float, fp]
     lib.eu3d_gemm_bf16.argtypes = [C.c_int, C.c_int, C.c_int, vp, vp, C.c_int, vp]
     lib.eu3d_mlp_stream.argtypes = [vp]
@@ -422,6 +423,12 @@ class MLP:
         _order_after_producer()
         n = feats_bf16.shape[0]
         self._chk(self.lib.eu3d_mlp_forward(self.h, n, _vp(feats_bf16), _vp(out_f32)), "mlp_forward")
+
+    def forward_layers(self, feats_bf16, out_f32):
+        """The same inference layer by layer (the training path's GEMMs)."""
+        _order_after_producer()
+        n = feats_bf16.shape[0]
+        self._chk(self.lib.eu3d_mlp_forward_layers(self.h, n, _vp(feats_bf16), _vp(out_f32)), "mlp_forward_layers")
 
     def train_step(self, feats_bf16, targets_f32, lr: float, wd: float = 0.0) -> float:
         _order_after_producer()

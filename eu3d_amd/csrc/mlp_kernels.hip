@@ -23,13 +23,8 @@ typedef uint16_t bf16_t;
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
 
-__host__ __device__ __forceinline__ bf16_t f2bf(float f) {
-    uint32_t u;
-    memcpy(&u, &f, 4);
-    if ((u & 0x7F800000u) == 0x7F800000u) return (bf16_t)((u >> 16) | ((u & 0xFFFF) ? 0x40 : 0));
-    u += 0x7FFFu + ((u >> 16) & 1u);   // round to nearest even
-    return (bf16_t)(u >> 16);
-}
+// FP32 -> BF16, round to nearest even (NaN stays NaN): the gfx950 conversion instruction.
+__device__ __forceinline__ bf16_t f2bf(float f) { return __builtin_bit_cast(bf16_t, static_cast<__bf16>(f)); }
 __host__ __device__ __forceinline__ float bf2f(bf16_t h) {
     uint32_t u = (uint32_t)h << 16;
     float f;
@@ -37,9 +32,21 @@ __host__ __device__ __forceinline__ float bf2f(bf16_t h) {
     return f;
 }
 
-__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+// GELU in its tanh form, 0.5 z (1 + tanh(sqrt(2/pi) (z + 0.044715 z^3))), evaluated as the equal
+// z * sigmoid(s) with s = 2 sqrt(2/pi) (z + 0.044715 z^3): one exp2 and one reciprocal per
+// element, so the epilogue of the fused network kernel stays short next to its MFMA work.
+// Training, inference and the CPU checker all use this form.
+constexpr float GELU_C1 = 1.5957691216057308f;                 // 2 sqrt(2/pi)
+constexpr float GELU_C2 = GELU_C1 * 0.044715f;
+constexpr float LOG2E = 1.4426950408889634f;
+__device__ __forceinline__ float gelu_sig(float z) {          // sigmoid(s)
+    const float w = z * fmaf(-GELU_C2 * LOG2E, z * z, -GELU_C1 * LOG2E);   // -s log2(e)
+    return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(w));
+}
+__device__ __forceinline__ float gelu(float z) { return z * gelu_sig(z); }
 __device__ __forceinline__ float dgelu(float z) {
-    return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * expf(-0.5f * z * z);
+    const float g = gelu_sig(z);
+    return g + z * g * (1.0f - g) * fmaf(3.0f * GELU_C2, z * z, GELU_C1);
 }
 
 // ------------------------------------------------------------------ GEMM
@@ -244,6 +251,111 @@ __global__ void k_f32_to_bf16(const float* __restrict__ x, bf16_t* __restrict__ 
         y[t] = f2bf(x[t]);
 }
 
+// ------------------------------------------------------------------ fused inference
+// The whole network for a tile of FM rows in one workgroup: the activations stay in LDS from
+// the input features to the last layer, so HBM sees only the features in (64 B per row) and the
+// outputs out (64 B per row). Each layer is [FM x K] x [K x N] on MFMA 16x16x32 BF16. Eight
+// waves (two per SIMD, so one wave's epilogue overlaps the other's MFMAs) each own 16 rows and
+// every output column (N/16 accumulator tiles). K goes in chunks of 32 whose weights (W^T rows,
+// K-contiguous) pass through registers into a double-buffered LDS stage shared by the eight
+// waves; each chunk is loaded two chunks ahead of its use. The epilogue (bias, BF16 rounding of
+// the pre-activation, GELU, BF16) writes each wave's own rows back in place, which no other
+// wave reads.
+constexpr int FM = 128, FT = 512, FPITCH = 256 + 8, WPITCH = 32 + 8;
+
+struct MlpView {
+    const bf16_t* wt[5];
+    const float* b[5];
+};
+
+// chunk c of W^T (N rows x 32 k): thread tid holds 16-byte pieces tid and tid + FT
+struct WPiece {
+    uint4 a, b;
+};
+template <int N, int K>
+__device__ __forceinline__ WPiece load_w(const bf16_t* __restrict__ wt, int c, int tid) {
+    WPiece r{make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0)};
+    const int p0 = tid, p1 = tid + FT;
+    if (p0 < N * 4) r.a = *reinterpret_cast<const uint4*>(wt + (size_t)(p0 >> 2) * K + c * 32 + (p0 & 3) * 8);
+    if (p1 < N * 4) r.b = *reinterpret_cast<const uint4*>(wt + (size_t)(p1 >> 2) * K + c * 32 + (p1 & 3) * 8);
+    return r;
+}
+template <int N>
+__device__ __forceinline__ void store_w(bf16_t (*ws)[WPITCH], const WPiece& r, int tid) {
+    const int p0 = tid, p1 = tid + FT;
+    if (p0 < N * 4) *reinterpret_cast<uint4*>(&ws[p0 >> 2][(p0 & 3) * 8]) = r.a;
+    if (p1 < N * 4) *reinterpret_cast<uint4*>(&ws[p1 >> 2][(p1 & 3) * 8]) = r.b;
+}
+
+template <int K, int N, bool LAST>
+__device__ __forceinline__ void fused_layer(bf16_t (*act)[FPITCH], bf16_t (*ws)[256][WPITCH], const bf16_t* __restrict__ wt,
+                                            const float* __restrict__ bias, int tid, int lane, int wr, long long row0,
+                                            int n, float* __restrict__ Y) {
+    constexpr int NT = N / 16, KS = K / 32;
+    const int fr = lane & 15, fk = (lane >> 4) * 8;
+    f32x4_t acc[NT];
+#pragma unroll
+    for (int j = 0; j < NT; j++) acc[j] = (f32x4_t){0.f, 0.f, 0.f, 0.f};
+    // chunk c waits in register set c % 2 from its load (two chunks ahead) to its LDS store
+    WPiece r0 = load_w<N, K>(wt, 0, tid), r1{};
+    if (KS > 1) r1 = load_w<N, K>(wt, 1, tid);
+    __syncthreads();   // the previous layer is done with ws and act
+    store_w<N>(ws[0], r0, tid);
+    if (KS > 2) r0 = load_w<N, K>(wt, 2, tid);
+#pragma unroll
+    for (int ks = 0; ks < KS; ks++) {   // unrolled: the register set of each chunk is known
+        __syncthreads();   // chunk ks staged; the other buffer's readers are done
+        if (ks + 1 < KS) store_w<N>(ws[(ks + 1) & 1], ((ks + 1) & 1) ? r1 : r0, tid);
+        if (ks + 3 < KS) {
+            if ((ks + 1) & 1) r1 = load_w<N, K>(wt, ks + 3, tid);
+            else r0 = load_w<N, K>(wt, ks + 3, tid);
+        }
+        const bf16x8_t a = *reinterpret_cast<const bf16x8_t*>(&act[wr + fr][ks * 32 + fk]);
+#pragma unroll
+        for (int j = 0; j < NT; j++) {
+            const bf16x8_t b = *reinterpret_cast<const bf16x8_t*>(&ws[ks & 1][16 * j + fr][fk]);
+            acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[j], 0, 0, 0);
+        }
+    }
+    // D fragment: lane holds rows 4 * (lane / 16) + r, column lane % 16 of each 16 x 16 tile
+    if (!LAST) {
+#pragma unroll
+        for (int j = 0; j < NT; j++) {
+            const int col = 16 * j + fr;
+            const float bv = bias[col];
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const bf16_t zb = f2bf(acc[j][r] + bv);
+                act[wr + 4 * (lane >> 4) + r][col] = f2bf(gelu(bf2f(zb)));
+            }
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const long long row = row0 + wr + 4 * (lane >> 4) + r;
+            if (row < n) Y[row * N + fr] = acc[0][r] + bias[fr];
+        }
+    }
+}
+
+__global__ void __launch_bounds__(FT) k_mlp_fused(MlpView v, int n, const bf16_t* __restrict__ X, float* __restrict__ Y) {
+    __shared__ bf16_t act[FM][FPITCH];
+    __shared__ bf16_t ws[2][256][WPITCH];
+    const int tid = threadIdx.x, lane = tid & 63, wr = (tid >> 6) * 16;
+    const long long row0 = (long long)blockIdx.x * FM;
+    for (int c = tid; c < FM * 4; c += FT) {   // input features: FM rows x 32 BF16
+        const int r = c >> 2, kc = (c & 3) * 8;
+        uint4 val = make_uint4(0, 0, 0, 0);
+        if (row0 + r < n) val = *reinterpret_cast<const uint4*>(X + (size_t)(row0 + r) * 32 + kc);
+        *reinterpret_cast<uint4*>(&act[r][kc]) = val;
+    }
+    fused_layer<32, 256, false>(act, ws, v.wt[0], v.b[0], tid, lane, wr, row0, n, Y);
+    fused_layer<256, 256, false>(act, ws, v.wt[1], v.b[1], tid, lane, wr, row0, n, Y);
+    fused_layer<256, 256, false>(act, ws, v.wt[2], v.b[2], tid, lane, wr, row0, n, Y);
+    fused_layer<256, 256, false>(act, ws, v.wt[3], v.b[3], tid, lane, wr, row0, n, Y);
+    fused_layer<256, 16, true>(act, ws, v.wt[4], v.b[4], tid, lane, wr, row0, n, Y);
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------ MLP object
@@ -260,6 +372,9 @@ struct eu3d_mlp {
     bf16_t* zs[5] = {};           // pre-activations of layers 1..4
     bf16_t *dz_a = nullptr, *dz_b = nullptr;
     float* yout = nullptr;        // [cap x 16]
+    int icap = 0;                 // inference rows (features and outputs only)
+    bf16_t* ifeat = nullptr;      // [icap x 32]
+    float* iout = nullptr;        // [icap x 16]
     float* part = nullptr;        // split-K / reduction partials
     size_t part_cap = 0;
     int adam_t = 0;
@@ -308,6 +423,22 @@ hipError_t ensure_ws(eu3d_mlp* m, int n) {
     return hipSuccess;
 }
 
+// Inference needs only the features and the outputs of each row (the fused kernel keeps the
+// activations on chip): 128 bytes per row.
+hipError_t ensure_infer(eu3d_mlp* m, int n) {
+    if (n <= m->icap) return hipSuccess;
+    if (m->ifeat) (void)hipFree(m->ifeat);
+    if (m->iout) (void)hipFree(m->iout);
+    m->ifeat = nullptr;
+    m->iout = nullptr;
+    m->icap = 0;
+    hipError_t e;
+    if ((e = hipMalloc(&m->ifeat, (size_t)n * NIN * 2)) != hipSuccess) return e;
+    if ((e = hipMalloc(&m->iout, (size_t)n * NOUT * 4)) != hipSuccess) return e;
+    m->icap = n;
+    return hipSuccess;
+}
+
 hipError_t ensure_part(eu3d_mlp* m, size_t nfloats) {
     if (nfloats <= m->part_cap) return hipSuccess;
     if (m->part) (void)hipFree(m->part);
@@ -335,6 +466,17 @@ hipError_t forward(eu3d_mlp* m, int n, bool save_z, hipStream_t st = nullptr) {
     a.B = m->wt[4]; a.ldb = m->din[4];
     a.Cf = m->yout; a.bias = m->b[4]; a.ldc = m->dout[4];
     return gemm<true, true, EPI_BIAS_F32>(s0, a);
+}
+
+// Inference through the fused kernel: features (BF16 [n x 32]) -> outputs (FP32 [n x 16]).
+hipError_t forward_fused(eu3d_mlp* m, int n, const bf16_t* X, float* Y, hipStream_t st) {
+    MlpView v;
+    for (int l = 0; l < 5; l++) {
+        v.wt[l] = m->wt[l];
+        v.b[l] = m->b[l];
+    }
+    k_mlp_fused<<<(unsigned)((n + FM - 1) / FM), FT, 0, st>>>(v, n, X, Y);
+    return hipGetLastError();
 }
 
 }  // namespace
@@ -633,7 +775,7 @@ int eu3d_mlp_destroy(eu3d_mlp* m) {
         if (m->act[l]) (void)hipFree(m->act[l]);
         if (m->zs[l]) (void)hipFree(m->zs[l]);
     }
-    for (void* p : {(void*)m->dz_a, (void*)m->dz_b, (void*)m->yout, (void*)m->part})
+    for (void* p : {(void*)m->dz_a, (void*)m->dz_b, (void*)m->yout, (void*)m->part, (void*)m->ifeat, (void*)m->iout})
         if (p) (void)hipFree(p);
     (void)hipStreamDestroy(m->stream);
     delete m;
@@ -676,6 +818,15 @@ int eu3d_mlp_get_grads(eu3d_mlp* m, float* host) { return m && host ? xfer_param
 // Inference on n rows of BF16 features (device, [n x 32], zero padded) -> FP32 outputs
 // (device, [n x 16]; columns 9..15 are padding).
 int eu3d_mlp_forward(eu3d_mlp* m, int n, const void* feats_bf16, float* out_f32) {
+    if (!m || n <= 0 || !feats_bf16 || !out_f32) return EU3D_ERR_ARG;
+    if (forward_fused(m, n, static_cast<const bf16_t*>(feats_bf16), out_f32, m->stream) != hipSuccess)
+        return EU3D_ERR_HIP;
+    return hipStreamSynchronize(m->stream) == hipSuccess ? EU3D_OK : EU3D_ERR_HIP;
+}
+
+// The same inference layer by layer through the GEMM kernel (the path training uses), with the
+// activations in HBM.
+int eu3d_mlp_forward_layers(eu3d_mlp* m, int n, const void* feats_bf16, float* out_f32) {
     if (!m || n <= 0 || !feats_bf16 || !out_f32) return EU3D_ERR_ARG;
     if (ensure_ws(m, n) != hipSuccess) return EU3D_ERR_HIP;
     if (hipMemcpyAsync(m->act[0], feats_bf16, (size_t)n * NIN * 2, hipMemcpyDeviceToDevice, m->stream) != hipSuccess)
@@ -830,16 +981,16 @@ static int sur_stats(eu3d_ctx* c, eu3d_surrogate_stats* st, long long n) {
 int eu3d_surrogate_batch(eu3d_ctx* c, eu3d_mlp* m, long long n, double* rho, double* T, double* E, double dtm,
                          int nchem_max, eu3d_surrogate_stats* st) {
     if (!c || !m || !m->sp_set || n <= 0 || n >= (1ll << 31) || !rho || !T || !E || !(dtm > 0)) return EU3D_ERR_ARG;
-    if (ensure_ws(m, (int)n) != hipSuccess || sur_ensure(c, n) != EU3D_OK) return EU3D_ERR_HIP;
+    if (ensure_infer(m, (int)n) != hipSuccess || sur_ensure(c, n) != EU3D_OK) return EU3D_ERR_HIP;
     unsigned* list = c->sur_list;
     unsigned* work = c->sur_list + c->sur_cap;
     unsigned* cnt = c->sur_list + 2 * c->sur_cap;
     int* keep = reinterpret_cast<int*>(cnt + 64);
     const unsigned nb = (unsigned)((n + 255) / 256);
     k_zero_u32<<<1, 64, 0, c->stream>>>(cnt, 64);
-    k_sur_feat_batch<<<nb, 256, 0, c->stream>>>(m->sp, n, rho, T, dtm, m->act[0]);
-    if (forward(m, (int)n, false, c->stream) != hipSuccess) return EU3D_ERR_HIP;
-    k_sur_apply_batch<<<nb, 256, 0, c->stream>>>(m->sp, n, rho, T, E, m->yout, dtm, c->quirks, cnt, list);
+    k_sur_feat_batch<<<nb, 256, 0, c->stream>>>(m->sp, n, rho, T, dtm, m->ifeat);
+    if (forward_fused(m, (int)n, m->ifeat, m->iout, c->stream) != hipSuccess) return EU3D_ERR_HIP;
+    k_sur_apply_batch<<<nb, 256, 0, c->stream>>>(m->sp, n, rho, T, E, m->iout, dtm, c->quirks, cnt, list);
     if (eu3d::launch_batch_list(c, n, rho, T, E, dtm, list, cnt, (unsigned)n, work, keep, nchem_max) != hipSuccess)
         return EU3D_ERR_HIP;
     return sur_stats(c, st, n);
@@ -877,16 +1028,16 @@ namespace eu3d {
 hipError_t launch_surrogate_grid(eu3d_ctx* c, int nchem_max) {
     eu3d_mlp* m = c->mlp;
     const long long n = (long long)c->g.ni * c->g.nj * c->g.nk;
-    if (!m || ensure_ws(m, (int)n) != hipSuccess || sur_ensure(c, n) != EU3D_OK) return hipErrorInvalidValue;
+    if (!m || ensure_infer(m, (int)n) != hipSuccess || sur_ensure(c, n) != EU3D_OK) return hipErrorInvalidValue;
     unsigned* list = c->sur_list;
     unsigned* work = c->sur_list + c->sur_cap;
     unsigned* cnt = c->sur_list + 2 * c->sur_cap;
     const dim3 grid((c->g.ni + 63) / 64, (c->g.nj + 3) / 4, c->g.nk), blk(64, 4);
     k_zero_u32<<<1, 64, 0, c->stream>>>(cnt, 64);
-    k_sur_feat_grid<<<grid, blk, 0, c->stream>>>(m->sp, c->g, c->fl, c->sc, m->act[0]);
-    hipError_t e = forward(m, (int)n, false, c->stream);
+    k_sur_feat_grid<<<grid, blk, 0, c->stream>>>(m->sp, c->g, c->fl, c->sc, m->ifeat);
+    hipError_t e = forward_fused(m, (int)n, m->ifeat, m->iout, c->stream);
     if (e != hipSuccess) return e;
-    k_sur_apply_grid<<<grid, blk, 0, c->stream>>>(m->sp, c->g, c->fl, c->sc, m->yout, c->quirks, cnt, list);
+    k_sur_apply_grid<<<grid, blk, 0, c->stream>>>(m->sp, c->g, c->fl, c->sc, m->iout, c->quirks, cnt, list);
     return launch_trap_list(c, list, cnt, (unsigned)n, work, nchem_max);
 }
 

@@ -219,7 +219,11 @@ size_t eu3d_mlp_param_count(const eu3d_mlp *m);
 int eu3d_mlp_get_params(eu3d_mlp *m, float *host);      /* per layer: W^T [dout x din] then bias */
 int eu3d_mlp_set_params(eu3d_mlp *m, const float *host);
 int eu3d_mlp_get_grads(eu3d_mlp *m, float *host);
+/* Inference: BF16 features [n x 32] -> FP32 outputs [n x 16], the whole network in one fused
+ * kernel (activations kept in LDS). eu3d_mlp_forward_layers runs the same network layer by layer
+ * through the GEMM kernel that training uses. */
 int eu3d_mlp_forward(eu3d_mlp *m, int n, const void *feats_bf16, float *out_f32);
+int eu3d_mlp_forward_layers(eu3d_mlp *m, int n, const void *feats_bf16, float *out_f32);
 int eu3d_mlp_train_step(eu3d_mlp *m, int n, const void *feats_bf16, const float *targets, float lr, float wd,
                         float *loss_out);
 int eu3d_gemm_bf16(int M, int N, int K, const void *A, const void *B, int b_kc, float *C);

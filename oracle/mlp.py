@@ -1,13 +1,12 @@
 """FP64 CPU restatement of the chemistry-surrogate MLP (TEST INFRASTRUCTURE ONLY).
 
-The checker for the HIP network: forward with exact-erf GELU and hand-written backward in
+The checker for the HIP network: forward with GELU (tanh form) and hand-written backward in
 float64 numpy, reading the same parameter vector layout as eu3d_mlp_get_params (per layer
 W^T [dout x din] then bias [dout]; padded inputs 32, hidden 256, padded outputs 16). The
 surrogate has no counterpart in the reference, so its pins are this restatement, finite
 differences of it, and the BF16 rounding model below.
 """
 import numpy as np
-from scipy.special import erf
 
 DIN = [32, 256, 256, 256, 256]
 DOUT = [256, 256, 256, 256, 16]
@@ -34,12 +33,18 @@ def unpack(params):
     return Ws, bs
 
 
+C_GELU = np.sqrt(2.0 / np.pi)
+
+
 def gelu(z):
-    return 0.5 * z * (1.0 + erf(z / np.sqrt(2.0)))
+    """GELU, tanh form: 0.5 z (1 + tanh(sqrt(2/pi) (z + 0.044715 z^3))) (the kernels evaluate the
+    equal z * sigmoid(2 sqrt(2/pi) (z + 0.044715 z^3)))."""
+    return 0.5 * z * (1.0 + np.tanh(C_GELU * (z + 0.044715 * z ** 3)))
 
 
 def dgelu(z):
-    return 0.5 * (1.0 + erf(z / np.sqrt(2.0))) + z * np.exp(-0.5 * z * z) / np.sqrt(2 * np.pi)
+    t = np.tanh(C_GELU * (z + 0.044715 * z ** 3))
+    return 0.5 * (1.0 + t) + 0.5 * z * (1.0 - t * t) * C_GELU * (1.0 + 3 * 0.044715 * z * z)
 
 
 def forward(params, X, bf16=False):

@@ -301,3 +301,26 @@ def test_autotune_caches(tmp_path):
     best, table = orig(g, inputs, 7e-6, candidates=cands)
     assert len(table) == 3 and best["ms"] == min(table.values())
     say("  tuned", first, "table", table)
+
+
+@pytest.mark.parametrize("n", [128, 1000, 77777])
+def test_fused_forward_matches_layers(n):
+    """The fused network kernel (activations in LDS) equals the layer-by-layer GEMM path,
+    including a ragged last tile."""
+    torch = _t()
+    from eu3d_amd.api import MLP
+    say("case fused forward", n)
+    m = MLP(seed=9)
+    _random_params(m, 3)
+    rng = np.random.default_rng(n)
+    X = np.zeros((n, 32))
+    X[:, :12] = rng.normal(size=(n, 12))
+    xb = bf16_bits(X)
+    a = torch.empty((n, 16), dtype=torch.float32, device="cuda")
+    b = torch.full((n, 16), float("nan"), dtype=torch.float32, device="cuda")
+    m.forward(xb, a)
+    m.forward_layers(xb, b)
+    torch.cuda.synchronize()
+    d = (a - b).abs().max().item()
+    say("  max |fused - layers|", d, "bitwise", bool(torch.equal(a, b)))
+    assert d <= 1e-3 * b.abs().max().item()

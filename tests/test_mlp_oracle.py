@@ -36,8 +36,14 @@ def test_gradients_match_finite_differences():
 
 
 def test_gelu_closed_forms():
+    """GELU's tanh form: zero at 0, identity far right, zero far left, within 2e-3 of the exact
+    erf form everywhere, and its derivative matches finite differences."""
+    from scipy.special import erf
     assert om.gelu(np.array([0.0]))[0] == 0.0
     assert abs(om.gelu(np.array([10.0]))[0] - 10.0) < 1e-12
+    assert abs(om.gelu(np.array([-10.0]))[0]) < 1e-12
+    z = np.linspace(-6, 6, 241)
+    assert np.abs(om.gelu(z) - 0.5 * z * (1 + erf(z / np.sqrt(2)))).max() < 2e-3
     z = np.linspace(-4, 4, 33)
     fd = (om.gelu(z + 1e-6) - om.gelu(z - 1e-6)) / 2e-6
     assert np.allclose(fd, om.dgelu(z), atol=1e-8)
