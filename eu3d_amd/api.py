@@ -88,12 +88,13 @@ class SurrogateParams(C.Structure):
     _fields_ = [("feat_mean", C.c_float * 12), ("feat_std", C.c_float * 12), ("out_mean", C.c_float * 9),
                 ("out_std", C.c_float * 9), ("hull_lo", C.c_float * 12), ("hull_hi", C.c_float * 12),
                 ("dT_max", C.c_double), ("elem_tol", C.c_double), ("y_neg_tol", C.c_double),
-                ("T_cold", C.c_double)]
+                ("T_cold", C.c_double), ("nchem_threshold", C.c_int)]
 
 
 class SurrogateStats(C.Structure):
     _fields_ = [("cells", C.c_longlong), ("fallback", C.c_longlong), ("hull", C.c_longlong),
-                ("negative", C.c_longlong), ("element", C.c_longlong), ("dT", C.c_longlong)]
+                ("negative", C.c_longlong), ("element", C.c_longlong), ("dT", C.c_longlong),
+                ("network", C.c_longlong)]
 
 
 _lib = None

@@ -9,9 +9,9 @@ States follow the batched-chemistry configuration:
 Two burnt compositions are available:
 - "complete": the burnt composition is complete combustion to H2O (configs.chem_states).
 - "equilibrium": the complete-combustion products at the sample's T and p are relaxed at constant
-  density and internal energy with the solver's own FP64 chemistry over a long interval. The burnt
-  composition then carries the radical pool of the H2-air equilibrium line, and the cells have
-  uneven sub-step counts at dt = 1e-8 s.
+  density and internal energy with the solver's own FP64 chemistry (10 us in 20 steps, at most
+  20000 sub-steps each). The burnt composition then carries the radical pool of the H2-air
+  equilibrium line, and the cells have uneven sub-step counts at dt = 1e-8 s.
 
 Device arrays are torch tensors used as plain memory: rho [ns][n], T [n], E [n].
 """
@@ -36,7 +36,8 @@ def to_device(rho, T, E):
             torch.tensor(E, device="cuda"))
 
 
-def states(ctx, n: int, seed: int = 2024, burnt: str = "complete", relax_time: float = 2e-5, relax_steps: int = 40):
+def states(ctx, n: int, seed: int = 2024, burnt: str = "complete", relax_time: float = 1e-5, relax_steps: int = 20,
+           relax_nchem_max: int = 20000, log=None):
     """Seeded batch states on the device: (rho [ns][n], T [n], E [n]) torch tensors, plus the
     host copies (rho [n][ns], T, E) they were made from."""
     mech = load_default()
@@ -72,9 +73,13 @@ def states(ctx, n: int, seed: int = 2024, burnt: str = "complete", relax_time: f
     Eb = configs._nasa_e(mech, T_s, Yb)
     r, t, e = to_device(rho_b, T_s, Eb)
     dt = relax_time / relax_steps
-    for _ in range(relax_steps):
-        ctx.chem_batch(n, r.data_ptr(), t.data_ptr(), e.data_ptr(), dt, 100000)
+    for k in range(relax_steps):
+        st = ctx.chem_batch(n, r.data_ptr(), t.data_ptr(), e.data_ptr(), dt, relax_nchem_max)
+        if log:
+            log(f"  relax {k + 1}/{relax_steps}: active {st['active_cells']} max sub-steps {st['nchem_max_seen']}")
     Yeq = (r / r.sum(0)).T.cpu().numpy()
+    bad = ~np.isfinite(Yeq).all(1)            # relaxations that left the finite range keep Y_b
+    Yeq[bad] = Yb[bad]
     Yeq = np.maximum(Yeq, 0.0)
     Yeq /= Yeq.sum(1, keepdims=True)
     Y = (1 - c)[:, None] * Yu + c[:, None] * Yeq

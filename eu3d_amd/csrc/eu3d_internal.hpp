@@ -83,7 +83,10 @@ hipError_t launch_explicit(eu3d_ctx* c);
 hipError_t launch_diagonalized(eu3d_ctx* c);
 hipError_t launch_update_imex(eu3d_ctx* c);
 hipError_t launch_trapezoid(eu3d_ctx* c);
-hipError_t launch_trap_adaptive(eu3d_ctx* c, int nchem_max, bool predict_only);
+// nthr > 0: cells above T_cold needing more than nthr sub-steps keep their state and are listed
+// in slist / scount for the surrogate network instead of taking the FP64 sub-steps.
+hipError_t launch_trap_adaptive(eu3d_ctx* c, int nchem_max, bool predict_only, int nthr = 0, double T_cold = 0.0,
+                                unsigned* slist = nullptr, unsigned* scount = nullptr);
 hipError_t launch_check_finite(eu3d_ctx* c);
 hipError_t launch_set_scalars(eu3d_ctx* c, double dt, double time);
 hipError_t launch_step(eu3d_ctx* c, const eu3d_step_params* p, bool first);
@@ -94,6 +97,7 @@ hipError_t launch_trap_list(eu3d_ctx* c, const unsigned* list, const unsigned* c
 hipError_t launch_batch_list(eu3d_ctx* c, long long n, double* rho, double* T, double* E, double dtm,
                              const unsigned* list, const unsigned* count, unsigned max_count, unsigned* work,
                              int* nchem_keep, int nchem_max);
-hipError_t launch_chem_batch(eu3d_ctx* c, long long n, double* rho, double* T, double* E, double dtm,
-                             int nchem_max, int* nchem_out);
+hipError_t launch_chem_batch(eu3d_ctx* c, long long n, double* rho, double* T, double* E, double dtm, int nchem_max,
+                             int* nchem_out, int nthr = 0, double T_cold = 0.0, unsigned* slist = nullptr,
+                             unsigned* scount = nullptr);
 }  // namespace eu3d

@@ -898,16 +898,30 @@ __device__ __forceinline__ void chem_count(Scalars* sc, int nch, bool active, un
     if (active) list[base + __popcll(am & ((1ull << lane) - 1ull))] = id;
 }
 
+// Wave-aggregated append of the flagged lanes' ids to a device list (one atomic per wave).
+__device__ __forceinline__ void wave_append(unsigned* list, unsigned* count, bool flag, unsigned id) {
+    const unsigned long long m = __ballot(flag);
+    if (!m) return;   // the same for every lane of the wave
+    const int lane = __lane_id();
+    const int leader = __ffsll(m) - 1;
+    unsigned base = 0;
+    if (lane == leader) base = atomicAdd(count, (unsigned)__popcll(m));
+    base = __shfl(base, leader);
+    if (flag) list[base + __popcll(m & ((1ull << lane) - 1ull))] = id;
+}
+
 // Prediction + first sub-step for every interior cell; cells needing more sub-steps are
-// compacted into the work list.
+// compacted into the work list. With nthr > 0 (surrogate chemistry), a cell at or above T_cold
+// needing more than nthr sub-steps keeps its state and goes to the network list instead.
 template <bool C3>
 __global__ void __launch_bounds__(256) k_trap_first(G3 g, Fields F, uint32_t q, const Scalars* __restrict__ scc,
-                                                    Scalars* sc, unsigned* list, int nmax, int predict_only) {
+                                                    Scalars* sc, unsigned* list, int nmax, int predict_only, int nthr,
+                                                    double T_cold, unsigned* slist, unsigned* scount) {
     int i, j, k;
     const bool in = interior_ijk(g, i, j, k);
     const size_t c = in ? IDX(g, i, j, k) : 0;
     int nch = 0;
-    bool active = false;
+    bool active = false, to_net = false;
     if (in) {
         ChemCell cell;
 #pragma unroll
@@ -927,7 +941,8 @@ __global__ void __launch_bounds__(256) k_trap_first(G3 g, Fields F, uint32_t q, 
         nch = chem_nchem(ko, Dr, Y, dtm, nmax);
         FLD(F, EU3D_FIELD_NCHEM)[c] = (double)nch;
         if (nch <= 0) atomicOr(&sc->nonfinite, 2);
-        if (!predict_only && nch > 0) {
+        to_net = nthr > 0 && nch > nthr && cell.T >= T_cold;
+        if (!predict_only && nch > 0 && !to_net) {
             double cs[NC];
             const Prim o = chem_apply(cell, ko, Dr, dtm / nch, q, cs);
             GridCells acc{g, F};
@@ -935,7 +950,8 @@ __global__ void __launch_bounds__(256) k_trap_first(G3 g, Fields F, uint32_t q, 
             active = nch > 1;
         }
     }
-    chem_count(sc, in ? nch : 0, active, list, (unsigned)c);
+    chem_count(sc, in && !to_net ? nch : 0, active, list, (unsigned)c);
+    if (nthr > 0) wave_append(slist, scount, to_net, (unsigned)c);
 }
 
 // Persistent chemistry kernel: every lane pulls cells from the compacted list through a
@@ -995,11 +1011,12 @@ k_chem_persist(Acc acc, const unsigned* __restrict__ list, Scalars* sc, uint32_t
 // Batch: prediction + first sub-step on flat cells at rest.
 template <bool C3>
 __global__ void __launch_bounds__(256) k_batch_first(BatchCells acc, int* nchem_out, int* nchem_keep, uint32_t q,
-                                                     double dtm, int nmax, Scalars* sc, unsigned* list) {
+                                                     double dtm, int nmax, Scalars* sc, unsigned* list, int nthr,
+                                                     double T_cold, unsigned* slist, unsigned* scount) {
     const long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     const bool in = id < acc.n;
     int nch = 0;
-    bool active = false;
+    bool active = false, to_net = false;
     if (in) {
         ChemCell cell;
 #pragma unroll
@@ -1019,14 +1036,16 @@ __global__ void __launch_bounds__(256) k_batch_first(BatchCells acc, int* nchem_
         nchem_keep[id] = nch;
         if (nchem_out) nchem_out[id] = nch;
         if (nch <= 0) atomicOr(&sc->nonfinite, 2);
-        if (nch > 0) {
+        to_net = nthr > 0 && nch > nthr && cell.T >= T_cold;
+        if (nch > 0 && !to_net) {
             double cs[NC];
             const Prim o = chem_apply(cell, ko, Dr, dtm / nch, q, cs);
             acc.store((unsigned)id, cs, o, ko);
             active = nch > 1;
         }
     }
-    chem_count(sc, in ? nch : 0, active, list, (unsigned)id);
+    chem_count(sc, in && !to_net ? nch : 0, active, list, (unsigned)id);
+    if (nthr > 0) wave_append(slist, scount, to_net, (unsigned)id);
 }
 
 // Prediction + first sub-step for the cells of a list (the surrogate's fallback cells), grid
@@ -1301,17 +1320,20 @@ hipError_t launch_check_finite(eu3d_ctx* c) {
     return hipGetLastError();
 }
 
-hipError_t launch_trap_adaptive(eu3d_ctx* c, int nchem_max, bool predict_only) {
+hipError_t launch_trap_adaptive(eu3d_ctx* c, int nchem_max, bool predict_only, int nthr, double T_cold,
+                                unsigned* slist, unsigned* scount) {
     const int nmax = nchem_max > 0 ? nchem_max : 10000;
     double* slot = &c->sc->dtm;
     k_chem_reset<<<1, 1, 0, c->stream>>>(c->sc, slot, 0.0, 1);
     const bool c3 = c->quirks & Q_C3;
     if (c3)
         k_trap_first<true><<<interior_grid(c->g), dim3(BX, BY), 0, c->stream>>>(G(c), c->fl, c->quirks, c->sc, c->sc,
-                                                                           c->active, nmax, predict_only ? 1 : 0);
+                                                                           c->active, nmax, predict_only ? 1 : 0, nthr,
+                                                                           T_cold, slist, scount);
     else
         k_trap_first<false><<<interior_grid(c->g), dim3(BX, BY), 0, c->stream>>>(G(c), c->fl, c->quirks, c->sc, c->sc,
-                                                                            c->active, nmax, predict_only ? 1 : 0);
+                                                                            c->active, nmax, predict_only ? 1 : 0, nthr,
+                                                                            T_cold, slist, scount);
     if (!predict_only) {
         GridCells acc{c->g, c->fl};
         launch_persist(c, acc, c->active, slot);
@@ -1320,7 +1342,7 @@ hipError_t launch_trap_adaptive(eu3d_ctx* c, int nchem_max, bool predict_only) {
 }
 
 hipError_t launch_chem_batch(eu3d_ctx* c, long long n, double* rho, double* T, double* E, double dtm, int nchem_max,
-                             int* nchem_out) {
+                             int* nchem_out, int nthr, double T_cold, unsigned* slist, unsigned* scount) {
     const int nmax = nchem_max > 0 ? nchem_max : 10000;
     if (c->batch_cap < n) {
         if (c->batch_active) (void)hipFree(c->batch_active);
@@ -1337,9 +1359,11 @@ hipError_t launch_chem_batch(eu3d_ctx* c, long long n, double* rho, double* T, d
     BatchCells acc{n, rho, T, E, keep};
     const unsigned gb = (unsigned)((n + 255) / 256);
     if (c->quirks & Q_C3) {
-        k_batch_first<true><<<gb, 256, 0, c->stream>>>(acc, nchem_out, keep, c->quirks, dtm, nmax, c->sc, list);
+        k_batch_first<true><<<gb, 256, 0, c->stream>>>(acc, nchem_out, keep, c->quirks, dtm, nmax, c->sc, list, nthr,
+                                                       T_cold, slist, scount);
     } else {
-        k_batch_first<false><<<gb, 256, 0, c->stream>>>(acc, nchem_out, keep, c->quirks, dtm, nmax, c->sc, list);
+        k_batch_first<false><<<gb, 256, 0, c->stream>>>(acc, nchem_out, keep, c->quirks, dtm, nmax, c->sc, list, nthr,
+                                                        T_cold, slist, scount);
     }
     launch_persist(c, acc, list, slot);
     return hipGetLastError();

@@ -338,11 +338,15 @@ __device__ __forceinline__ void fused_layer(bf16_t (*act)[FPITCH], bf16_t (*ws)[
     }
 }
 
-__global__ void __launch_bounds__(FT) k_mlp_fused(MlpView v, int n, const bf16_t* __restrict__ X, float* __restrict__ Y) {
+// count (device, may be null): rows beyond *count are skipped, whole tiles at once.
+__global__ void __launch_bounds__(FT) k_mlp_fused(MlpView v, int n_max, const unsigned* __restrict__ count,
+                                                  const bf16_t* __restrict__ X, float* __restrict__ Y) {
     __shared__ bf16_t act[FM][FPITCH];
     __shared__ bf16_t ws[2][256][WPITCH];
     const int tid = threadIdx.x, lane = tid & 63, wr = (tid >> 6) * 16;
     const long long row0 = (long long)blockIdx.x * FM;
+    const int n = count ? min(n_max, (int)*count) : n_max;
+    if (row0 >= n) return;   // the same for the whole block
     for (int c = tid; c < FM * 4; c += FT) {   // input features: FM rows x 32 BF16
         const int r = c >> 2, kc = (c & 3) * 8;
         uint4 val = make_uint4(0, 0, 0, 0);
@@ -469,13 +473,13 @@ hipError_t forward(eu3d_mlp* m, int n, bool save_z, hipStream_t st = nullptr) {
 }
 
 // Inference through the fused kernel: features (BF16 [n x 32]) -> outputs (FP32 [n x 16]).
-hipError_t forward_fused(eu3d_mlp* m, int n, const bf16_t* X, float* Y, hipStream_t st) {
+hipError_t forward_fused(eu3d_mlp* m, int n, const bf16_t* X, float* Y, hipStream_t st, const unsigned* count = nullptr) {
     MlpView v;
     for (int l = 0; l < 5; l++) {
         v.wt[l] = m->wt[l];
         v.b[l] = m->b[l];
     }
-    k_mlp_fused<<<(unsigned)((n + FM - 1) / FM), FT, 0, st>>>(v, n, X, Y);
+    k_mlp_fused<<<(unsigned)((n + FM - 1) / FM), FT, 0, st>>>(v, n, count, X, Y);
     return hipGetLastError();
 }
 
@@ -488,6 +492,45 @@ constexpr int NEL = 3;   // H, O, N atoms per species (mechanism order H, O, H2O
 __device__ __forceinline__ int atoms(int e, int s) {
     const int H[NS] = {1, 0, 2, 1, 0, 2, 2, 1, 0}, O[NS] = {0, 1, 1, 1, 2, 0, 2, 2, 0}, N[NS] = {0, 0, 0, 0, 0, 0, 0, 0, 2};
     return e == 0 ? H[s] : (e == 1 ? O[s] : N[s]);
+}
+
+// Orthogonal projection of a mass-fraction increment onto the increments that conserve the H, O
+// and N atoms: dY -= A^T (A A^T)^-1 A dY with A[e][s] = atoms(e, s) / Mw_s. The network's raw
+// increment conserves the elements only to its training error; the projected one conserves them
+// to rounding.
+__device__ __forceinline__ void project_elements(double (&dY)[NS]) {
+    double M[NEL][NEL], b[NEL];
+#pragma unroll
+    for (int e = 0; e < NEL; e++) {
+        b[e] = 0.0;
+#pragma unroll
+        for (int s = 0; s < NS; s++) b[e] += atoms(e, s) * RMW(s) * dY[s];
+#pragma unroll
+        for (int f = 0; f < NEL; f++) {
+            double m = 0.0;
+#pragma unroll
+            for (int s = 0; s < NS; s++) m += (atoms(e, s) * RMW(s)) * (atoms(f, s) * RMW(s));
+            M[e][f] = m;
+        }
+    }
+    // 3 x 3 symmetric solve M lam = b by cofactors
+    const double c00 = M[1][1] * M[2][2] - M[1][2] * M[2][1], c01 = M[1][2] * M[2][0] - M[1][0] * M[2][2],
+                 c02 = M[1][0] * M[2][1] - M[1][1] * M[2][0];
+    const double det = M[0][0] * c00 + M[0][1] * c01 + M[0][2] * c02;
+    const double c10 = M[0][2] * M[2][1] - M[0][1] * M[2][2], c11 = M[0][0] * M[2][2] - M[0][2] * M[2][0],
+                 c12 = M[0][1] * M[2][0] - M[0][0] * M[2][1];
+    const double c20 = M[0][1] * M[1][2] - M[0][2] * M[1][1], c21 = M[0][2] * M[1][0] - M[0][0] * M[1][2],
+                 c22 = M[0][0] * M[1][1] - M[0][1] * M[1][0];
+    const double rd = 1.0 / det;
+    const double lam[NEL] = {(c00 * b[0] + c10 * b[1] + c20 * b[2]) * rd, (c01 * b[0] + c11 * b[1] + c21 * b[2]) * rd,
+                             (c02 * b[0] + c12 * b[1] + c22 * b[2]) * rd};
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        double corr = 0.0;
+#pragma unroll
+        for (int e = 0; e < NEL; e++) corr += atoms(e, s) * RMW(s) * lam[e];
+        dY[s] -= corr;
+    }
 }
 
 // Surrogate features of one cell: T, ln(p / 1 atm), ln(max(Y_k, 1e-20)), ln(dt), normalised.
@@ -576,8 +619,10 @@ __device__ __forceinline__ int sur_apply(const eu3d_surrogate_params& sp, const 
     if (!sur_features(sp, Y, c.T, D, dt, f)) return 1;
     double dY[NS], Yn[NS], ymin = 1.0;
 #pragma unroll
+    for (int s = 0; s < NS; s++) dY[s] = (double)out[s] * sp.out_std[s] + sp.out_mean[s];
+    project_elements(dY);
+#pragma unroll
     for (int s = 0; s < NS; s++) {
-        dY[s] = (double)out[s] * sp.out_std[s] + sp.out_mean[s];
         Yn[s] = Y[s] + dY[s];
         ymin = fmin(ymin, Yn[s]);
     }
@@ -672,6 +717,45 @@ __global__ void __launch_bounds__(256) k_sur_feat_grid(eu3d_surrogate_params sp,
     store_feats(feats, r, f);
 }
 
+// Apply the network's output row to grid cell c with the checks; on acceptance write the cell's
+// conserved state and everything Update_after_CS leaves. Returns the rejection code (0 accepted).
+__device__ __forceinline__ int sur_apply_grid_cell(const eu3d_surrogate_params& sp, const G3& g, const Fields& F, double dt,
+                                                   uint32_t q, size_t c, const float* out) {
+    SurCell cl;
+#pragma unroll
+    for (int s = 0; s < NS; s++) cl.Di[s] = FLD(F, EU3D_FIELD_CS)[s * g.n3 + c];
+    cl.U = FLD(F, EU3D_FIELD_U)[c];
+    cl.V = FLD(F, EU3D_FIELD_V)[c];
+    cl.W = FLD(F, EU3D_FIELD_W)[c];
+    cl.E = FLD(F, EU3D_FIELD_E)[c];
+    cl.T = FLD(F, EU3D_FIELD_T)[c];
+    double cs[NC];
+    Prim o;
+    const int why = sur_apply(sp, cl, out, dt, q, cs, o);
+    if (why == 0) {
+        double* CS = FLD(F, EU3D_FIELD_CS);
+#pragma unroll
+        for (int s = 0; s < NC; s++) CS[s * g.n3 + c] = cs[s];
+        FLD(F, EU3D_FIELD_D)[c] = o.D;
+        FLD(F, EU3D_FIELD_T)[c] = o.T;
+        FLD(F, EU3D_FIELD_U)[c] = o.U;
+        FLD(F, EU3D_FIELD_V)[c] = o.V;
+        FLD(F, EU3D_FIELD_W)[c] = o.W;
+        FLD(F, EU3D_FIELD_E)[c] = o.E;
+        FLD(F, EU3D_FIELD_P)[c] = o.P;
+        FLD(F, EU3D_FIELD_H)[c] = o.H;
+        FLD(F, EU3D_FIELD_GAMMA)[c] = o.Gamma;
+        FLD(F, EU3D_FIELD_C)[c] = o.C;
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+            FLD(F, EU3D_FIELD_DI)[s * g.n3 + c] = cs[s];
+            FLD(F, EU3D_FIELD_MC)[s * g.n3 + c] = cs[s] * RMW(s) * 1000;
+            FLD(F, EU3D_FIELD_YI)[s * g.n3 + c] = o.Y[s];
+        }
+    }
+    return why;
+}
+
 __global__ void __launch_bounds__(256) k_sur_apply_grid(eu3d_surrogate_params sp, G3 g, Fields F, const Scalars* sc,
                                                         const float* __restrict__ out, uint32_t q, unsigned* cnt,
                                                         unsigned* list) {
@@ -681,40 +765,83 @@ __global__ void __launch_bounds__(256) k_sur_apply_grid(eu3d_surrogate_params sp
     int why = 0;
     if (in && FLD(F, EU3D_FIELD_T)[c] >= sp.T_cold) {
         const size_t r = ((size_t)(k - g.bc) * g.nj + (j - g.bc)) * g.ni + (i - g.bc);
-        SurCell cl;
-#pragma unroll
-        for (int s = 0; s < NS; s++) cl.Di[s] = FLD(F, EU3D_FIELD_CS)[s * g.n3 + c];
-        cl.U = FLD(F, EU3D_FIELD_U)[c];
-        cl.V = FLD(F, EU3D_FIELD_V)[c];
-        cl.W = FLD(F, EU3D_FIELD_W)[c];
-        cl.E = FLD(F, EU3D_FIELD_E)[c];
-        cl.T = FLD(F, EU3D_FIELD_T)[c];
-        double cs[NC];
-        Prim o;
-        why = sur_apply(sp, cl, out + r * NOUT, sc->dt, q, cs, o);
-        if (why == 0) {
-            double* CS = FLD(F, EU3D_FIELD_CS);
-#pragma unroll
-            for (int s = 0; s < NC; s++) CS[s * g.n3 + c] = cs[s];
-            FLD(F, EU3D_FIELD_D)[c] = o.D;
-            FLD(F, EU3D_FIELD_T)[c] = o.T;
-            FLD(F, EU3D_FIELD_U)[c] = o.U;
-            FLD(F, EU3D_FIELD_V)[c] = o.V;
-            FLD(F, EU3D_FIELD_W)[c] = o.W;
-            FLD(F, EU3D_FIELD_E)[c] = o.E;
-            FLD(F, EU3D_FIELD_P)[c] = o.P;
-            FLD(F, EU3D_FIELD_H)[c] = o.H;
-            FLD(F, EU3D_FIELD_GAMMA)[c] = o.Gamma;
-            FLD(F, EU3D_FIELD_C)[c] = o.C;
-#pragma unroll
-            for (int s = 0; s < NS; s++) {
-                FLD(F, EU3D_FIELD_DI)[s * g.n3 + c] = cs[s];
-                FLD(F, EU3D_FIELD_MC)[s * g.n3 + c] = cs[s] / MW[s] * 1000;
-                FLD(F, EU3D_FIELD_YI)[s * g.n3 + c] = o.Y[s];
-            }
-        }
+        why = sur_apply_grid_cell(sp, g, F, sc->dt, q, c, out + r * NOUT);
     }
     sur_count(cnt, list, (unsigned)c, why);
+}
+
+// ---- network lists (surrogate chemistry on the cells whose sub-step count exceeds the
+// threshold): row r of the features / outputs is cell list[r], r < *count.
+__global__ void __launch_bounds__(256) k_sur_feat_batch_list(eu3d_surrogate_params sp, long long n, const double* __restrict__ rho,
+                                                             const double* __restrict__ T, double dt,
+                                                             const unsigned* __restrict__ list, const unsigned* __restrict__ count,
+                                                             bf16_t* __restrict__ feats) {
+    const unsigned r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= *count) return;
+    const size_t id = list[r];
+    double Y[NS], D = 0;
+#pragma unroll
+    for (int s = 0; s < NS; s++) D += rho[(size_t)s * n + id];
+#pragma unroll
+    for (int s = 0; s < NS; s++) Y[s] = rho[(size_t)s * n + id] / D;
+    float f[NIN_REAL];
+    sur_features(sp, Y, T[id], D, dt, f);
+    store_feats(feats, r, f);
+}
+
+__global__ void __launch_bounds__(256) k_sur_apply_batch_list(eu3d_surrogate_params sp, long long n, double* rho, double* T,
+                                                              double* E, const float* __restrict__ out, double dt, uint32_t q,
+                                                              const unsigned* __restrict__ list, const unsigned* __restrict__ count,
+                                                              unsigned* cnt, unsigned* rej) {
+    const unsigned r = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool in = r < *count;
+    const unsigned id = in ? list[r] : 0;
+    int why = 0;
+    if (in) {
+        SurCell c;
+#pragma unroll
+        for (int s = 0; s < NS; s++) c.Di[s] = rho[(size_t)s * n + id];
+        c.U = c.V = c.W = 0.0;
+        c.E = E[id];
+        c.T = T[id];
+        double cs[NC];
+        Prim o;
+        why = sur_apply(sp, c, out + (size_t)r * NOUT, dt, q, cs, o);
+        if (why == 0) {
+#pragma unroll
+            for (int s = 0; s < NS; s++) rho[(size_t)s * n + id] = cs[s];
+            T[id] = o.T;
+            E[id] = o.E;
+        }
+    }
+    sur_count(cnt, rej, id, why);
+}
+
+__global__ void __launch_bounds__(256) k_sur_feat_grid_list(eu3d_surrogate_params sp, G3 g, Fields F, const Scalars* sc,
+                                                            const unsigned* __restrict__ list, const unsigned* __restrict__ count,
+                                                            bf16_t* __restrict__ feats) {
+    const unsigned r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= *count) return;
+    const size_t c = list[r];
+    double Y[NS], D = 0;
+#pragma unroll
+    for (int s = 0; s < NS; s++) D += FLD(F, EU3D_FIELD_CS)[s * g.n3 + c];
+#pragma unroll
+    for (int s = 0; s < NS; s++) Y[s] = FLD(F, EU3D_FIELD_CS)[s * g.n3 + c] / D;
+    float f[NIN_REAL];
+    sur_features(sp, Y, FLD(F, EU3D_FIELD_T)[c], D, sc->dt, f);
+    store_feats(feats, r, f);
+}
+
+__global__ void __launch_bounds__(256) k_sur_apply_grid_list(eu3d_surrogate_params sp, G3 g, Fields F, const Scalars* sc,
+                                                             const float* __restrict__ out, uint32_t q,
+                                                             const unsigned* __restrict__ list, const unsigned* __restrict__ count,
+                                                             unsigned* cnt, unsigned* rej) {
+    const unsigned r = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool in = r < *count;
+    const size_t c = in ? list[r] : 0;
+    const int why = in ? sur_apply_grid_cell(sp, g, F, sc->dt, q, c, out + (size_t)r * NOUT) : 0;
+    sur_count(cnt, rej, (unsigned)c, why);
 }
 
 __global__ void k_zero_u32(unsigned* p, int n) {
@@ -950,19 +1077,34 @@ int eu3d_surrogate_targets(eu3d_mlp* m, long long n, const double* rho0, const d
     return hipStreamSynchronize(m->stream) == hipSuccess ? EU3D_OK : EU3D_ERR_HIP;
 }
 
+// Surrogate work buffers of a context: the rejected-cell list, the FP64 work list and the network
+// list (n entries each), 64 counters (0 rejected cells, 1..4 rejection reasons, 8 network cells)
+// and the per-cell sub-step counts.
+struct SurBufs {
+    unsigned *rej, *work, *net, *cnt;
+    int* keep;
+};
+static SurBufs sur_bufs(eu3d_ctx* c) {
+    unsigned* b = c->sur_list;
+    const size_t n = (size_t)c->sur_cap;
+    return {b, b + n, b + 2 * n, b + 3 * n, reinterpret_cast<int*>(b + 3 * n + 64)};
+}
+constexpr int CNT_NET = 8;
+
 static int sur_ensure(eu3d_ctx* c, long long n) {
     if (c->sur_cap >= n) return EU3D_OK;
     if (c->sur_list) (void)hipFree(c->sur_list);
     c->sur_list = nullptr;
-    if (hipMalloc(&c->sur_list, (size_t)n * 2 * sizeof(unsigned) + 64 * sizeof(unsigned) + (size_t)n * sizeof(int)) != hipSuccess)
+    c->sur_cap = 0;
+    if (hipMalloc(&c->sur_list, (size_t)n * 3 * sizeof(unsigned) + 64 * sizeof(unsigned) + (size_t)n * sizeof(int)) != hipSuccess)
         return EU3D_ERR_HIP;
     c->sur_cap = n;
     return EU3D_OK;
 }
 
 static int sur_stats(eu3d_ctx* c, eu3d_surrogate_stats* st, long long n) {
-    unsigned h[8];
-    if (hipMemcpyAsync(h, c->sur_list + 2 * c->sur_cap, sizeof(h), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+    unsigned h[16];
+    if (hipMemcpyAsync(h, sur_bufs(c).cnt, sizeof(h), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
         hipStreamSynchronize(c->stream) != hipSuccess)
         return EU3D_ERR_HIP;
     if (st) {
@@ -972,6 +1114,7 @@ static int sur_stats(eu3d_ctx* c, eu3d_surrogate_stats* st, long long n) {
         st->negative = h[2];
         st->element = h[3];
         st->dT = h[4];
+        st->network = h[CNT_NET];
     }
     return EU3D_OK;
 }
@@ -982,16 +1125,24 @@ int eu3d_surrogate_batch(eu3d_ctx* c, eu3d_mlp* m, long long n, double* rho, dou
                          int nchem_max, eu3d_surrogate_stats* st) {
     if (!c || !m || !m->sp_set || n <= 0 || n >= (1ll << 31) || !rho || !T || !E || !(dtm > 0)) return EU3D_ERR_ARG;
     if (ensure_infer(m, (int)n) != hipSuccess || sur_ensure(c, n) != EU3D_OK) return EU3D_ERR_HIP;
-    unsigned* list = c->sur_list;
-    unsigned* work = c->sur_list + c->sur_cap;
-    unsigned* cnt = c->sur_list + 2 * c->sur_cap;
-    int* keep = reinterpret_cast<int*>(cnt + 64);
+    const SurBufs b = sur_bufs(c);
+    const eu3d_surrogate_params& sp = m->sp;
     const unsigned nb = (unsigned)((n + 255) / 256);
-    k_zero_u32<<<1, 64, 0, c->stream>>>(cnt, 64);
-    k_sur_feat_batch<<<nb, 256, 0, c->stream>>>(m->sp, n, rho, T, dtm, m->ifeat);
-    if (forward_fused(m, (int)n, m->ifeat, m->iout, c->stream) != hipSuccess) return EU3D_ERR_HIP;
-    k_sur_apply_batch<<<nb, 256, 0, c->stream>>>(m->sp, n, rho, T, E, m->iout, dtm, c->quirks, cnt, list);
-    if (eu3d::launch_batch_list(c, n, rho, T, E, dtm, list, cnt, (unsigned)n, work, keep, nchem_max) != hipSuccess)
+    k_zero_u32<<<1, 64, 0, c->stream>>>(b.cnt, 64);
+    if (sp.nchem_threshold <= 0) {   // every cell through the network
+        k_sur_feat_batch<<<nb, 256, 0, c->stream>>>(sp, n, rho, T, dtm, m->ifeat);
+        if (forward_fused(m, (int)n, m->ifeat, m->iout, c->stream) != hipSuccess) return EU3D_ERR_HIP;
+        k_sur_apply_batch<<<nb, 256, 0, c->stream>>>(sp, n, rho, T, E, m->iout, dtm, c->quirks, b.cnt, b.rej);
+    } else {   // stiff cells through the network, the rest through the FP64 sub-steps
+        if (eu3d::launch_chem_batch(c, n, rho, T, E, dtm, nchem_max, nullptr, sp.nchem_threshold, sp.T_cold, b.net,
+                                    b.cnt + CNT_NET) != hipSuccess)
+            return EU3D_ERR_HIP;
+        k_sur_feat_batch_list<<<nb, 256, 0, c->stream>>>(sp, n, rho, T, dtm, b.net, b.cnt + CNT_NET, m->ifeat);
+        if (forward_fused(m, (int)n, m->ifeat, m->iout, c->stream, b.cnt + CNT_NET) != hipSuccess) return EU3D_ERR_HIP;
+        k_sur_apply_batch_list<<<nb, 256, 0, c->stream>>>(sp, n, rho, T, E, m->iout, dtm, c->quirks, b.net,
+                                                          b.cnt + CNT_NET, b.cnt, b.rej);
+    }
+    if (eu3d::launch_batch_list(c, n, rho, T, E, dtm, b.rej, b.cnt, (unsigned)n, b.work, b.keep, nchem_max) != hipSuccess)
         return EU3D_ERR_HIP;
     return sur_stats(c, st, n);
 }
@@ -1029,16 +1180,25 @@ hipError_t launch_surrogate_grid(eu3d_ctx* c, int nchem_max) {
     eu3d_mlp* m = c->mlp;
     const long long n = (long long)c->g.ni * c->g.nj * c->g.nk;
     if (!m || ensure_infer(m, (int)n) != hipSuccess || sur_ensure(c, n) != EU3D_OK) return hipErrorInvalidValue;
-    unsigned* list = c->sur_list;
-    unsigned* work = c->sur_list + c->sur_cap;
-    unsigned* cnt = c->sur_list + 2 * c->sur_cap;
-    const dim3 grid((c->g.ni + 63) / 64, (c->g.nj + 3) / 4, c->g.nk), blk(64, 4);
-    k_zero_u32<<<1, 64, 0, c->stream>>>(cnt, 64);
-    k_sur_feat_grid<<<grid, blk, 0, c->stream>>>(m->sp, c->g, c->fl, c->sc, m->ifeat);
-    hipError_t e = forward_fused(m, (int)n, m->ifeat, m->iout, c->stream);
-    if (e != hipSuccess) return e;
-    k_sur_apply_grid<<<grid, blk, 0, c->stream>>>(m->sp, c->g, c->fl, c->sc, m->iout, c->quirks, cnt, list);
-    return launch_trap_list(c, list, cnt, (unsigned)n, work, nchem_max);
+    const SurBufs b = sur_bufs(c);
+    const eu3d_surrogate_params& sp = m->sp;
+    k_zero_u32<<<1, 64, 0, c->stream>>>(b.cnt, 64);
+    if (sp.nchem_threshold <= 0) {   // every cell through the network
+        const dim3 grid((c->g.ni + 63) / 64, (c->g.nj + 3) / 4, c->g.nk), blk(64, 4);
+        k_sur_feat_grid<<<grid, blk, 0, c->stream>>>(sp, c->g, c->fl, c->sc, m->ifeat);
+        hipError_t e = forward_fused(m, (int)n, m->ifeat, m->iout, c->stream);
+        if (e != hipSuccess) return e;
+        k_sur_apply_grid<<<grid, blk, 0, c->stream>>>(sp, c->g, c->fl, c->sc, m->iout, c->quirks, b.cnt, b.rej);
+    } else {   // stiff cells through the network, the rest through the FP64 sub-steps
+        hipError_t e = launch_trap_adaptive(c, nchem_max, false, sp.nchem_threshold, sp.T_cold, b.net, b.cnt + CNT_NET);
+        if (e != hipSuccess) return e;
+        const unsigned nb = (unsigned)((n + 255) / 256);
+        k_sur_feat_grid_list<<<nb, 256, 0, c->stream>>>(sp, c->g, c->fl, c->sc, b.net, b.cnt + CNT_NET, m->ifeat);
+        if ((e = forward_fused(m, (int)n, m->ifeat, m->iout, c->stream, b.cnt + CNT_NET)) != hipSuccess) return e;
+        k_sur_apply_grid_list<<<nb, 256, 0, c->stream>>>(sp, c->g, c->fl, c->sc, m->iout, c->quirks, b.net,
+                                                         b.cnt + CNT_NET, b.cnt, b.rej);
+    }
+    return launch_trap_list(c, b.rej, b.cnt, (unsigned)n, b.work, nchem_max);
 }
 
 hipError_t surrogate_grid_stats(eu3d_ctx* c) {

@@ -54,11 +54,14 @@ class Harvest:
         self.rho0, self.rho1, self.T0, self.dt = [], [], [], []
 
     def add(self, rho0, rho1, T0, dt):
+        """Keep the pairs whose states before and after are finite (a few cells of the FP64
+        chemistry leave the finite range at large dt)."""
         torch = _torch()
-        self.rho0.append(rho0)
-        self.rho1.append(rho1)
-        self.T0.append(T0)
-        self.dt.append(torch.full_like(T0, dt))
+        ok = torch.isfinite(rho0).all(0) & torch.isfinite(rho1).all(0) & torch.isfinite(T0)
+        self.rho0.append(rho0[:, ok].contiguous())
+        self.rho1.append(rho1[:, ok].contiguous())
+        self.T0.append(T0[ok].contiguous())
+        self.dt.append(torch.full_like(self.T0[-1], dt))
 
     def cat(self):
         torch = _torch()
@@ -142,8 +145,10 @@ def harvest_run(H: Harvest, wl, nsteps: int, every: int, cfl=0.5, ramp=(0.05, 20
     g.close()
 
 
-def fit_params(rho0, rho1, T0, dt, hull_margin=0.25, dT_max=40.0, elem_tol=2e-3, y_neg_tol=2e-4, T_cold=500.0):
-    """Normalisation and training hull from harvested pairs."""
+def fit_params(rho0, rho1, T0, dt, hull_margin=0.25, dT_max=40.0, elem_tol=2e-3, y_neg_tol=2e-4, T_cold=500.0,
+               nchem_threshold=0):
+    """Normalisation and training hull from harvested pairs. nchem_threshold > 0 sends only the
+    cells needing more FP64 sub-steps than that through the network."""
     torch = _torch()
     f = raw_features(rho0, T0, dt)
     dY = (rho1 / rho1.sum(0) - rho0 / rho0.sum(0)).T
@@ -162,6 +167,7 @@ def fit_params(rho0, rho1, T0, dt, hull_margin=0.25, dT_max=40.0, elem_tol=2e-3,
     for i in range(NOUT_REAL):
         p.out_mean[i], p.out_std[i] = float(om[i]), float(osd[i])
     p.dT_max, p.elem_tol, p.y_neg_tol, p.T_cold = dT_max, elem_tol, y_neg_tol, T_cold
+    p.nchem_threshold = nchem_threshold
     return p
 
 

@@ -1,0 +1,222 @@
+"""Train the chemistry surrogate on the solver's own FP64 increments and measure what it gains over
+the FP64 chemistry alone, and what it costs in accuracy.
+
+Two workloads, each a JSON report:
+- batch: 2^22 seeded cells on the H2-air equilibrium lines, one chemistry step of dt = 1e-8 s.
+  The FP64 adaptive trapezoid (persistent kernel, tuned schedule) is compared with the hybrid
+  surrogate chemistry. In the hybrid, cells needing more than `threshold` FP64 sub-steps go
+  through the network, the rest take the FP64 sub-steps, and the rejects fall back to FP64.
+  Training pairs come from other seeds of the same generator.
+- detonation: the 3-D planar detonation, plain FP64 steps against steps with the hybrid surrogate
+  chemistry, run side by side from the same initial state. It reports:
+  - step times;
+  - the front position (mean over y and z of the largest x with T > 1500 K);
+  - L-inf / L2 errors of T, p and Y_k;
+  - the fraction of cells sent through the network and the fraction rejected.
+  Training pairs are harvested from plain runs of a smaller transverse section of the same
+  configuration.
+
+Usage: python -m eu3d_amd.surrogate_eval batch|detonation [options]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import time
+
+import numpy as np
+
+from . import autotune, chemistry, configs, surrogate
+from .api import MLP, Context
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _timed(ctx_stream, fn):
+    torch = _torch()
+    s = torch.cuda.ExternalStream(ctx_stream)
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(s)
+    out = fn()
+    b.record(s)
+    b.synchronize()
+    return out, a.elapsed_time(b)
+
+
+def harvest_batch(ctx, seeds, n, dtm, log=print):
+    """(rho0, rho1, T0, dt) pairs: one FP64 chemistry step of equilibrium-line states."""
+    H = surrogate.Harvest()
+    for sd in seeds:
+        log(f"harvest seed {sd}: {n} states")
+        (r, t, e), _ = chemistry.states(ctx, n, seed=sd, burnt="equilibrium", log=log)
+        r0, t0 = r.clone(), t.clone()
+        ctx.chem_batch(n, r.data_ptr(), t.data_ptr(), e.data_ptr(), dtm, 10000)
+        H.add(r0, r, t0, dtm)
+    return H
+
+
+def train_network(H, epochs=12, batch=16384, lr=2e-3, threshold=0, log=print, **fit):
+    r0, r1, t0, dt = H.cat()
+    m = MLP(seed=11)
+    p = surrogate.fit_params(r0, r1, t0, dt, nchem_threshold=threshold, **fit)
+    m.set_surrogate(p)
+    X, Y = surrogate.make_dataset(m, r0, r1, t0, dt)
+    t = time.time()
+    hist = surrogate.train(m, X, Y, epochs=epochs, batch=batch, lr=lr, log=log)
+    return m, p, {"pairs": int(X.shape[0]), "epochs": epochs, "loss_first": hist[0], "loss_last": hist[-1],
+                  "train_s": time.time() - t}
+
+
+def errors(T_a, T_b, r_a, r_b):
+    """Errors of state a against reference b (device tensors; rho [ns][n])."""
+    fin = np.isfinite(T_b) & np.isfinite(T_a)
+    Ya = (r_a / r_a.sum(0))[:, fin]
+    Yb = (r_b / r_b.sum(0))[:, fin]
+    dT = np.abs(T_a[fin] - T_b[fin])
+    return {"T_Linf_K": float(dT.max()), "T_Linf_rel": float((dT / T_b[fin]).max()),
+            "T_L2_rel": float(np.sqrt((dT ** 2).mean()) / np.sqrt((T_b[fin] ** 2).mean())),
+            "Y_Linf": float(np.abs(Ya - Yb).max()), "nonfinite": int((~fin).sum())}
+
+
+def eval_batch(n=1 << 22, dtm=1e-8, threshold=8, train_seeds=(101, 102, 103), n_train=1 << 21, epochs=12, log=print):
+    torch = _torch()
+    x = configs.make_nodes(8, 1e-4)
+    ctx = Context(x, x, x, quirks=0)
+    H = harvest_batch(ctx, train_seeds, n_train, dtm, log=log)
+    log(f"training on {H.n} pairs")
+    m, p, tr = train_network(H, epochs=epochs, threshold=threshold, log=log)
+    del H
+    log(f"evaluation states: {n}")
+    (r0, t0, e0), _ = chemistry.states(ctx, n, seed=2024, burnt="equilibrium", log=log)
+    sched = autotune.load_or_tune(ctx, (r0, t0, e0), dtm)
+    log(f"schedule {sched}")
+    out = {"cells": n, "dt": dtm, "threshold": threshold, "training": tr,
+           "schedule": {k: sched[k] for k in ("block", "blocks_per_cu", "waves")}}
+    # FP64 reference
+    best = None
+    for _ in range(2):
+        r, t, e = r0.clone(), t0.clone(), e0.clone()
+        st, ms = _timed(ctx.stream(), lambda: ctx.chem_batch(n, r.data_ptr(), t.data_ptr(), e.data_ptr(), dtm, 10000))
+        best = ms if best is None else min(best, ms)
+    ref_r, ref_t = r.cpu().numpy(), t.cpu().numpy()
+    out["fp64"] = {"ms": best, "cells_per_s": n / (best * 1e-3), "stats": st}
+    # hybrid surrogate
+    best = None
+    for _ in range(2):
+        r, t, e = r0.clone(), t0.clone(), e0.clone()
+        sst, ms = _timed(ctx.stream(), lambda: ctx.surrogate_batch(m, n, r.data_ptr(), t.data_ptr(), e.data_ptr(), dtm,
+                                                                    10000))
+        best = ms if best is None else min(best, ms)
+    out["surrogate"] = {"ms": best, "cells_per_s": n / (best * 1e-3), "stats": sst,
+                        "network_fraction": sst["network"] / n,
+                        "rejected_fraction_of_network": sst["fallback"] / max(1, sst["network"])}
+    out["speedup"] = out["fp64"]["ms"] / best
+    out["errors_vs_fp64"] = errors(t.cpu().numpy(), ref_t, r.cpu().numpy(), ref_r)
+    ctx.close()
+    return out
+
+
+def front_x(T, xnode, bc=2, T_front=1500.0):
+    """Mean over (y, z) of the largest interior x with T > T_front."""
+    Ti = T[bc:-bc, bc:-bc, bc:-bc]
+    hot = Ti > T_front
+    idx = np.where(hot.any(0), Ti.shape[0] - 1 - np.argmax(hot[::-1], axis=0), 0)
+    return float(np.mean(xnode[bc + idx]))
+
+
+def eval_detonation(grid=(512, 256, 128), nsteps=150, threshold=4, train_grid=(512, 32, 32), train_steps=150,
+                    every=5, epochs=10, log=print):
+    torch = _torch()
+    # harvest from plain runs of a narrower section (same generator, other seed)
+    H = surrogate.Harvest()
+    wl_t = configs.det3d(*train_grid, seed=777)
+    log(f"harvest: plain run {train_grid}, {train_steps} steps")
+    surrogate.harvest_run(H, wl_t, train_steps, every, max_cells=1 << 20)
+    log(f"harvested {H.n} pairs")
+    m, p, tr = train_network(H, epochs=epochs, threshold=threshold, log=log)
+    del H
+    wl = configs.det3d(*grid)
+    ctxs = []
+    for _ in range(2):
+        g = Context(wl.xnode, wl.ynode, wl.znode, quirks=0)
+        for k in ("T", "P", "U", "V", "W", "YI"):
+            g.set(k, wl.prims[k])
+        g.field_initial()
+        ctxs.append(g)
+    plain, sur = ctxs
+    sur.set_surrogate(m)
+    kw = dict(wl.step)
+    kw.pop("cfl")
+    times = {"plain": 0.0, "surrogate": 0.0}
+    net = rej = cells = 0
+    for s in range(nsteps):
+        c = 0.05 + (0.5 - 0.05) * min(1.0, s / 20)
+        _, ms = _timed(plain.stream(), lambda: plain.step(1, cfl=c, bc_kinds=wl.bc_kinds, **kw))
+        times["plain"] += ms
+        kws = dict(kw, chem=4)
+        _, ms = _timed(sur.stream(), lambda: sur.step(1, cfl=c, bc_kinds=wl.bc_kinds, **kws))
+        times["surrogate"] += ms
+        st = sur.surrogate_stats()
+        if (s + 1) % 10 == 0:
+            log(f"step {s + 1}: plain {times['plain'] / (s + 1):.2f} ms/step, surrogate "
+                f"{times['surrogate'] / (s + 1):.2f} ms/step, network {st['network']}, rejected {st['fallback']}")
+        net += st["network"]
+        rej += st["fallback"]
+        cells += st["cells"]
+    Tp, Ts = plain.get("T"), sur.get("T")
+    Pp, Ps = plain.get("P"), sur.get("P")
+    Yp, Ys = plain.get("YI"), sur.get("YI")
+    sl = (slice(2, -2), slice(2, -2), slice(2, -2))
+    fin = np.isfinite(Tp[sl]) & np.isfinite(Ts[sl])
+    dT = np.abs(Ts[sl] - Tp[sl])[fin]
+    dP = np.abs(Ps[sl] - Pp[sl])[fin]
+    out = {"grid": list(grid), "steps": nsteps, "threshold": threshold, "training": tr,
+           "ms_per_step": {k: v / nsteps for k, v in times.items()},
+           "speedup": times["plain"] / times["surrogate"],
+           "front_x_m": {"plain": front_x(Tp, wl.xnode), "surrogate": front_x(Ts, wl.xnode)},
+           "T_Linf_rel": float((dT / np.abs(Tp[sl][fin])).max()),
+           "T_L2_rel": float(np.sqrt((dT ** 2).mean() / (Tp[sl][fin] ** 2).mean())),
+           "p_Linf_rel": float((dP / np.abs(Pp[sl][fin])).max()),
+           "p_L2_rel": float(np.sqrt((dP ** 2).mean() / (Pp[sl][fin] ** 2).mean())),
+           "Y_Linf": float(np.abs(Ys[sl] - Yp[sl])[fin].max()),
+           "network_fraction": net / max(1, cells), "rejected_fraction": rej / max(1, cells),
+           "nonfinite_cells": {"plain": int((~np.isfinite(Tp[sl])).sum()), "surrogate": int((~np.isfinite(Ts[sl])).sum())}}
+    dx = wl.xnode[3] - wl.xnode[2]
+    out["front_error_cells"] = abs(out["front_x_m"]["surrogate"] - out["front_x_m"]["plain"]) / dx
+    for g in ctxs:
+        g.close()
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("workload", choices=("batch", "detonation"))
+    ap.add_argument("--threshold", type=int, default=None)
+    ap.add_argument("--steps", type=int, default=150)
+    ap.add_argument("--grid", type=str, default="512x256x128")
+    ap.add_argument("--epochs", type=int, default=None)
+    a = ap.parse_args()
+    log = lambda s: print(s, flush=True)
+    if a.workload == "batch":
+        kw = {}
+        if a.threshold is not None:
+            kw["threshold"] = a.threshold
+        if a.epochs:
+            kw["epochs"] = a.epochs
+        res = eval_batch(log=log, **kw)
+    else:
+        kw = dict(grid=tuple(int(v) for v in a.grid.split("x")), nsteps=a.steps)
+        if a.threshold is not None:
+            kw["threshold"] = a.threshold
+        if a.epochs:
+            kw["epochs"] = a.epochs
+        res = eval_detonation(log=log, **kw)
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

@@ -209,9 +209,13 @@ typedef struct {
     double elem_tol;                     /* H/O/N conservation bound, relative to the cell's content */
     double y_neg_tol;                    /* most negative accepted mass fraction */
     double T_cold;                       /* cells below this temperature keep their state */
+    int nchem_threshold;                 /* 0: every cell goes through the network; n > 0: only
+                                            cells whose predicted sub-step count exceeds n, the
+                                            others take the FP64 sub-steps */
 } eu3d_surrogate_params;
 typedef struct {
     long long cells, fallback, hull, negative, element, dT;
+    long long network;                   /* cells sent through the network */
 } eu3d_surrogate_stats;
 int eu3d_mlp_create(eu3d_mlp **out, uint64_t seed);
 int eu3d_mlp_destroy(eu3d_mlp *m);

@@ -238,6 +238,7 @@ def test_surrogate_trained_small():
     dT = (t - tf).abs()
     say("  stats", st, "max |dT err|", float(dT.max()), "median", float(dT.median()))
     assert st["fallback"] < 0.5 * n
+    assert st["element"] == 0   # the increments are projected onto element-conserving ones
     assert float(dT.max()) <= 2 * p.dT_max + 1e-6
 
 
@@ -324,3 +325,68 @@ def test_fused_forward_matches_layers(n):
     d = (a - b).abs().max().item()
     say("  max |fused - layers|", d, "bitwise", bool(torch.equal(a, b)))
     assert d <= 1e-3 * b.abs().max().item()
+
+
+def _reject_all_params(m, r):
+    from eu3d_amd import surrogate
+    p = surrogate.fit_params(r, r * 1.0001, r[0] * 0 + 1500.0, 1e-8)
+    p.dT_max = -1.0    # every network cell fails the |dT| check
+    p.T_cold = 0.0
+    p.nchem_threshold = 1
+    m.set_surrogate(p)
+    return p
+
+
+def test_hybrid_batch_routes_stiff_cells():
+    """Hybrid surrogate chemistry: cells needing more than the threshold of sub-steps go to the
+    network, the others take the FP64 sub-steps; with every network cell rejected the result is
+    the FP64 result and the network saw exactly the stiff cells."""
+    torch = _t()
+    from eu3d_amd import configs
+    from eu3d_amd.api import MLP, Context
+    say("case hybrid batch")
+    n = 20000
+    rho, T, E, _ = configs.chem_states(n, seed=11)
+    x = configs.make_nodes(8, 1e-4)
+    g = Context(x, x, x, quirks=0)
+    m = MLP(seed=1)
+    r, t, e = (torch.tensor(a, device="cuda") for a in (rho.T.copy(), T, E))
+    _reject_all_params(m, r)
+    r2, t2, e2 = r.clone(), t.clone(), e.clone()
+    nch = torch.empty(n, dtype=torch.int32, device="cuda")
+    ref = g.chem_batch(n, r2.data_ptr(), t2.data_ptr(), e2.data_ptr(), 7e-6, 10000, nch.data_ptr())
+    st = g.surrogate_batch(m, n, r.data_ptr(), t.data_ptr(), e.data_ptr(), 7e-6, 10000)
+    torch.cuda.synchronize()
+    stiff = int((nch > 1).sum())
+    say("  stats", st, "stiff", stiff)
+    assert stiff > 0 and st["network"] == stiff and st["fallback"] == stiff
+    assert st["hull"] + st["negative"] + st["element"] + st["dT"] == stiff
+    fin = torch.isfinite(t2)
+    assert torch.equal(torch.isfinite(t), fin)
+    assert float((r - r2)[:, fin].abs().max()) <= 1e-13 * float(r2[:, fin].abs().max())
+    assert float((t - t2)[fin].abs().max()) <= 1e-12 * 3000
+
+
+def test_hybrid_step_equals_fp64_when_all_rejected():
+    """eu3d_step with surrogate chemistry in hybrid mode and every network cell rejected equals the
+    plain adaptive-trapezoid step on a small detonation."""
+    torch = _t()
+    from eu3d_amd.api import MLP, Context
+    from tests.common import small_det, init_state
+    say("case hybrid step")
+    wl = small_det()
+    a = Context(wl.xnode, wl.ynode, wl.znode, quirks=0)
+    b = Context(wl.xnode, wl.ynode, wl.znode, quirks=0)
+    for g in (a, b):
+        init_state(g, wl)
+    m = MLP(seed=1)
+    _reject_all_params(m, torch.ones((9, 4), dtype=torch.float64, device="cuda"))
+    a.set_surrogate(m)
+    kw = dict(cfl=0.05, bc_kinds=wl.bc_kinds, rk=1, nchem_max=10000)
+    a.step(3, chem=4, **kw)
+    b.step(3, chem=3, **kw)
+    st = a.surrogate_stats()
+    say("  stats", st)
+    for k in ("CS", "T"):
+        x, y = a.get(k), b.get(k)
+        assert np.abs(x - y).max() <= 1e-12 * np.abs(y).max(), k
