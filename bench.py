@@ -36,7 +36,7 @@ KERNEL_BYTES_PER_CELL = {
     "flux_z": (16 + 13) * 8,
     "flux_xy": (16 + 13 + 13) * 8,
 }
-KERNEL_NAMES = {"imex_fused": "k_imex_fused", "flux_z": "k_flux_z", "flux_xy": "k_flux_xy<32,8>"}
+KERNEL_NAMES = {"imex_fused": "k_imex_fused", "flux_z": "k_flux_z<16>", "flux_xy": "k_flux_xy<16,16,XCD order>"}
 PROFILE = "imex_fused_bubble3d_384x128x128_pmc.json"
 
 

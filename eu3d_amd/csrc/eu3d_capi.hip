@@ -88,6 +88,7 @@ int eu3d_create(eu3d_ctx** out, const eu3d_grid* grid, const eu3d_mech* mech, ui
     g.n3 = (size_t)g.NX * g.NY * g.NZ;
     if (g.n3 >= (1ull << 32)) { delete c; return EU3D_ERR_ARG; }
     if (const char* v = getenv("EU3D_FLUX_XY_TILE")) c->flux_variant = atoi(v);
+    if (const char* v = getenv("EU3D_FLUX_Z_CHUNK")) c->zchunk = atoi(v);
     c->dx0 = grid->xnode[1] - grid->xnode[0];
     c->dy0 = grid->ynode[1] - grid->ynode[0];
     c->dz0 = grid->znode[1] - grid->znode[0];

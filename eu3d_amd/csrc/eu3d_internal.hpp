@@ -53,7 +53,8 @@ struct eu3d_ctx {
     unsigned int* batch_active = nullptr;
     long long batch_cap = 0;
     eu3d_chem_schedule sched{256, 4, 2};
-    int flux_variant = 0;                                          // tile shape of the x/y advection pass
+    int flux_variant = 0;                                          // tile shape / order of the x/y advection pass
+    int zchunk = 16;                                               // z planes per column of the z pass
     struct eu3d_mlp* mlp = nullptr;                                // surrogate used by EU3D_CHEM_SURROGATE
     unsigned* sur_list = nullptr;                                  // fallback list, work list, counters, keep
     long long sur_cap = 0;

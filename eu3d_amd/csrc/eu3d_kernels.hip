@@ -422,7 +422,7 @@ __global__ void __launch_bounds__(256) k_explicit(G3 g, Fields F, uint32_t q) {
 //     and writes RHS = (-dt/dx dF - dt/dy dG) - dt/dz dQ for (X-1) x (Y-1) cells, EE's order of
 //     operations.
 constexpr int NV = 16;
-constexpr int ZX = 64, ZY = 4, ZCH = 32;
+constexpr int ZX = 64, ZY = 4;   // k_flux_z block: 64 x 4 columns; the z chunk is a template parameter
 
 __device__ __forceinline__ const double* prim_ptr(const Fields& F, const G3& g, int v) {
     switch (v) {
@@ -437,15 +437,29 @@ __device__ __forceinline__ const double* prim_ptr(const Fields& F, const G3& g, 
     }
 }
 
-template <int XY_X, int XY_Y>
+// XCD: workgroups are dispatched to the 8 XCDs round-robin by linear id, and each XCD has its own
+// L2. The remap hands every XCD a contiguous range of tiles (x fastest, then y, then z), so the
+// halo columns and rows a tile shares with its neighbours are fetched into one L2, not eight.
+template <int XY_X, int XY_Y, bool XCD>
 __global__ void __launch_bounds__(XY_X * XY_Y) k_flux_xy(G3 g, Fields F, int order, int limiter, uint32_t q,
                                                  const double* __restrict__ xn, const double* __restrict__ yn,
                                                  const Scalars* __restrict__ sc) {
     constexpr int XY_TX = XY_X + 3, XY_TY = XY_Y + 3;
     __shared__ double sp[NV][XY_TY][XY_TX];
+    int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
+    if (XCD) {
+        const unsigned nbx = gridDim.x, nby = gridDim.y, total = nbx * nby * gridDim.z;
+        const unsigned L = bx + nbx * (by + nby * bz), per = total / 8;
+        if (L < per * 8) {
+            const unsigned T = (L % 8) * per + L / 8;
+            bx = T % nbx;
+            by = (T / nbx) % nby;
+            bz = T / (nbx * nby);
+        }
+    }
     const int tx = threadIdx.x, ty = threadIdx.y, t = ty * XY_X + tx;
-    const int X0 = g.bc + blockIdx.x * (XY_X - 1), Y0 = g.bc + blockIdx.y * (XY_Y - 1);
-    const int z = g.bc + blockIdx.z;
+    const int X0 = g.bc + bx * (XY_X - 1), Y0 = g.bc + by * (XY_Y - 1);
+    const int z = g.bc + bz;
     const int x = X0 + tx, y = Y0 + ty;
     const int xe = g.ni + g.bc, ye = g.nj + g.bc;
     const bool out = tx < XY_X - 1 && ty < XY_Y - 1 && x < xe && y < ye;
@@ -509,6 +523,7 @@ __global__ void __launch_bounds__(XY_X * XY_Y) k_flux_xy(G3 g, Fields F, int ord
     }
 }
 
+template <int ZCH>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) k_flux_z(G3 g, Fields F, int order, int limiter, uint32_t q,
                                                 const double* __restrict__ zn, const Scalars* __restrict__ sc) {
     const int x = g.bc + blockIdx.x * ZX + threadIdx.x, y = g.bc + blockIdx.y * ZY + threadIdx.y;
@@ -1054,39 +1069,43 @@ template <bool C3>
 __global__ void __launch_bounds__(256) k_trap_first_list(G3 g, Fields F, uint32_t q, const unsigned* __restrict__ in_list,
                                                          const unsigned* __restrict__ in_count, Scalars* sc,
                                                          unsigned* out_list, int nmax) {
-    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool in = t < *in_count;
-    const size_t c = in ? in_list[t] : 0;
-    int nch = 0;
-    bool active = false;
-    if (in) {
-        ChemCell cell;
-#pragma unroll
-        for (int s = 0; s < NS; s++) cell.Di[s] = FLD(F, EU3D_FIELD_DI)[s * g.n3 + c];
-        cell.U = FLD(F, EU3D_FIELD_U)[c];
-        cell.V = FLD(F, EU3D_FIELD_V)[c];
-        cell.W = FLD(F, EU3D_FIELD_W)[c];
-        cell.E = FLD(F, EU3D_FIELD_E)[c];
-        cell.T = FLD(F, EU3D_FIELD_T)[c];
-        double Mc[NS], Dr[NS], Y[NS];
-        react_units(F, g, c, Mc, Dr);
-#pragma unroll
-        for (int s = 0; s < NS; s++) Y[s] = FLD(F, EU3D_FIELD_YI)[s * g.n3 + c];
-        KinOut ko;
-        kinetics<true, false, C3>(cell.T, Mc, Dr, Dr, (q & Q_C2) ? 1 : 2, q & Q_C4, ko);
-        const double dtm = sc->dtm;
-        nch = chem_nchem(ko, Dr, Y, dtm, nmax);
-        FLD(F, EU3D_FIELD_NCHEM)[c] = (double)nch;
-        if (nch <= 0) atomicOr(&sc->nonfinite, 2);
-        if (nch > 0) {
-            double cs[NC];
-            const Prim o = chem_apply(cell, ko, Dr, dtm / nch, q, cs);
-            GridCells acc{g, F};
-            acc.store((unsigned)c, cs, o, ko);
-            active = nch > 1;
+    // grid-stride over the listed cells (the count is known only on the device)
+    const unsigned cnt = *in_count;
+    for (unsigned b0 = blockIdx.x * blockDim.x; b0 < cnt; b0 += gridDim.x * blockDim.x) {
+        const unsigned t = b0 + threadIdx.x;
+        const bool in = t < cnt;
+        const size_t c = in ? in_list[t] : 0;
+        int nch = 0;
+        bool active = false;
+        if (in) {
+            ChemCell cell;
+    #pragma unroll
+            for (int s = 0; s < NS; s++) cell.Di[s] = FLD(F, EU3D_FIELD_DI)[s * g.n3 + c];
+            cell.U = FLD(F, EU3D_FIELD_U)[c];
+            cell.V = FLD(F, EU3D_FIELD_V)[c];
+            cell.W = FLD(F, EU3D_FIELD_W)[c];
+            cell.E = FLD(F, EU3D_FIELD_E)[c];
+            cell.T = FLD(F, EU3D_FIELD_T)[c];
+            double Mc[NS], Dr[NS], Y[NS];
+            react_units(F, g, c, Mc, Dr);
+    #pragma unroll
+            for (int s = 0; s < NS; s++) Y[s] = FLD(F, EU3D_FIELD_YI)[s * g.n3 + c];
+            KinOut ko;
+            kinetics<true, false, C3>(cell.T, Mc, Dr, Dr, (q & Q_C2) ? 1 : 2, q & Q_C4, ko);
+            const double dtm = sc->dtm;
+            nch = chem_nchem(ko, Dr, Y, dtm, nmax);
+            FLD(F, EU3D_FIELD_NCHEM)[c] = (double)nch;
+            if (nch <= 0) atomicOr(&sc->nonfinite, 2);
+            if (nch > 0) {
+                double cs[NC];
+                const Prim o = chem_apply(cell, ko, Dr, dtm / nch, q, cs);
+                GridCells acc{g, F};
+                acc.store((unsigned)c, cs, o, ko);
+                active = nch > 1;
+            }
         }
+        chem_count(sc, in ? nch : 0, active, out_list, (unsigned)c);
     }
-    chem_count(sc, in ? nch : 0, active, out_list, (unsigned)c);
 }
 
 template <bool C3>
@@ -1094,37 +1113,41 @@ __global__ void __launch_bounds__(256) k_batch_first_list(BatchCells acc, int* n
                                                           const unsigned* __restrict__ in_list,
                                                           const unsigned* __restrict__ in_count, Scalars* sc,
                                                           unsigned* out_list) {
-    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool in = t < *in_count;
-    const long long id = in ? in_list[t] : 0;
-    int nch = 0;
-    bool active = false;
-    if (in) {
-        ChemCell cell;
-#pragma unroll
-        for (int s = 0; s < NS; s++) cell.Di[s] = acc.rho[(size_t)s * acc.n + id];
-        cell.U = cell.V = cell.W = 0.0;
-        cell.E = acc.E[id];
-        cell.T = acc.T[id];
-        double Mc[NS], Dr[NS], Y[NS], D = 0;
-        chem_inputs(cell, Mc, Dr);
-#pragma unroll
-        for (int s = 0; s < NS; s++) D += cell.Di[s];
-#pragma unroll
-        for (int s = 0; s < NS; s++) Y[s] = cell.Di[s] / D;
-        KinOut ko;
-        kinetics<true, false, C3>(cell.T, Mc, Dr, Dr, (q & Q_C2) ? 1 : 2, q & Q_C4, ko);
-        nch = chem_nchem(ko, Dr, Y, dtm, nmax);
-        nchem_keep[id] = nch;
-        if (nch <= 0) atomicOr(&sc->nonfinite, 2);
-        if (nch > 0) {
-            double cs[NC];
-            const Prim o = chem_apply(cell, ko, Dr, dtm / nch, q, cs);
-            acc.store((unsigned)id, cs, o, ko);
-            active = nch > 1;
+    // grid-stride over the listed cells (the count is known only on the device)
+    const unsigned cnt = *in_count;
+    for (unsigned b0 = blockIdx.x * blockDim.x; b0 < cnt; b0 += gridDim.x * blockDim.x) {
+        const unsigned t = b0 + threadIdx.x;
+        const bool in = t < cnt;
+        const long long id = in ? in_list[t] : 0;
+        int nch = 0;
+        bool active = false;
+        if (in) {
+            ChemCell cell;
+    #pragma unroll
+            for (int s = 0; s < NS; s++) cell.Di[s] = acc.rho[(size_t)s * acc.n + id];
+            cell.U = cell.V = cell.W = 0.0;
+            cell.E = acc.E[id];
+            cell.T = acc.T[id];
+            double Mc[NS], Dr[NS], Y[NS], D = 0;
+            chem_inputs(cell, Mc, Dr);
+    #pragma unroll
+            for (int s = 0; s < NS; s++) D += cell.Di[s];
+    #pragma unroll
+            for (int s = 0; s < NS; s++) Y[s] = cell.Di[s] / D;
+            KinOut ko;
+            kinetics<true, false, C3>(cell.T, Mc, Dr, Dr, (q & Q_C2) ? 1 : 2, q & Q_C4, ko);
+            nch = chem_nchem(ko, Dr, Y, dtm, nmax);
+            nchem_keep[id] = nch;
+            if (nch <= 0) atomicOr(&sc->nonfinite, 2);
+            if (nch > 0) {
+                double cs[NC];
+                const Prim o = chem_apply(cell, ko, Dr, dtm / nch, q, cs);
+                acc.store((unsigned)id, cs, o, ko);
+                active = nch > 1;
+            }
         }
+        chem_count(sc, in ? nch : 0, active, out_list, (unsigned)id);
     }
-    chem_count(sc, in ? nch : 0, active, out_list, (unsigned)id);
 }
 
 __global__ void k_chem_reset(Scalars* sc, double* dtm_slot, double dtm, int use_dt) {
@@ -1229,32 +1252,41 @@ hipError_t launch_cfl(eu3d_ctx* c, double cfl, double final_time) {
     return hipGetLastError();
 }
 
-template <int X, int Y>
+template <int X, int Y, bool XCD>
 static void launch_xy(eu3d_ctx* c, int diff, int limiter) {
     const G3& g = c->g;
     dim3 gxy((g.ni + X - 2) / (X - 1), (g.nj + Y - 2) / (Y - 1), g.nk);
-    k_flux_xy<X, Y><<<gxy, dim3(X, Y), 0, c->stream>>>(g, c->fl, diff, limiter, c->quirks, c->xnode, c->ynode, c->sc);
+    k_flux_xy<X, Y, XCD><<<gxy, dim3(X, Y), 0, c->stream>>>(g, c->fl, diff, limiter, c->quirks, c->xnode, c->ynode, c->sc);
+}
+
+template <int ZCH>
+static void launch_z(eu3d_ctx* c, int diff, int limiter) {
+    const G3& g = c->g;
+    dim3 gz((g.ni + ZX - 1) / ZX, (g.nj + ZY - 1) / ZY, (g.nk + ZCH - 1) / ZCH);
+    k_flux_z<ZCH><<<gz, dim3(ZX, ZY), 0, c->stream>>>(g, c->fl, diff, limiter, c->quirks, c->znode, c->sc);
 }
 
 hipError_t launch_flux_z(eu3d_ctx* c, int diff, int limiter) {
-    const G3& g = c->g;
-    dim3 gz((g.ni + ZX - 1) / ZX, (g.nj + ZY - 1) / ZY, (g.nk + ZCH - 1) / ZCH);
-    k_flux_z<<<gz, dim3(ZX, ZY), 0, c->stream>>>(g, c->fl, diff, limiter, c->quirks, c->znode, c->sc);
+    // z planes per column chunk: shorter chunks give more waves to hide latency with, at the cost
+    // of re-reading 3 planes of stencil per chunk (bubble3d 384x128x128: 16 planes 0.466 ms,
+    // 8 planes 0.492 ms, 32 planes 0.499 ms)
+    switch (c->zchunk) {
+    case 8: launch_z<8>(c, diff, limiter); break;
+    case 32: launch_z<32>(c, diff, limiter); break;
+    default: launch_z<16>(c, diff, limiter);
+    }
     return hipGetLastError();
 }
 
 hipError_t launch_flux_xy(eu3d_ctx* c, int diff, int limiter) {
-    // tile shape of the x/y pass (threads in x, y); 32 x 8 measured fastest on bubble3d
-    // 384x128x128 among 64x8, 64x6, 64x4, 32x8, 32x4 (the others stay selectable)
+    // tile shape of the x/y pass (threads in x, y) and the XCD-aware tile order. On bubble3d
+    // 384x128x128: 16x16 with the XCD order 0.671 ms, 32x8 0.693 ms (0.697 with the XCD order),
+    // 64x8 1.185 ms; 64x6, 64x4, 32x4 were slower than 32x8 in an earlier sweep.
     switch (c->flux_variant) {
-    case 1: launch_xy<64, 8>(c, diff, limiter); break;
-    case 2: launch_xy<64, 4>(c, diff, limiter); break;
-    case 3: launch_xy<64, 6>(c, diff, limiter); break;
-    case 4: launch_xy<32, 4>(c, diff, limiter); break;
-    case 5: launch_xy<16, 16>(c, diff, limiter); break;
-    case 6: launch_xy<32, 12>(c, diff, limiter); break;
-    case 7: launch_xy<16, 8>(c, diff, limiter); break;
-    default: launch_xy<32, 8>(c, diff, limiter);
+    case 1: launch_xy<32, 8, true>(c, diff, limiter); break;
+    case 2: launch_xy<64, 8, true>(c, diff, limiter); break;
+    case 3: launch_xy<32, 8, false>(c, diff, limiter); break;
+    default: launch_xy<16, 16, true>(c, diff, limiter);
     }
     return hipGetLastError();
 }
@@ -1375,7 +1407,7 @@ hipError_t launch_trap_list(eu3d_ctx* c, const unsigned* list, const unsigned* c
                             unsigned* work, int nchem_max) {
     const int nmax = nchem_max > 0 ? nchem_max : 10000;
     k_chem_reset<<<1, 1, 0, c->stream>>>(c->sc, &c->sc->dtm, 0.0, 1);
-    const unsigned nb = (max_count + 255) / 256;
+    const unsigned nb = std::min<unsigned>((max_count + 255) / 256, (unsigned)c->num_cu * 8);
     const bool c3 = c->quirks & Q_C3;
     if (nb) {
         if (c3)
@@ -1395,7 +1427,7 @@ hipError_t launch_batch_list(eu3d_ctx* c, long long n, double* rho, double* T, d
     const int nmax = nchem_max > 0 ? nchem_max : 10000;
     k_chem_reset<<<1, 1, 0, c->stream>>>(c->sc, &c->sc->dtm, dtm, 0);
     BatchCells acc{n, rho, T, E, nchem_keep};
-    const unsigned nb = (max_count + 255) / 256;
+    const unsigned nb = std::min<unsigned>((max_count + 255) / 256, (unsigned)c->num_cu * 8);
     const bool c3 = c->quirks & Q_C3;
     if (nb) {
         if (c3)

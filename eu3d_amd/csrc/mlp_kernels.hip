@@ -344,20 +344,22 @@ __global__ void __launch_bounds__(FT) k_mlp_fused(MlpView v, int n_max, const un
     __shared__ bf16_t act[FM][FPITCH];
     __shared__ bf16_t ws[2][256][WPITCH];
     const int tid = threadIdx.x, lane = tid & 63, wr = (tid >> 6) * 16;
-    const long long row0 = (long long)blockIdx.x * FM;
     const int n = count ? min(n_max, (int)*count) : n_max;
-    if (row0 >= n) return;   // the same for the whole block
-    for (int c = tid; c < FM * 4; c += FT) {   // input features: FM rows x 32 BF16
-        const int r = c >> 2, kc = (c & 3) * 8;
-        uint4 val = make_uint4(0, 0, 0, 0);
-        if (row0 + r < n) val = *reinterpret_cast<const uint4*>(X + (size_t)(row0 + r) * 32 + kc);
-        *reinterpret_cast<uint4*>(&act[r][kc]) = val;
+    // persistent over the row tiles: the grid is sized for n_max, the work by the device count
+    for (long long row0 = (long long)blockIdx.x * FM; row0 < n; row0 += (long long)gridDim.x * FM) {
+        __syncthreads();   // the previous tile's last layer is done with act
+        for (int c = tid; c < FM * 4; c += FT) {   // input features: FM rows x 32 BF16
+            const int r = c >> 2, kc = (c & 3) * 8;
+            uint4 val = make_uint4(0, 0, 0, 0);
+            if (row0 + r < n) val = *reinterpret_cast<const uint4*>(X + (size_t)(row0 + r) * 32 + kc);
+            *reinterpret_cast<uint4*>(&act[r][kc]) = val;
+        }
+        fused_layer<32, 256, false>(act, ws, v.wt[0], v.b[0], tid, lane, wr, row0, n, Y);
+        fused_layer<256, 256, false>(act, ws, v.wt[1], v.b[1], tid, lane, wr, row0, n, Y);
+        fused_layer<256, 256, false>(act, ws, v.wt[2], v.b[2], tid, lane, wr, row0, n, Y);
+        fused_layer<256, 256, false>(act, ws, v.wt[3], v.b[3], tid, lane, wr, row0, n, Y);
+        fused_layer<256, 16, true>(act, ws, v.wt[4], v.b[4], tid, lane, wr, row0, n, Y);
     }
-    fused_layer<32, 256, false>(act, ws, v.wt[0], v.b[0], tid, lane, wr, row0, n, Y);
-    fused_layer<256, 256, false>(act, ws, v.wt[1], v.b[1], tid, lane, wr, row0, n, Y);
-    fused_layer<256, 256, false>(act, ws, v.wt[2], v.b[2], tid, lane, wr, row0, n, Y);
-    fused_layer<256, 256, false>(act, ws, v.wt[3], v.b[3], tid, lane, wr, row0, n, Y);
-    fused_layer<256, 16, true>(act, ws, v.wt[4], v.b[4], tid, lane, wr, row0, n, Y);
 }
 
 }  // namespace
@@ -376,6 +378,7 @@ struct eu3d_mlp {
     bf16_t* zs[5] = {};           // pre-activations of layers 1..4
     bf16_t *dz_a = nullptr, *dz_b = nullptr;
     float* yout = nullptr;        // [cap x 16]
+    int num_cu = 256;             // compute units of the device (grid sizes of the list kernels)
     int icap = 0;                 // inference rows (features and outputs only)
     bf16_t* ifeat = nullptr;      // [icap x 32]
     float* iout = nullptr;        // [icap x 16]
@@ -479,7 +482,10 @@ hipError_t forward_fused(eu3d_mlp* m, int n, const bf16_t* X, float* Y, hipStrea
         v.wt[l] = m->wt[l];
         v.b[l] = m->b[l];
     }
-    k_mlp_fused<<<(unsigned)((n + FM - 1) / FM), FT, 0, st>>>(v, n, count, X, Y);
+    // one 8-wave block per CU fits its LDS; a device count means a grid-stride over the tiles
+    const unsigned tiles = (unsigned)((n + FM - 1) / FM);
+    const unsigned nb = count ? std::min<unsigned>(tiles, (unsigned)m->num_cu) : tiles;
+    k_mlp_fused<<<nb, FT, 0, st>>>(v, n, count, X, Y);
     return hipGetLastError();
 }
 
@@ -776,72 +782,80 @@ __global__ void __launch_bounds__(256) k_sur_feat_batch_list(eu3d_surrogate_para
                                                              const double* __restrict__ T, double dt,
                                                              const unsigned* __restrict__ list, const unsigned* __restrict__ count,
                                                              bf16_t* __restrict__ feats) {
-    const unsigned r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= *count) return;
-    const size_t id = list[r];
-    double Y[NS], D = 0;
+    const unsigned cnt = *count;
+    for (unsigned r = blockIdx.x * blockDim.x + threadIdx.x; r < cnt; r += gridDim.x * blockDim.x) {
+        const size_t id = list[r];
+        double Y[NS], D = 0;
 #pragma unroll
-    for (int s = 0; s < NS; s++) D += rho[(size_t)s * n + id];
+        for (int s = 0; s < NS; s++) D += rho[(size_t)s * n + id];
 #pragma unroll
-    for (int s = 0; s < NS; s++) Y[s] = rho[(size_t)s * n + id] / D;
-    float f[NIN_REAL];
-    sur_features(sp, Y, T[id], D, dt, f);
-    store_feats(feats, r, f);
+        for (int s = 0; s < NS; s++) Y[s] = rho[(size_t)s * n + id] / D;
+        float f[NIN_REAL];
+        sur_features(sp, Y, T[id], D, dt, f);
+        store_feats(feats, r, f);
+    }
 }
 
 __global__ void __launch_bounds__(256) k_sur_apply_batch_list(eu3d_surrogate_params sp, long long n, double* rho, double* T,
                                                               double* E, const float* __restrict__ out, double dt, uint32_t q,
                                                               const unsigned* __restrict__ list, const unsigned* __restrict__ count,
-                                                              unsigned* cnt, unsigned* rej) {
-    const unsigned r = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool in = r < *count;
-    const unsigned id = in ? list[r] : 0;
-    int why = 0;
-    if (in) {
-        SurCell c;
+                                                              unsigned* cnt_out, unsigned* rej) {
+    const unsigned cnt = *count;
+    for (unsigned b0 = blockIdx.x * blockDim.x; b0 < cnt; b0 += gridDim.x * blockDim.x) {
+        const unsigned r = b0 + threadIdx.x;
+        const bool in = r < cnt;
+        const unsigned id = in ? list[r] : 0;
+        int why = 0;
+        if (in) {
+            SurCell c;
 #pragma unroll
-        for (int s = 0; s < NS; s++) c.Di[s] = rho[(size_t)s * n + id];
-        c.U = c.V = c.W = 0.0;
-        c.E = E[id];
-        c.T = T[id];
-        double cs[NC];
-        Prim o;
-        why = sur_apply(sp, c, out + (size_t)r * NOUT, dt, q, cs, o);
-        if (why == 0) {
+            for (int s = 0; s < NS; s++) c.Di[s] = rho[(size_t)s * n + id];
+            c.U = c.V = c.W = 0.0;
+            c.E = E[id];
+            c.T = T[id];
+            double cs[NC];
+            Prim o;
+            why = sur_apply(sp, c, out + (size_t)r * NOUT, dt, q, cs, o);
+            if (why == 0) {
 #pragma unroll
-            for (int s = 0; s < NS; s++) rho[(size_t)s * n + id] = cs[s];
-            T[id] = o.T;
-            E[id] = o.E;
+                for (int s = 0; s < NS; s++) rho[(size_t)s * n + id] = cs[s];
+                T[id] = o.T;
+                E[id] = o.E;
+            }
         }
+        sur_count(cnt_out, rej, id, why);
     }
-    sur_count(cnt, rej, id, why);
 }
 
 __global__ void __launch_bounds__(256) k_sur_feat_grid_list(eu3d_surrogate_params sp, G3 g, Fields F, const Scalars* sc,
                                                             const unsigned* __restrict__ list, const unsigned* __restrict__ count,
                                                             bf16_t* __restrict__ feats) {
-    const unsigned r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= *count) return;
-    const size_t c = list[r];
-    double Y[NS], D = 0;
+    const unsigned cnt = *count;
+    for (unsigned r = blockIdx.x * blockDim.x + threadIdx.x; r < cnt; r += gridDim.x * blockDim.x) {
+        const size_t c = list[r];
+        double Y[NS], D = 0;
 #pragma unroll
-    for (int s = 0; s < NS; s++) D += FLD(F, EU3D_FIELD_CS)[s * g.n3 + c];
+        for (int s = 0; s < NS; s++) D += FLD(F, EU3D_FIELD_CS)[s * g.n3 + c];
 #pragma unroll
-    for (int s = 0; s < NS; s++) Y[s] = FLD(F, EU3D_FIELD_CS)[s * g.n3 + c] / D;
-    float f[NIN_REAL];
-    sur_features(sp, Y, FLD(F, EU3D_FIELD_T)[c], D, sc->dt, f);
-    store_feats(feats, r, f);
+        for (int s = 0; s < NS; s++) Y[s] = FLD(F, EU3D_FIELD_CS)[s * g.n3 + c] / D;
+        float f[NIN_REAL];
+        sur_features(sp, Y, FLD(F, EU3D_FIELD_T)[c], D, sc->dt, f);
+        store_feats(feats, r, f);
+    }
 }
 
 __global__ void __launch_bounds__(256) k_sur_apply_grid_list(eu3d_surrogate_params sp, G3 g, Fields F, const Scalars* sc,
                                                              const float* __restrict__ out, uint32_t q,
                                                              const unsigned* __restrict__ list, const unsigned* __restrict__ count,
-                                                             unsigned* cnt, unsigned* rej) {
-    const unsigned r = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool in = r < *count;
-    const size_t c = in ? list[r] : 0;
-    const int why = in ? sur_apply_grid_cell(sp, g, F, sc->dt, q, c, out + (size_t)r * NOUT) : 0;
-    sur_count(cnt, rej, (unsigned)c, why);
+                                                             unsigned* cnt_out, unsigned* rej) {
+    const unsigned cnt = *count;
+    for (unsigned b0 = blockIdx.x * blockDim.x; b0 < cnt; b0 += gridDim.x * blockDim.x) {
+        const unsigned r = b0 + threadIdx.x;
+        const bool in = r < cnt;
+        const size_t c = in ? list[r] : 0;
+        const int why = in ? sur_apply_grid_cell(sp, g, F, sc->dt, q, c, out + (size_t)r * NOUT) : 0;
+        sur_count(cnt_out, rej, (unsigned)c, why);
+    }
 }
 
 __global__ void k_zero_u32(unsigned* p, int n) {
@@ -860,6 +874,12 @@ int eu3d_mlp_create(eu3d_mlp** out, uint64_t seed) {
     const int rin[5] = {NIN_REAL, NH, NH, NH, NH}, rout[5] = {NH, NH, NH, NH, NOUT_REAL};
     // blocking stream: ordered after the caller's null-stream work on the buffers it passes in
     if (hipStreamCreateWithFlags(&m->stream, hipStreamDefault) != hipSuccess) { delete m; return EU3D_ERR_HIP; }
+    {
+        int dev = 0, ncu = 0;
+        if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+            ncu > 0)
+            m->num_cu = ncu;
+    }
     uint64_t st = seed * 6364136223846793005ull + 1442695040888963407ull;
     auto rnd = [&]() {   // uniform (-1, 1) from a 64-bit LCG
         st = st * 6364136223846793005ull + 1442695040888963407ull;
@@ -1137,9 +1157,10 @@ int eu3d_surrogate_batch(eu3d_ctx* c, eu3d_mlp* m, long long n, double* rho, dou
         if (eu3d::launch_chem_batch(c, n, rho, T, E, dtm, nchem_max, nullptr, sp.nchem_threshold, sp.T_cold, b.net,
                                     b.cnt + CNT_NET) != hipSuccess)
             return EU3D_ERR_HIP;
-        k_sur_feat_batch_list<<<nb, 256, 0, c->stream>>>(sp, n, rho, T, dtm, b.net, b.cnt + CNT_NET, m->ifeat);
+        const unsigned nl = std::min<unsigned>(nb, (unsigned)c->num_cu * 8);
+        k_sur_feat_batch_list<<<nl, 256, 0, c->stream>>>(sp, n, rho, T, dtm, b.net, b.cnt + CNT_NET, m->ifeat);
         if (forward_fused(m, (int)n, m->ifeat, m->iout, c->stream, b.cnt + CNT_NET) != hipSuccess) return EU3D_ERR_HIP;
-        k_sur_apply_batch_list<<<nb, 256, 0, c->stream>>>(sp, n, rho, T, E, m->iout, dtm, c->quirks, b.net,
+        k_sur_apply_batch_list<<<nl, 256, 0, c->stream>>>(sp, n, rho, T, E, m->iout, dtm, c->quirks, b.net,
                                                           b.cnt + CNT_NET, b.cnt, b.rej);
     }
     if (eu3d::launch_batch_list(c, n, rho, T, E, dtm, b.rej, b.cnt, (unsigned)n, b.work, b.keep, nchem_max) != hipSuccess)
@@ -1192,7 +1213,7 @@ hipError_t launch_surrogate_grid(eu3d_ctx* c, int nchem_max) {
     } else {   // stiff cells through the network, the rest through the FP64 sub-steps
         hipError_t e = launch_trap_adaptive(c, nchem_max, false, sp.nchem_threshold, sp.T_cold, b.net, b.cnt + CNT_NET);
         if (e != hipSuccess) return e;
-        const unsigned nb = (unsigned)((n + 255) / 256);
+        const unsigned nb = std::min<unsigned>((unsigned)((n + 255) / 256), (unsigned)c->num_cu * 8);
         k_sur_feat_grid_list<<<nb, 256, 0, c->stream>>>(sp, c->g, c->fl, c->sc, b.net, b.cnt + CNT_NET, m->ifeat);
         if ((e = forward_fused(m, (int)n, m->ifeat, m->iout, c->stream, b.cnt + CNT_NET)) != hipSuccess) return e;
         k_sur_apply_grid_list<<<nb, 256, 0, c->stream>>>(sp, c->g, c->fl, c->sc, m->iout, c->quirks, b.net,
