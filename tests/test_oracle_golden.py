@@ -450,3 +450,62 @@ def test_field_initial_matches_shipped_output():
     post = [q for q in pts if q["class"] == 0]
     assert post and all(abs(q["U"] - up) < 5e-6 * up and abs(q["P"] - pp) < 5e-6 * pp and
                         abs(q["T"] - tp) < 5e-6 * tp for q in post)
+
+
+# ------------------------------------------------------------------ ghost kinds and dT/drho_s
+PRIM_FIELDS = ("U", "V", "W", "P", "D", "T", "H", "GAMMA", "E", "YI")
+
+
+def test_ghost_kinds_closed_forms():
+    """Outflow copies the first interior cell, periodic copies the cell one period away, reflect
+    mirrors across the face with the normal velocity negated (x faces over the interior j, k)."""
+    o = flow(ni=6, nj=5, nk=4)
+    random_prims(o, seed=3)
+    bc, ni = 2, 6
+    for kind in (1, 2, 3):
+        random_prims(o, seed=10 + kind)
+        before = {f: o.get(f) for f in PRIM_FIELDS}
+        o.fill_ghosts((kind, kind, 4, 4, 4, 4))
+        after = {f: o.get(f) for f in PRIM_FIELDS}
+        J, K = slice(bc, bc + 5), slice(bc, bc + 4)
+        for f in PRIM_FIELDS:
+            sign = -1.0 if (kind == 2 and f == "U") else 1.0
+            for g in range(bc):
+                lo, hi = bc - 1 - g, ni + bc + g              # ghosts at distance g + 1 from the faces
+                if kind == 1:
+                    slo, shi = bc, ni + bc - 1
+                elif kind == 2:
+                    slo, shi = bc + g, ni + bc - 1 - g
+                else:
+                    slo, shi = ni + bc - 1 - g, bc + g
+                assert np.array_equal(after[f][lo, J, K], sign * before[f][slo, J, K]), (kind, f, g, "lo")
+                assert np.array_equal(after[f][hi, J, K], sign * before[f][shi, J, K]), (kind, f, g, "hi")
+        # interior untouched
+        for f in PRIM_FIELDS:
+            assert np.array_equal(after[f][bc:bc + ni, J, K], before[f][bc:bc + ni, J, K])
+
+
+def test_partial_t_matches_finite_difference():
+    """GetPartial_T (FIXED units): dT/drho_s at constant rho u and rho E equals the central finite
+    difference of the energy Newton, (1/2 |u|^2 - e_s) / (rho c_v)."""
+    o = flow(ni=4, nj=3, nk=3)
+    random_prims(o, seed=5, T=(900.0, 2400.0))
+    o.update_after_cs()
+    o.partial_t()
+    pt = o.get("PT")
+    cs0, T0 = o.get("CS"), o.get("T")
+    cell = (3, 3, 3)
+    for s in range(M.ns):
+        d = 0.5 / abs(pt[cell + (s,)])          # a perturbation that moves T by about 0.5 K
+        Ts = []
+        for sgn in (1, -1):
+            cs = cs0.copy()
+            cs[cell + (s,)] += sgn * d
+            o.set("CS", cs)
+            o.set("T", T0)
+            o.update_after_cs()
+            Ts.append(o.get("T")[cell])
+        fd = (Ts[0] - Ts[1]) / (2 * d)
+        assert abs(fd - pt[cell + (s,)]) <= 2e-6 * abs(pt[cell + (s,)]), (s, fd, pt[cell + (s,)])
+    o.set("CS", cs0)
+    o.set("T", T0)
