@@ -81,7 +81,7 @@ class ChemStats(C.Structure):
 
 
 class ChemSchedule(C.Structure):
-    _fields_ = [("block", C.c_int), ("blocks_per_cu", C.c_int), ("waves", C.c_int)]
+    _fields_ = [("block", C.c_int), ("blocks_per_cu", C.c_int), ("waves", C.c_int), ("tail", C.c_int)]
 
 
 class SurrogateParams(C.Structure):
@@ -318,16 +318,19 @@ class Context:
         return {"cells": s.cells, "active_cells": s.active_cells, "substeps_total": s.substeps_total,
                 "nchem_max_seen": s.nchem_max_seen}
 
-    def set_schedule(self, block: int, blocks_per_cu: int, waves: int = 2):
-        """Persistent chemistry kernel schedule: threads per block, blocks per CU and the
-        occupancy target (waves per SIMD: 1, 2 or 4) of its register allocation."""
-        s = ChemSchedule(block, blocks_per_cu, waves)
+    def set_schedule(self, block: int, blocks_per_cu: int, waves: int = 2, tail: int | None = None):
+        """Persistent chemistry kernel schedule: threads per block, blocks per CU, the occupancy
+        target (waves per SIMD: 1, 2 or 4) of its register allocation, and the sub-step count above
+        which a cell runs one-per-wavefront (None keeps the current value, 0 disables)."""
+        if tail is None:
+            tail = self.get_schedule()["tail"]
+        s = ChemSchedule(block, blocks_per_cu, waves, tail)
         self._chk(self.lib.eu3d_chem_set_schedule(self.h, C.byref(s)), "set_schedule")
 
     def get_schedule(self):
         s = ChemSchedule()
         self._chk(self.lib.eu3d_chem_get_schedule(self.h, C.byref(s)), "get_schedule")
-        return {"block": s.block, "blocks_per_cu": s.blocks_per_cu, "waves": s.waves}
+        return {"block": s.block, "blocks_per_cu": s.blocks_per_cu, "waves": s.waves, "tail": s.tail}
 
     def set_surrogate(self, mlp):
         self._chk(self.lib.eu3d_set_surrogate(self.h, mlp.h if mlp is not None else None), "set_surrogate")

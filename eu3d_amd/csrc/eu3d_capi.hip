@@ -136,6 +136,7 @@ int eu3d_destroy(eu3d_ctx* c) {
     if (c->active) (void)hipFree(c->active);
     if (c->part) (void)hipFree(c->part);
     if (c->rk_save) (void)hipFree(c->rk_save);
+    if (c->coop_list) (void)hipFree(c->coop_list);
     if (c->batch_active) (void)hipFree(c->batch_active);
     if (c->sur_list) (void)hipFree(c->sur_list);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -277,7 +278,7 @@ static int chem_status(eu3d_ctx* c) {
     Scalars s;
     HIPCHK(c, hipMemcpyAsync(&s, c->sc, sizeof(Scalars), hipMemcpyDeviceToHost, c->stream), "chem stats");
     HIPCHK(c, hipStreamSynchronize(c->stream), "chem sync");
-    c->stats.active_cells = (long long)s.chem_active;
+    c->stats.active_cells = (long long)s.chem_active + (long long)s.coop_count;
     c->stats.substeps_total = (long long)s.chem_substeps;
     c->stats.nchem_max_seen = (int)s.chem_nmax;
     if (s.nonfinite & 2) return fail(c, EU3D_ERR_NCHEM, "Nchem <= 0 (src/Reaction.cpp:664-668)");
@@ -348,6 +349,7 @@ int eu3d_chem_set_schedule(eu3d_ctx* c, const eu3d_chem_schedule* s) {
     if (s->blocks_per_cu < 1 || s->blocks_per_cu > 64) return fail(c, EU3D_ERR_ARG, "blocks_per_cu in [1,64]");
     c->sched = *s;
     if (c->sched.waves != 1 && c->sched.waves != 2 && c->sched.waves != 4) c->sched.waves = 2;
+    if (c->sched.tail < 0) c->sched.tail = 0;
     return EU3D_OK;
 }
 

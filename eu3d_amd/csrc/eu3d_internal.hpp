@@ -27,11 +27,12 @@ struct Fields {
 struct Scalars {
     double dt, time, dtm;            // dtm: chemistry interval of the current adaptive call
     int nonfinite;
-    int pad;
+    unsigned coop_count;             // cells queued for the one-cell-per-wavefront kernel
     unsigned long long chem_head;    // persistent-kernel work queue head
     unsigned long long chem_active;  // number of queued cells
     unsigned long long chem_substeps;
     unsigned long long chem_nmax;
+    unsigned long long coop_head;    // work queue head of the one-cell-per-wavefront kernel
 };
 
 }  // namespace eu3d
@@ -52,7 +53,9 @@ struct eu3d_ctx {
     // batched chemistry scratch
     unsigned int* batch_active = nullptr;
     long long batch_cap = 0;
-    eu3d_chem_schedule sched{256, 4, 2};
+    eu3d_chem_schedule sched{256, 4, 2, 512};
+    unsigned int* coop_list = nullptr;                             // cells with the longest sub-step chains
+    long long coop_cap = 0;
     int flux_variant = 0;                                          // tile shape / order of the x/y advection pass
     int zchunk = 16;                                               // z planes per column of the z pass
     struct eu3d_mlp* mlp = nullptr;                                // surrogate used by EU3D_CHEM_SURROGATE
@@ -89,6 +92,7 @@ hipError_t launch_trapezoid(eu3d_ctx* c);
 hipError_t launch_trap_adaptive(eu3d_ctx* c, int nchem_max, bool predict_only, int nthr = 0, double T_cold = 0.0,
                                 unsigned* slist = nullptr, unsigned* scount = nullptr);
 hipError_t launch_check_finite(eu3d_ctx* c);
+hipError_t ensure_coop(eu3d_ctx* c, long long n);
 hipError_t launch_set_scalars(eu3d_ctx* c, double dt, double time);
 hipError_t launch_step(eu3d_ctx* c, const eu3d_step_params* p, bool first);
 hipError_t launch_surrogate_grid(eu3d_ctx* c, int nchem_max);

@@ -913,6 +913,13 @@ __device__ __forceinline__ void chem_count(Scalars* sc, int nch, bool active, un
     if (active) list[base + __popcll(am & ((1ull << lane) - 1ull))] = id;
 }
 
+// Where a cell with more than one sub-step continues: the persistent kernel's list, or, above the
+// tail threshold, the list of the one-cell-per-wavefront kernel.
+struct TailRoute {
+    unsigned* list;
+    int tail;   // 0: no routing
+};
+
 // Wave-aggregated append of the flagged lanes' ids to a device list (one atomic per wave).
 __device__ __forceinline__ void wave_append(unsigned* list, unsigned* count, bool flag, unsigned id) {
     const unsigned long long m = __ballot(flag);
@@ -925,13 +932,21 @@ __device__ __forceinline__ void wave_append(unsigned* list, unsigned* count, boo
     if (flag) list[base + __popcll(m & ((1ull << lane) - 1ull))] = id;
 }
 
+// chem_count with the tail split: active cells needing more than rt.tail sub-steps go to rt.list.
+__device__ __forceinline__ void chem_count_tail(Scalars* sc, int nch, bool active, unsigned* list, unsigned id,
+                                                const TailRoute& rt) {
+    const bool tail = active && rt.tail > 0 && nch > rt.tail;
+    chem_count(sc, nch, active && !tail, list, id);
+    if (rt.tail > 0) wave_append(rt.list, &sc->coop_count, tail, id);
+}
+
 // Prediction + first sub-step for every interior cell; cells needing more sub-steps are
 // compacted into the work list. With nthr > 0 (surrogate chemistry), a cell at or above T_cold
 // needing more than nthr sub-steps keeps its state and goes to the network list instead.
 template <bool C3>
 __global__ void __launch_bounds__(256) k_trap_first(G3 g, Fields F, uint32_t q, const Scalars* __restrict__ scc,
                                                     Scalars* sc, unsigned* list, int nmax, int predict_only, int nthr,
-                                                    double T_cold, unsigned* slist, unsigned* scount) {
+                                                    double T_cold, unsigned* slist, unsigned* scount, TailRoute rt) {
     int i, j, k;
     const bool in = interior_ijk(g, i, j, k);
     const size_t c = in ? IDX(g, i, j, k) : 0;
@@ -965,7 +980,7 @@ __global__ void __launch_bounds__(256) k_trap_first(G3 g, Fields F, uint32_t q, 
             active = nch > 1;
         }
     }
-    chem_count(sc, in && !to_net ? nch : 0, active, list, (unsigned)c);
+    chem_count_tail(sc, in && !to_net ? nch : 0, active, list, (unsigned)c, rt);
     if (nthr > 0) wave_append(slist, scount, to_net, (unsigned)c);
 }
 
@@ -1023,11 +1038,314 @@ k_chem_persist(Acc acc, const unsigned* __restrict__ list, Scalars* sc, uint32_t
     }
 }
 
+// ---------------------------------------------------------------- tail kernel
+// Cells with the longest sub-step chains run two to a wavefront, 32 lanes per cell. A sub-step's
+// independent pieces go to lanes: lane s < NS evaluates species s (reaction units, G/RT and, after
+// the reduction, its trapezoid update), lane r < NR reaction r, lane q < 3 NS one reaction sum,
+// lane k < 17 one coefficient of the mixture fit. Values cross lanes through LDS. Each piece keeps
+// the arithmetic of the per-lane path (src/Reaction.cpp:548-585, src/Flowfield.cpp:1246-1284),
+// and every sum runs in the same order, so the results agree with it to rounding (FMA
+// contraction differs). A sub-step then issues a few hundred wavefront instructions instead of
+// several thousand, which is what bounds a chain of thousands of dependent sub-steps.
+
+// Sparse stoichiometry of each reaction: the species with a non-zero forward or backward
+// coefficient, in species order, padded with species NS (a sink row that is never summed).
+constexpr int RX_TERMS = 4;
+struct RxnSparse {
+    int sp[NR][RX_TERMS], nf[NR][RX_TERMS], nb[NR][RX_TERMS], xr[NR];
+};
+constexpr int rxn_max_terms() {
+    int m = 0;
+    for (int r = 0; r < NR; r++) {
+        int c = 0;
+        for (int s = 0; s < NS; s++) c += (SF[s][r] != 0 || SB[s][r] != 0) ? 1 : 0;
+        m = c > m ? c : m;
+    }
+    return m;
+}
+static_assert(rxn_max_terms() <= RX_TERMS, "a reaction involves more species than the tail kernel holds");
+constexpr RxnSparse make_rxn_sparse() {
+    RxnSparse t{};
+    for (int r = 0; r < NR; r++) {
+        int j = 0, x = 0;
+        for (int s = 0; s < NS; s++) {
+            x += SB[s][r] - SF[s][r];
+            if (SF[s][r] != 0 || SB[s][r] != 0) {
+                t.sp[r][j] = s;
+                t.nf[r][j] = SF[s][r];
+                t.nb[r][j] = SB[s][r];
+                j++;
+            }
+        }
+        for (; j < RX_TERMS; j++) {
+            t.sp[r][j] = NS;
+            t.nf[r][j] = 0;
+            t.nb[r][j] = 0;
+        }
+        t.xr[r] = x;
+    }
+    return t;
+}
+__constant__ RxnSparse RXS = make_rxn_sparse();
+
+// LDS of one cell (one half wavefront).
+struct TailLds {
+    double mc[NS + 1];               // reaction units [mol/cm^3]; slot NS pads the sparse lists
+    double gx[NS + 1];               // G_s / (R T)
+    double part[3][NS + 1][NR];      // per-reaction terms of w, Q, P; row NS is the sink
+    double fin[3 * NS];              // w, Q, P after the sums and scaling
+    double cs[NS];                   // updated partial densities
+    double mix[2][8];                // mixture fit, low and high range: a1..a7, b1
+    double mixd[2][4];               // a4/2, a5/3, a6/4, a7/5 of each range
+    double rmix;                     // sum_s Y_s R_s
+};
+
+// Newton on the mixture energy (Flowfield::Get_temp, src/Flowfield.cpp:1542-1566) with the
+// coefficients of the enthalpy's T^2..T^5 terms pre-divided and the powers of the starting T given.
+__device__ __forceinline__ double tail_newton(const TailLds& S, TPow p, double E) {
+    const double Ek = E * 1e-3;
+    double Tn = p.T;
+    for (int count = 0; count < 10; count++) {
+        const double T0 = p.T;
+        const int r = T0 < 1000 ? 0 : 1;
+        const double* c = S.mix[r];
+        const double* d = S.mixd[r];
+        const double h = -c[0] * p.Tm1 + c[1] * p.lnT + c[2] * p.T + d[0] * p.T2 + d[1] * p.T3 + d[2] * p.T4 +
+                         d[3] * p.T5 + c[7];
+        const double cp = c[0] * p.Tm2 + c[1] * p.Tm1 + c[2] + c[3] * p.T + c[4] * p.T2 + c[5] * p.T3 + c[6] * p.T4;
+        const double f = (h - Ek) - S.rmix * T0;
+        const double fp = cp - S.rmix;
+        Tn = T0 - f / fp;
+        if (fabs(Tn - T0) < 1e-6) break;
+        p = tpow(Tn);
+    }
+    return Tn;
+}
+
+template <class Acc, bool C3>
+__global__ void __launch_bounds__(64) k_chem_tail(Acc acc, const unsigned* __restrict__ list, Scalars* sc, uint32_t q,
+                                                  const double* __restrict__ dtm_p) {
+    __shared__ TailLds L[2];
+    const int lane = threadIdx.x, hl = lane & 31;
+    TailLds& S = L[lane >> 5];
+    const bool c4 = q & Q_C4;
+    const int rtb = (q & Q_C2) ? 1 : 2;
+    // lane constants: species sS, reaction rR, sum row, mixture coefficient
+    const int sS = hl < NS ? hl : 0, rR = hl < NR ? hl : 0, kM = hl < 17 ? hl : 0;
+    double clo[9], chi[9];   // a1..a7 b1 b2 of species sS, both ranges (C4: low b1, b2 throughout)
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        clo[k] = C0[k][sS];
+        chi[k] = (k >= 7 && c4) ? C0[k][sS] : C1[k][sS];
+    }
+    // reaction rR: k_f = A T^b exp(-E/(Ru T)) with b in {1, -1, -2} as a product (src/Reaction.cpp:522-530)
+    const double af = AF[rR], bf = BF[rR], eaf = EAF[rR];
+    const int tmode = bf == 1.0 ? 1 : bf == -1.0 ? 2 : bf == -2.0 ? 3 : 0;
+    const bool b_exp = tmode == 0 && bf != 0.0;
+    const bool use_exp = b_exp || eaf != 0.0;
+    double tbr[NS], cf[NS];
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        tbr[s] = TB[s][rR];
+        cf[s] = kM < 8 ? C0[kM][s] : kM < 16 ? ((kM == 15 && c4) ? C0[7][s] : C1[kM - 8][s]) : 1.0;
+    }
+    int sp[RX_TERMS], nfj[RX_TERMS], nbj[RX_TERMS];
+#pragma unroll
+    for (int j = 0; j < RX_TERMS; j++) {
+        sp[j] = RXS.sp[rR][j];
+        nfj[j] = RXS.nf[rR][j];
+        nbj[j] = RXS.nb[rR][j];
+    }
+    const int xr = RXS.xr[rR];
+    const int qt = hl / NS, qs = hl - (hl / NS) * NS;   // sum lane: quantity, species
+    const double mwq = MW[qs];
+    const double rmw = RMW(sS);
+    // mixture lanes k = 3..6 and 11..14 also give a_k / (k - 1) (T^2..T^5 terms of h)
+    const int kd = kM & 7;
+    const bool has_div = kM < 16 && kd >= 3 && kd <= 6;
+    const double kdiv = (double)(kd - 1);
+    for (int i = hl; i < 3 * (NS + 1) * NR; i += 32) (&S.part[0][0][0])[i] = 0.0;
+    if (hl == 0) {
+        S.mc[NS] = 1.0;
+        S.gx[NS] = 0.0;
+    }
+    __syncthreads();
+
+    const unsigned cnt = sc->coop_count;
+    const double dtm = *dtm_p;
+    bool have = false, drained = false;
+    int rem = 0, nch = 1;
+    unsigned id = 0;
+    double T = 0, E = 0, U = 0, V = 0, W = 0, di = 0, dt = 0;
+    while (true) {
+        const bool need = !have && !drained;   // the same on the 32 lanes of a cell
+        unsigned long long slot = 0;
+        if (need && hl == 0) slot = atomicAdd(&sc->coop_head, 1ull);
+        slot = __shfl(slot, lane & 32);
+        if (need) {
+            if (slot < cnt) {
+                id = list[slot];
+                ChemCell c;
+                acc.load(id, c, nch);
+#pragma unroll
+                for (int s = 0; s < NS; s++)
+                    if (s == sS) di = c.Di[s];
+                T = c.T;
+                E = c.E;
+                U = c.U;
+                V = c.V;
+                W = c.W;
+                dt = dtm / nch;
+                rem = nch - 1;   // the first sub-step ran in the compaction pass
+                have = rem > 0;
+            } else {
+                drained = true;
+            }
+        }
+        if (__ballot(have) == 0) {
+            if (__ballot(!drained) == 0) break;
+            continue;
+        }
+        if (!have) continue;
+        const TPow p = tpow(T);
+        const bool lo = T < 1000;
+        // species lanes: reaction units and G/RT (Reaction::GetHi/GetSi, src/Reaction.cpp:906-948)
+        double Dr = 0.0;
+        if (hl < NS) {
+            S.mc[hl] = (di * rmw * 1000) * 1e-6;
+            Dr = di * 1e-3;
+            double a[9];
+#pragma unroll
+            for (int k = 0; k < 9; k++) a[k] = lo ? clo[k] : chi[k];
+            const double h = -a[0] * p.Tm1 + a[1] * p.lnT + a[2] * p.T + a[3] * p.T2 / 2 + a[4] * p.T3 / 3 +
+                             a[5] * p.T4 / 4 + a[6] * p.T5 / 5 + a[7];
+            const double sr = -a[0] * p.Tm2 / 2 - a[1] * p.Tm1 + a[2] * p.lnT + a[3] * p.T + a[4] * p.T2 / 2 +
+                              a[5] * p.T3 / 3 + a[6] * p.T4 / 4 + a[8];
+            S.gx[hl] = h * p.Tm1 - sr;
+        }
+        __syncthreads();
+        // reaction lanes: rates and the per-species terms (src/Reaction.cpp:548-585)
+        if (hl < NR) {
+            const double RT = R_UNIV * T;
+            constexpr double P0R = P0_PA * 1e-6;
+            const double rRuT = 1.0 / (RU_CAL * T);
+            const double tfac = tmode == 1 ? T : tmode == 2 ? p.Tm1 : tmode == 3 ? p.Tm2 : 1.0;
+            const double ex = (b_exp ? bf * p.lnT : 0.0) - eaf * rRuT;
+            const double ekf = exp(ex);
+            double kf = af * tfac;
+            if (use_exp) kf *= ekf;
+            double mc[NS];
+#pragma unroll
+            for (int s = 0; s < NS; s++) mc[s] = S.mc[s];
+            double tb = 0.0;
+#pragma unroll
+            for (int s = 0; s < NS; s++) tb += mc[s] * tbr[s];
+            double xk = 0.0, rf = 1.0, rb = 1.0;
+#pragma unroll
+            for (int j = 0; j < RX_TERMS; j++) {
+                const double g = S.gx[sp[j]], m = S.mc[sp[j]];
+                const int dn = nbj[j] - nfj[j];
+                xk += dn != 0 ? (double)dn * g : 0.0;
+                rf *= nfj[j] == 0 ? 1.0 : nfj[j] == 1 ? m : m * m;
+                rb *= nbj[j] == 0 ? 1.0 : nbj[j] == 1 ? m : m * m;
+            }
+            double kb = kf * exp(xk);
+            kb *= xr == 1 ? RT / P0R : xr == -1 ? P0R / RT : 1.0;
+            if (rtb == 1) tb = 0.0;
+            else if (rtb == 2 && tb == 0.0) tb = 1.0;
+            const double rr = kf * rf - kb * rb;
+#pragma unroll
+            for (int j = 0; j < RX_TERMS; j++) {
+                const int nf = nfj[j], nb = nbj[j], dn = nb - nf, s = sp[j];
+                const double qf = nb ? (double)nb * kf * rf : 0.0;
+                const double qb = nf ? (double)nf * kb * rb : 0.0;
+                const double pb = nb ? (double)nb * kb * rb : 0.0;
+                const double pf = nf ? (double)nf * kf * rf : 0.0;
+                S.part[0][s][hl] = dn != 0 ? (double)dn * tb * rr : 0.0;
+                if (C3) {
+                    S.part[1][s][hl] = qf + qb * tb;
+                    S.part[2][s][hl] = pb + pf * tb;
+                } else {
+                    S.part[1][s][hl] = (qf + qb) * tb;
+                    S.part[2][s][hl] = (pb + pf) * tb;
+                }
+            }
+        }
+        __syncthreads();
+        // sum lanes: w, Q, P of one species over the reactions, in reaction order
+        if (hl < 3 * NS) {
+            const double* row = &S.part[qt][qs][0];
+            double a = 0.0;
+#pragma unroll
+            for (int r = 0; r < NR; r++) a += row[r];
+            if (qt < 2) a *= mwq;
+            else {
+                const double m = S.mc[qs];
+                a = (m == 0) ? 0.0 : a / m;
+            }
+            S.fin[hl] = a;
+        }
+        __syncthreads();
+        // species lanes: point-implicit trapezoid update (src/Reaction.cpp:481,587)
+        if (hl < NS) {
+            const double Qs = S.fin[NS + hl], Ps = S.fin[2 * NS + hl];
+            di = ((1.0 - dt / 2.0 * Ps) * Dr + dt * Qs) / (1.0 + dt / 2.0 * Ps) * 1e3;
+            S.cs[hl] = di;
+        }
+        __syncthreads();
+        // every lane: the conserved state (Flowfield::Explicit for the cell)
+        double cs[NC];
+        double D = 0.0;
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+            cs[s] = S.cs[s];
+            D += cs[s];
+        }
+        cs[NS] = U * D;
+        cs[NS + 1] = V * D;
+        cs[NS + 2] = W * D;
+        cs[NS + 3] = D * E + 0.5 * (U * U + V * V + W * W) * D;
+        if (--rem == 0) {
+            // last sub-step: the full primitive state, stored as the persistent kernel stores it
+            const Prim o = prim_from_cs(cs, T, E, q);
+            if (hl == 0) {
+                KinOut ko{};
+#pragma unroll
+                for (int s = 0; s < NS; s++) ko.w[s] = S.fin[s];
+                acc.store(id, cs, o, ko);
+            }
+            have = false;
+            continue;
+        }
+        // Update_after_CS with the mixture fit summed one coefficient per lane
+        const double rD = 1.0 / D;
+        double Y[NS];
+#pragma unroll
+        for (int s = 0; s < NS; s++) Y[s] = cs[s] * rD;
+        const double Un = cs[NS] * rD, Vn = cs[NS + 1] * rD, Wn = cs[NS + 2] * rD;
+        const double En = (cs[NS + 3] - 0.5 * D * (Un * Un + Vn * Vn + Wn * Wn)) * rD;
+        if (hl < 17) {
+            double a = 0.0;
+#pragma unroll
+            for (int s = 0; s < NS; s++) a += (Y[s] * RI(s)) * cf[s];
+            if (hl < 16) S.mix[hl >> 3][kd] = a;
+            else S.rmix = a;
+            if (has_div) S.mixd[hl >> 3][kd - 3] = a / kdiv;
+        }
+        __syncthreads();
+        T = tail_newton(S, p, (q & Q_F3) ? E : En);
+        E = En;
+        U = Un;
+        V = Vn;
+        W = Wn;
+    }
+}
+
 // Batch: prediction + first sub-step on flat cells at rest.
 template <bool C3>
 __global__ void __launch_bounds__(256) k_batch_first(BatchCells acc, int* nchem_out, int* nchem_keep, uint32_t q,
                                                      double dtm, int nmax, Scalars* sc, unsigned* list, int nthr,
-                                                     double T_cold, unsigned* slist, unsigned* scount) {
+                                                     double T_cold, unsigned* slist, unsigned* scount, TailRoute rt) {
     const long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     const bool in = id < acc.n;
     int nch = 0;
@@ -1059,7 +1377,7 @@ __global__ void __launch_bounds__(256) k_batch_first(BatchCells acc, int* nchem_
             active = nch > 1;
         }
     }
-    chem_count(sc, in && !to_net ? nch : 0, active, list, (unsigned)id);
+    chem_count_tail(sc, in && !to_net ? nch : 0, active, list, (unsigned)id, rt);
     if (nthr > 0) wave_append(slist, scount, to_net, (unsigned)id);
 }
 
@@ -1068,7 +1386,7 @@ __global__ void __launch_bounds__(256) k_batch_first(BatchCells acc, int* nchem_
 template <bool C3>
 __global__ void __launch_bounds__(256) k_trap_first_list(G3 g, Fields F, uint32_t q, const unsigned* __restrict__ in_list,
                                                          const unsigned* __restrict__ in_count, Scalars* sc,
-                                                         unsigned* out_list, int nmax) {
+                                                         unsigned* out_list, int nmax, TailRoute rt) {
     // grid-stride over the listed cells (the count is known only on the device)
     const unsigned cnt = *in_count;
     for (unsigned b0 = blockIdx.x * blockDim.x; b0 < cnt; b0 += gridDim.x * blockDim.x) {
@@ -1104,7 +1422,7 @@ __global__ void __launch_bounds__(256) k_trap_first_list(G3 g, Fields F, uint32_
                 active = nch > 1;
             }
         }
-        chem_count(sc, in ? nch : 0, active, out_list, (unsigned)c);
+        chem_count_tail(sc, in ? nch : 0, active, out_list, (unsigned)c, rt);
     }
 }
 
@@ -1112,7 +1430,7 @@ template <bool C3>
 __global__ void __launch_bounds__(256) k_batch_first_list(BatchCells acc, int* nchem_keep, uint32_t q, double dtm, int nmax,
                                                           const unsigned* __restrict__ in_list,
                                                           const unsigned* __restrict__ in_count, Scalars* sc,
-                                                          unsigned* out_list) {
+                                                          unsigned* out_list, TailRoute rt) {
     // grid-stride over the listed cells (the count is known only on the device)
     const unsigned cnt = *in_count;
     for (unsigned b0 = blockIdx.x * blockDim.x; b0 < cnt; b0 += gridDim.x * blockDim.x) {
@@ -1146,11 +1464,13 @@ __global__ void __launch_bounds__(256) k_batch_first_list(BatchCells acc, int* n
                 active = nch > 1;
             }
         }
-        chem_count(sc, in ? nch : 0, active, out_list, (unsigned)id);
+        chem_count_tail(sc, in ? nch : 0, active, out_list, (unsigned)id, rt);
     }
 }
 
 __global__ void k_chem_reset(Scalars* sc, double* dtm_slot, double dtm, int use_dt) {
+    sc->coop_count = 0;
+    sc->coop_head = 0;
     sc->chem_head = 0;
     sc->chem_active = 0;
     sc->chem_substeps = 0;
@@ -1243,6 +1563,26 @@ hipError_t launch_advection(eu3d_ctx* c, int diff, int limiter) {
 static int n_interior_blocks(const G3& g) {
     const dim3 b = interior_grid(g);
     return (int)(b.x * b.y * b.z);
+}
+
+// The tail kernel after the persistent kernel: one wavefront per queued cell, grid-stride over the
+// device count.
+template <class Acc>
+void launch_coop(eu3d_ctx* c, const Acc& acc, const double* dtm_slot) {
+    if (c->sched.tail <= 0) return;
+    const unsigned nb = (unsigned)c->num_cu * 16;
+    if (c->quirks & Q_C3) k_chem_tail<Acc, true><<<nb, 64, 0, c->stream>>>(acc, c->coop_list, c->sc, c->quirks, dtm_slot);
+    else k_chem_tail<Acc, false><<<nb, 64, 0, c->stream>>>(acc, c->coop_list, c->sc, c->quirks, dtm_slot);
+}
+
+hipError_t ensure_coop(eu3d_ctx* c, long long n) {
+    if (c->coop_cap >= n) return hipSuccess;
+    if (c->coop_list) (void)hipFree(c->coop_list);
+    c->coop_list = nullptr;
+    c->coop_cap = 0;
+    hipError_t e = hipMalloc(&c->coop_list, (size_t)n * sizeof(unsigned));
+    if (e == hipSuccess) c->coop_cap = n;
+    return e;
 }
 
 hipError_t launch_cfl(eu3d_ctx* c, double cfl, double final_time) {
@@ -1356,19 +1696,23 @@ hipError_t launch_trap_adaptive(eu3d_ctx* c, int nchem_max, bool predict_only, i
                                 unsigned* slist, unsigned* scount) {
     const int nmax = nchem_max > 0 ? nchem_max : 10000;
     double* slot = &c->sc->dtm;
+    hipError_t e = ensure_coop(c, (long long)c->g.ni * c->g.nj * c->g.nk);
+    if (e != hipSuccess) return e;
+    const TailRoute rt{c->coop_list, predict_only ? 0 : c->sched.tail};
     k_chem_reset<<<1, 1, 0, c->stream>>>(c->sc, slot, 0.0, 1);
     const bool c3 = c->quirks & Q_C3;
     if (c3)
         k_trap_first<true><<<interior_grid(c->g), dim3(BX, BY), 0, c->stream>>>(G(c), c->fl, c->quirks, c->sc, c->sc,
                                                                            c->active, nmax, predict_only ? 1 : 0, nthr,
-                                                                           T_cold, slist, scount);
+                                                                           T_cold, slist, scount, rt);
     else
         k_trap_first<false><<<interior_grid(c->g), dim3(BX, BY), 0, c->stream>>>(G(c), c->fl, c->quirks, c->sc, c->sc,
                                                                             c->active, nmax, predict_only ? 1 : 0, nthr,
-                                                                            T_cold, slist, scount);
+                                                                            T_cold, slist, scount, rt);
     if (!predict_only) {
         GridCells acc{c->g, c->fl};
         launch_persist(c, acc, c->active, slot);
+        launch_coop(c, acc, slot);
     }
     return hipGetLastError();
 }
@@ -1387,17 +1731,21 @@ hipError_t launch_chem_batch(eu3d_ctx* c, long long n, double* rho, double* T, d
     unsigned* list = c->batch_active;
     int* keep = reinterpret_cast<int*>(list + c->batch_cap);
     double* slot = &c->sc->dtm;
+    hipError_t e = ensure_coop(c, n);
+    if (e != hipSuccess) return e;
+    const TailRoute rt{c->coop_list, c->sched.tail};
     k_chem_reset<<<1, 1, 0, c->stream>>>(c->sc, slot, dtm, 0);
     BatchCells acc{n, rho, T, E, keep};
     const unsigned gb = (unsigned)((n + 255) / 256);
     if (c->quirks & Q_C3) {
         k_batch_first<true><<<gb, 256, 0, c->stream>>>(acc, nchem_out, keep, c->quirks, dtm, nmax, c->sc, list, nthr,
-                                                       T_cold, slist, scount);
+                                                       T_cold, slist, scount, rt);
     } else {
         k_batch_first<false><<<gb, 256, 0, c->stream>>>(acc, nchem_out, keep, c->quirks, dtm, nmax, c->sc, list, nthr,
-                                                        T_cold, slist, scount);
+                                                        T_cold, slist, scount, rt);
     }
     launch_persist(c, acc, list, slot);
+    launch_coop(c, acc, slot);
     return hipGetLastError();
 }
 
@@ -1406,17 +1754,21 @@ hipError_t launch_chem_batch(eu3d_ctx* c, long long n, double* rho, double* T, d
 hipError_t launch_trap_list(eu3d_ctx* c, const unsigned* list, const unsigned* count, unsigned max_count,
                             unsigned* work, int nchem_max) {
     const int nmax = nchem_max > 0 ? nchem_max : 10000;
+    hipError_t e = ensure_coop(c, max_count);
+    if (e != hipSuccess) return e;
+    const TailRoute rt{c->coop_list, c->sched.tail};
     k_chem_reset<<<1, 1, 0, c->stream>>>(c->sc, &c->sc->dtm, 0.0, 1);
     const unsigned nb = std::min<unsigned>((max_count + 255) / 256, (unsigned)c->num_cu * 8);
     const bool c3 = c->quirks & Q_C3;
     if (nb) {
         if (c3)
-            k_trap_first_list<true><<<nb, 256, 0, c->stream>>>(G(c), c->fl, c->quirks, list, count, c->sc, work, nmax);
+            k_trap_first_list<true><<<nb, 256, 0, c->stream>>>(G(c), c->fl, c->quirks, list, count, c->sc, work, nmax, rt);
         else
-            k_trap_first_list<false><<<nb, 256, 0, c->stream>>>(G(c), c->fl, c->quirks, list, count, c->sc, work, nmax);
+            k_trap_first_list<false><<<nb, 256, 0, c->stream>>>(G(c), c->fl, c->quirks, list, count, c->sc, work, nmax, rt);
     }
     GridCells acc{c->g, c->fl};
     launch_persist(c, acc, work, &c->sc->dtm);
+    launch_coop(c, acc, &c->sc->dtm);
     return hipGetLastError();
 }
 
@@ -1425,17 +1777,23 @@ hipError_t launch_batch_list(eu3d_ctx* c, long long n, double* rho, double* T, d
                              const unsigned* list, const unsigned* count, unsigned max_count, unsigned* work,
                              int* nchem_keep, int nchem_max) {
     const int nmax = nchem_max > 0 ? nchem_max : 10000;
+    hipError_t e = ensure_coop(c, max_count);
+    if (e != hipSuccess) return e;
+    const TailRoute rt{c->coop_list, c->sched.tail};
     k_chem_reset<<<1, 1, 0, c->stream>>>(c->sc, &c->sc->dtm, dtm, 0);
     BatchCells acc{n, rho, T, E, nchem_keep};
     const unsigned nb = std::min<unsigned>((max_count + 255) / 256, (unsigned)c->num_cu * 8);
     const bool c3 = c->quirks & Q_C3;
     if (nb) {
         if (c3)
-            k_batch_first_list<true><<<nb, 256, 0, c->stream>>>(acc, nchem_keep, c->quirks, dtm, nmax, list, count, c->sc, work);
+            k_batch_first_list<true><<<nb, 256, 0, c->stream>>>(acc, nchem_keep, c->quirks, dtm, nmax, list, count, c->sc,
+                                                                 work, rt);
         else
-            k_batch_first_list<false><<<nb, 256, 0, c->stream>>>(acc, nchem_keep, c->quirks, dtm, nmax, list, count, c->sc, work);
+            k_batch_first_list<false><<<nb, 256, 0, c->stream>>>(acc, nchem_keep, c->quirks, dtm, nmax, list, count, c->sc,
+                                                                  work, rt);
     }
     launch_persist(c, acc, work, &c->sc->dtm);
+    launch_coop(c, acc, &c->sc->dtm);
     return hipGetLastError();
 }
 

@@ -190,6 +190,9 @@ typedef struct {
     int blocks_per_cu;  /* persistent grid = blocks_per_cu * CU count */
     int waves;          /* occupancy target of the persistent kernel's register allocation:
                            1, 2 or 4 waves per SIMD (other values select 2) */
+    int tail;           /* cells needing more sub-steps than this run one cell per wavefront
+                           (reactions across the lanes); 0 keeps every cell in the persistent
+                           kernel */
 } eu3d_chem_schedule;
 int eu3d_chem_batch(eu3d_ctx *ctx, long long n, double *rho_s, double *T, double *E, double dtm,
                     int nchem_max, int *nchem_out, eu3d_chem_stats *stats);
