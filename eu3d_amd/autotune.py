@@ -2,16 +2,21 @@
 
 The persistent kernel takes cells from a compacted work queue, and its cost per cell varies with
 the sub-step count of that cell. The best launch therefore depends on the load: how many cells are
-active and how uneven their sub-step counts are. The launch is set by three parameters:
+active and how uneven their sub-step counts are. The schedule has four parameters:
 - threads per block;
 - blocks per CU (with the block size, this sets how many lanes pull from the queue);
 - the occupancy target of the register allocation (waves per SIMD: 1 keeps every value in
-  registers, 2 and 4 trade spills for latency hiding).
+  registers, 2 and 4 trade spills for latency hiding);
+- the tail threshold: cells needing more sub-steps than this run in the tail kernel, two cells
+  per wavefront with each sub-step spread over the lanes (0 keeps every cell in the persistent
+  kernel). Long chains of dependent sub-steps bound the batch's time, and the tail kernel
+  shortens each sub-step's latency at the cost of throughput.
 
-`tune` times every candidate on the caller's own batch with HIP events on the context stream,
-restoring the inputs between runs, and returns the fastest. Results are cached in a JSON file,
-keyed by the GPU (architecture and CU count), the mechanism, the quirk set and a signature of
-the load histogram.
+`tune` times the candidates on the caller's own batch with HIP events on the context stream,
+restoring the inputs between runs. It first picks the tail threshold at the default launch, then
+the launch at that threshold, and returns the fastest. Results are cached in a JSON file, keyed by
+the GPU (architecture and CU count), the mechanism, the quirk set and a signature of the load
+histogram.
 """
 from __future__ import annotations
 
@@ -29,6 +34,8 @@ from .mech import load_default
 BLOCKS = (64, 128, 256)
 BLOCKS_PER_CU = (1, 2, 4, 8, 16)
 WAVES = (1, 2, 4)
+TAILS = (0, 128, 256, 512, 1024, 2048)
+DEFAULT_LAUNCH = (256, 4, 2)
 DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "eu3d_amd", "chem_schedule.json")
 
 
@@ -84,27 +91,38 @@ def _time_batch(ctx: Context, inputs, dtm, nchem_max, reps):
     return best
 
 
-def tune(ctx: Context, inputs, dtm: float, nchem_max: int = 10000, reps: int = 2, candidates=None, log=None):
-    """Time every schedule on the batch `inputs` = (rho [ns][n], T, E) device tensors (left
-    unchanged). Returns (best schedule dict, table of {schedule: ms})."""
+def tune(ctx: Context, inputs, dtm: float, nchem_max: int = 10000, reps: int = 2, candidates=None, tails=None,
+         log=None):
+    """Time the schedules on the batch `inputs` = (rho [ns][n], T, E) device tensors (left
+    unchanged): the tail thresholds `tails` at the default launch, then every launch in
+    `candidates` (block, blocks per CU, waves) at the best threshold. Returns (best schedule dict,
+    table of {schedule: ms})."""
     table = {}
-    cands = candidates or [(b, k, w) for w in WAVES for b in BLOCKS for k in BLOCKS_PER_CU]
-    for b, k, w in cands:
-        ctx.set_schedule(b, k, w)
+
+    def run(b, k, w, tail):
+        ctx.set_schedule(b, k, w, tail)
         ms = _time_batch(ctx, inputs, dtm, nchem_max, reps)
-        table[f"{b}x{k}/w{w}"] = ms
+        table[f"{b}x{k}/w{w}/t{tail}"] = ms
         if log:
-            log(f"  schedule block {b} blocks/CU {k} waves {w}: {ms:.3f} ms")
+            log(f"  schedule block {b} blocks/CU {k} waves {w} tail {tail}: {ms:.3f} ms")
+        return ms
+
+    b0, k0, w0 = DEFAULT_LAUNCH
+    best_tail = min(tails or TAILS, key=lambda t: run(b0, k0, w0, t))
+    for b, k, w in candidates or [(b, k, w) for w in WAVES for b in BLOCKS for k in BLOCKS_PER_CU]:
+        if f"{b}x{k}/w{w}/t{best_tail}" not in table:
+            run(b, k, w, best_tail)
     key = min(table, key=table.get)
     b, rest = key.split("x")
-    k, w = rest.split("/w")
-    best = {"block": int(b), "blocks_per_cu": int(k), "waves": int(w), "ms": table[key]}
-    ctx.set_schedule(best["block"], best["blocks_per_cu"], best["waves"])
+    k, rest = rest.split("/w")
+    w, t = rest.split("/t")
+    best = {"block": int(b), "blocks_per_cu": int(k), "waves": int(w), "tail": int(t), "ms": table[key]}
+    ctx.set_schedule(best["block"], best["blocks_per_cu"], best["waves"], best["tail"])
     return best, table
 
 
 def cache_key(ctx: Context, nchem: np.ndarray) -> str:
-    return f"{device_key()}|{mech_key()}|q{ctx.quirks}|{load_signature(nchem)}"
+    return f"{device_key()}|{mech_key()}|q{ctx.quirks}|{load_signature(nchem)}|tail"
 
 
 def load_or_tune(ctx: Context, inputs, dtm: float, nchem_max: int = 10000, cache: str | None = None, log=None):
@@ -124,12 +142,12 @@ def load_or_tune(ctx: Context, inputs, dtm: float, nchem_max: int = 10000, cache
             db = json.load(f)
     if key in db:
         s = db[key]
-        ctx.set_schedule(s["block"], s["blocks_per_cu"], s["waves"])
+        ctx.set_schedule(s["block"], s["blocks_per_cu"], s["waves"], s.get("tail", 0))
         return dict(s, cached=True, key=key)
     t_start = time.time()
     best, table = tune(ctx, inputs, dtm, nchem_max, log=log)
     best.update(key=key, tuned_s=time.time() - t_start, candidates=len(table))
-    db[key] = {k: best[k] for k in ("block", "blocks_per_cu", "waves", "ms")}
+    db[key] = {k: best[k] for k in ("block", "blocks_per_cu", "waves", "tail", "ms")}
     os.makedirs(os.path.dirname(cache), exist_ok=True)
     with open(cache, "w") as f:
         json.dump(db, f, indent=1)

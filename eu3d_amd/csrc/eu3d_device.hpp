@@ -56,6 +56,20 @@ __device__ __forceinline__ double cp_R(int s, bool lo, const TPow& p) {
     const double a6 = lo ? C0[6][s] : C1[6][s];
     return a0 * p.Tm2 + a1 * p.Tm1 + a2 + a3 * p.T + a4 * p.T2 + a5 * p.T3 + a6 * p.T4;
 }
+
+// x / D, correctly rounded, for the constant divisors D = 3 and 5 of the h/R and s/R integrals:
+// the product with RN(1/D) refined by one FMA residual step (Markstein's correction). It equals
+// IEEE division on every finite, normal operand sampled by the oracle's check (6e8 operands
+// across the exponent range, both divisors) at three instructions instead of a full division.
+template <int D>
+__device__ __forceinline__ double div_const(double x) {
+    static_assert(D == 3 || D == 5, "checked for the divisors 3 and 5 only");
+    constexpr double R = 1.0 / D;
+    const double q = x * R;
+    const double r = __builtin_fma(-q, (double)D, x);
+    return __builtin_fma(r, R, q);
+}
+
 // h/R (Reaction::GetHi, src/Reaction.cpp:916-926); quirk C4 keeps the low-range b1.
 __device__ __forceinline__ double h_R(int s, bool lo, bool c4, const TPow& p) {
     const double a0 = lo ? C0[0][s] : C1[0][s], a1 = lo ? C0[1][s] : C1[1][s];
@@ -63,8 +77,8 @@ __device__ __forceinline__ double h_R(int s, bool lo, bool c4, const TPow& p) {
     const double a4 = lo ? C0[4][s] : C1[4][s], a5 = lo ? C0[5][s] : C1[5][s];
     const double a6 = lo ? C0[6][s] : C1[6][s];
     const double b1 = (lo || c4) ? C0[7][s] : C1[7][s];
-    return -a0 * p.Tm1 + a1 * p.lnT + a2 * p.T + a3 * p.T2 / 2 + a4 * p.T3 / 3 + a5 * p.T4 / 4 +
-           a6 * p.T5 / 5 + b1;
+    return -a0 * p.Tm1 + a1 * p.lnT + a2 * p.T + a3 * p.T2 / 2 + div_const<3>(a4 * p.T3) + a5 * p.T4 / 4 +
+           div_const<5>(a6 * p.T5) + b1;
 }
 // s/R (Reaction::GetSi, src/Reaction.cpp:938-948); quirk C4 keeps the low-range b2.
 __device__ __forceinline__ double s_R(int s, bool lo, bool c4, const TPow& p) {
@@ -73,8 +87,28 @@ __device__ __forceinline__ double s_R(int s, bool lo, bool c4, const TPow& p) {
     const double a4 = lo ? C0[4][s] : C1[4][s], a5 = lo ? C0[5][s] : C1[5][s];
     const double a6 = lo ? C0[6][s] : C1[6][s];
     const double b2 = (lo || c4) ? C0[8][s] : C1[8][s];
-    return -a0 * p.Tm2 / 2 - a1 * p.Tm1 + a2 * p.lnT + a3 * p.T + a4 * p.T2 / 2 + a5 * p.T3 / 3 +
+    return -a0 * p.Tm2 / 2 - a1 * p.Tm1 + a2 * p.lnT + a3 * p.T + a4 * p.T2 / 2 + div_const<3>(a5 * p.T3) +
            a6 * p.T4 / 4 + b2;
+}
+
+// G_s / (R T) of every species at p.T, and h_s / R when H. The NASA range is taken per wavefront:
+// each range runs only if some lane needs it, with that range's coefficients as compile-time
+// constants, so a wavefront whose lanes agree (the usual case) selects no coefficients per lane.
+template <bool H>
+__device__ __forceinline__ void species_thermo(const TPow& p, bool c4, double (&gx)[NS], double (&hr)[NS]) {
+    const bool lo = p.T < 1000;
+#pragma unroll
+    for (int pass = 0; pass < 2; pass++) {
+        const bool range = pass == 0;
+        if (__ballot(lo == range) != 0 && lo == range) {
+#pragma unroll
+            for (int s = 0; s < NS; s++) {
+                const double h = h_R(s, range, c4, p);
+                gx[s] = h * p.Tm1 - s_R(s, range, c4, p);
+                if (H) hr[s] = h;
+            }
+        }
+    }
 }
 
 // Mixture fit: sum_s w_s * (fit of s), for both ranges, w_s = Y_s * Ri_s [kJ/kg/K].
@@ -113,8 +147,8 @@ __device__ __forceinline__ double newton_T(const MixFit& m, double T, double E, 
         const TPow p = tpow(T0);
         const bool lo = T0 < 1000;
         const double* c = lo ? m.lo : m.hi;
-        const double h = -c[0] * p.Tm1 + c[1] * p.lnT + c[2] * p.T + c[3] * p.T2 / 2 + c[4] * p.T3 / 3 +
-                         c[5] * p.T4 / 4 + c[6] * p.T5 / 5 + c[7];
+        const double h = -c[0] * p.Tm1 + c[1] * p.lnT + c[2] * p.T + c[3] * p.T2 / 2 + div_const<3>(c[4] * p.T3) +
+                         c[5] * p.T4 / 4 + div_const<5>(c[6] * p.T5) + c[7];
         const double cp = c[0] * p.Tm2 + c[1] * p.Tm1 + c[2] + c[3] * p.T + c[4] * p.T2 + c[5] * p.T3 +
                           c[6] * p.T4;
         const double f = (h - Ek) - m.rmix * T0;
@@ -172,18 +206,23 @@ __device__ __forceinline__ Prim prim_from_cs(const double (&cs)[NC], double T_ol
 }
 
 // dT/drho_s (Flowfield::GetPartial_T, src/Flowfield.cpp:1578-1595); quirk C7 keeps h_s in kJ/kg.
-__device__ __forceinline__ void partial_T(const Prim& o, uint32_t q, double (&pt)[NS]) {
-    const TPow p = tpow(o.T);
-    const bool lo = o.T < 1000;
+// hr: h_s / R at o.T of every species.
+__device__ __forceinline__ void partial_T_h(const Prim& o, uint32_t q, const double (&hr)[NS], double (&pt)[NS]) {
     const double CV = o.Cp - o.Rgas;
     const double ke = 0.5 * (o.U * o.U + o.V * o.V + o.W * o.W);
     const double rden = 1.0 / (CV * o.D);
     const double hs = (q & Q_C7) ? 1.0 : 1e3;
 #pragma unroll
     for (int s = 0; s < NS; s++) {
-        const double hi = RI(s) * h_R(s, lo, q & Q_C4, p) * hs;
+        const double hi = RI(s) * hr[s] * hs;
         pt[s] = (ke - hi + RI(s) * 1e3 * o.T) * rden;
     }
+}
+__device__ __forceinline__ void partial_T(const Prim& o, uint32_t q, double (&pt)[NS]) {
+    const TPow p = tpow(o.T);
+    double gx[NS], hr[NS];
+    species_thermo<true>(p, q & Q_C4, gx, hr);
+    partial_T_h(o, q, hr, pt);
 }
 
 // ------------------------------------------------------------------ reconstruction
@@ -372,15 +411,13 @@ struct KinOut {
     double w[NS], P[NS], Q[NS], md[NS];
 };
 
+// gx: G_s / (R T) at p.T (species_thermo).
 template <bool TRAP, bool DIAG, bool C3 = false>
-__device__ __forceinline__ void kinetics(double T, const double (&Mc)[NS], const double (&Dr)[NS],
-                                         const double (&pts)[NS], int rtb, bool c4, KinOut& o) {
-    const TPow p = tpow(T);
-    const bool lo = T < 1000;
-    double gx[NS];   // G_s / (R T)
+__device__ __forceinline__ void kinetics_g(const TPow& p, const double (&gx)[NS], const double (&Mc)[NS],
+                                           const double (&Dr)[NS], const double (&pts)[NS], int rtb, KinOut& o) {
+    const double T = p.T;
 #pragma unroll
     for (int s = 0; s < NS; s++) {
-        gx[s] = h_R(s, lo, c4, p) * p.Tm1 - s_R(s, lo, c4, p);
         o.w[s] = 0.0;
         if (TRAP) { o.P[s] = 0.0; o.Q[s] = 0.0; }
         if (DIAG) o.md[s] = 0.0;
@@ -466,6 +503,15 @@ __device__ __forceinline__ void kinetics(double T, const double (&Mc)[NS], const
         }
         if (DIAG) o.md[s] *= MW[s];
     }
+}
+
+template <bool TRAP, bool DIAG, bool C3 = false>
+__device__ __forceinline__ void kinetics(double T, const double (&Mc)[NS], const double (&Dr)[NS],
+                                         const double (&pts)[NS], int rtb, bool c4, KinOut& o) {
+    const TPow p = tpow(T);
+    double gx[NS], hr[NS];
+    species_thermo<false>(p, c4, gx, hr);
+    kinetics_g<TRAP, DIAG, C3>(p, gx, Mc, Dr, pts, rtb, o);
 }
 
 // Point-implicit trapezoid update (src/Reaction.cpp:481,587): Di' [kg/m^3] from Di [g/cm^3].

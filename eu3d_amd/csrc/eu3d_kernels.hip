@@ -606,12 +606,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
             }
             o.Rgas = R_UNIV * 1000 * sy;
             o.Cp = mix_cp(mix_fit(Y, q & Q_C4), T0) * 1000;
-            double pt[NS];
-            partial_T(o, q, pt);
+            const TPow p = tpow(T0);
+            double gx[NS], hr[NS], pt[NS];
+            species_thermo<true>(p, q & Q_C4, gx, hr);
+            partial_T_h(o, q, hr, pt);
             const double tsc = (q & Q_C6) ? 1.0 : 1e3;
 #pragma unroll
             for (int s = 0; s < NS; s++) pt[s] *= tsc;
-            kinetics<false, true>(T0, Mc, Dr, pt, 2, q & Q_C4, ko);
+            kinetics_g<false, true>(p, gx, Mc, Dr, pt, 2, ko);
         }
         __builtin_amdgcn_sched_barrier(0);
         // the conserved state is read again here rather than held in registers through the
@@ -1096,12 +1098,11 @@ struct TailLds {
     double fin[3 * NS];              // w, Q, P after the sums and scaling
     double cs[NS];                   // updated partial densities
     double mix[2][8];                // mixture fit, low and high range: a1..a7, b1
-    double mixd[2][4];               // a4/2, a5/3, a6/4, a7/5 of each range
     double rmix;                     // sum_s Y_s R_s
 };
 
-// Newton on the mixture energy (Flowfield::Get_temp, src/Flowfield.cpp:1542-1566) with the
-// coefficients of the enthalpy's T^2..T^5 terms pre-divided and the powers of the starting T given.
+// Newton on the mixture energy (Flowfield::Get_temp, src/Flowfield.cpp:1542-1566), newton_T's
+// arithmetic with the mixture fit in LDS and the powers of the starting T given.
 __device__ __forceinline__ double tail_newton(const TailLds& S, TPow p, double E) {
     const double Ek = E * 1e-3;
     double Tn = p.T;
@@ -1109,9 +1110,8 @@ __device__ __forceinline__ double tail_newton(const TailLds& S, TPow p, double E
         const double T0 = p.T;
         const int r = T0 < 1000 ? 0 : 1;
         const double* c = S.mix[r];
-        const double* d = S.mixd[r];
-        const double h = -c[0] * p.Tm1 + c[1] * p.lnT + c[2] * p.T + d[0] * p.T2 + d[1] * p.T3 + d[2] * p.T4 +
-                         d[3] * p.T5 + c[7];
+        const double h = -c[0] * p.Tm1 + c[1] * p.lnT + c[2] * p.T + c[3] * p.T2 / 2 + div_const<3>(c[4] * p.T3) +
+                         c[5] * p.T4 / 4 + div_const<5>(c[6] * p.T5) + c[7];
         const double cp = c[0] * p.Tm2 + c[1] * p.Tm1 + c[2] + c[3] * p.T + c[4] * p.T2 + c[5] * p.T3 + c[6] * p.T4;
         const double f = (h - Ek) - S.rmix * T0;
         const double fp = cp - S.rmix;
@@ -1160,10 +1160,7 @@ __global__ void __launch_bounds__(64) k_chem_tail(Acc acc, const unsigned* __res
     const int qt = hl / NS, qs = hl - (hl / NS) * NS;   // sum lane: quantity, species
     const double mwq = MW[qs];
     const double rmw = RMW(sS);
-    // mixture lanes k = 3..6 and 11..14 also give a_k / (k - 1) (T^2..T^5 terms of h)
     const int kd = kM & 7;
-    const bool has_div = kM < 16 && kd >= 3 && kd <= 6;
-    const double kdiv = (double)(kd - 1);
     for (int i = hl; i < 3 * (NS + 1) * NR; i += 32) (&S.part[0][0][0])[i] = 0.0;
     if (hl == 0) {
         S.mc[NS] = 1.0;
@@ -1217,10 +1214,10 @@ __global__ void __launch_bounds__(64) k_chem_tail(Acc acc, const unsigned* __res
             double a[9];
 #pragma unroll
             for (int k = 0; k < 9; k++) a[k] = lo ? clo[k] : chi[k];
-            const double h = -a[0] * p.Tm1 + a[1] * p.lnT + a[2] * p.T + a[3] * p.T2 / 2 + a[4] * p.T3 / 3 +
-                             a[5] * p.T4 / 4 + a[6] * p.T5 / 5 + a[7];
+            const double h = -a[0] * p.Tm1 + a[1] * p.lnT + a[2] * p.T + a[3] * p.T2 / 2 +
+                             div_const<3>(a[4] * p.T3) + a[5] * p.T4 / 4 + div_const<5>(a[6] * p.T5) + a[7];
             const double sr = -a[0] * p.Tm2 / 2 - a[1] * p.Tm1 + a[2] * p.lnT + a[3] * p.T + a[4] * p.T2 / 2 +
-                              a[5] * p.T3 / 3 + a[6] * p.T4 / 4 + a[8];
+                              div_const<3>(a[5] * p.T3) + a[6] * p.T4 / 4 + a[8];
             S.gx[hl] = h * p.Tm1 - sr;
         }
         __syncthreads();
@@ -1330,7 +1327,6 @@ __global__ void __launch_bounds__(64) k_chem_tail(Acc acc, const unsigned* __res
             for (int s = 0; s < NS; s++) a += (Y[s] * RI(s)) * cf[s];
             if (hl < 16) S.mix[hl >> 3][kd] = a;
             else S.rmix = a;
-            if (has_div) S.mixd[hl >> 3][kd - 3] = a / kdiv;
         }
         __syncthreads();
         T = tail_newton(S, p, (q & Q_F3) ? E : En);

@@ -95,7 +95,7 @@ def eval_batch(n=1 << 22, dtm=1e-8, threshold=8, train_seeds=(101, 102, 103), n_
     sched = autotune.load_or_tune(ctx, (r0, t0, e0), dtm)
     log(f"schedule {sched}")
     out = {"cells": n, "dt": dtm, "threshold": threshold, "training": tr,
-           "schedule": {k: sched[k] for k in ("block", "blocks_per_cu", "waves")}}
+           "schedule": {k: sched[k] for k in ("block", "blocks_per_cu", "waves", "tail")}}
     # FP64 reference
     best = None
     for _ in range(2):

@@ -75,6 +75,8 @@ def load():
     lib.orc_sum.restype = C.c_double
     lib.orc_muscl.argtypes = [C.c_int, C.c_int, Fp, _dp, _dp, C.c_int]
     lib.orc_set_num_threads.argtypes = [C.c_int]
+    lib.orc_check_div_const.argtypes = [C.c_double, C.c_long, C.c_ulonglong]
+    lib.orc_check_div_const.restype = C.c_long
     _lib = lib
     return lib
 

@@ -777,3 +777,32 @@ void orc_field_from_prims(orc_flow *f, const orc_mech *m) {
                                         0.5 * (pow(f->U[c], 2) + pow(f->V[c], 2) + pow(f->W[c], 2)) * f->D[c];
             }
 }
+
+/* Check of the device kernels' division by the constant divisors 3 and 5 of the NASA-9 integrals
+ * (x * RN(1/d) refined by one FMA residual step) against IEEE division: n operands drawn by a
+ * xorshift generator over signs, mantissas and exponents 2^-300..2^300, then the n/4 mantissas
+ * next to 1.0 and next to 2.0. Returns the number of operands where the two differ. */
+long orc_check_div_const(double d, long n, unsigned long long seed) {
+    uint64_t s = seed ? seed : 88172645463325252ull;
+    const double R = 1.0 / d;
+    long bad = 0;
+    for (long i = 0; i < n + n / 2; i++) {
+        uint64_t b;
+        if (i < n) {
+            s ^= s << 13; s ^= s >> 7; s ^= s << 17;
+            const uint64_t r = s;
+            s ^= s << 13; s ^= s >> 7; s ^= s << 17;
+            const uint64_t e = 1023 - 300 + s % 600;
+            b = (r & 0x800FFFFFFFFFFFFFull) | (e << 52);
+        } else {
+            const uint64_t m = (uint64_t)(i - n) / 2;
+            b = (i - n) % 2 ? (1023ull << 52) | m : (1023ull << 52) | (0xFFFFFFFFFFFFFull - m);
+        }
+        double x;
+        memcpy(&x, &b, sizeof x);
+        const double q = x * R;
+        const double r = fma(-q, d, x);
+        if (fma(r, R, q) != x / d) bad++;
+    }
+    return bad;
+}

@@ -90,6 +90,7 @@ void orc_trapezoid_adaptive(orc_flow *f, const orc_mech *m, double dtm);
 void orc_field_from_prims(orc_flow *f, const orc_mech *m);
 
 int orc_num_threads(void);
+long orc_check_div_const(double d, long n, unsigned long long seed);
 void orc_set_num_threads(int n);
 
 #ifdef __cplusplus

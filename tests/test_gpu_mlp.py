@@ -156,7 +156,12 @@ def _oracle_batch(rho, T, E, dtm):
 def test_chem_batch_matches_oracle(dtm, tail):
     """Batched FP64 chemistry (compaction + persistent kernel) equals the oracle's
     Euler::Trapezoid(type 1) cell loop on the same states (7e-6 s: up to 4 sub-steps). With
-    tail = 1 every multi-sub-step cell runs in the one-cell-per-wavefront kernel instead."""
+    tail = 1 every multi-sub-step cell runs in the tail kernel instead.
+
+    The bound is 1e-10 relative, or four times the problem's own conditioning where that is
+    larger: at 7e-6 s (700 times the configured chemistry step) a 1-ulp change of the inputs moves
+    the oracle's own result by about 3e-10, so ulp-level differences (exp/log implementations,
+    FMA contraction) cannot be held to 1e-10 there."""
     torch = _t()
     from eu3d_amd import configs
     from eu3d_amd.api import Context
@@ -164,6 +169,9 @@ def test_chem_batch_matches_oracle(dtm, tail):
     n = 3000
     rho, T, E, _ = configs.chem_states(n, seed=11)
     T_in, E_in, ro, To, nch_o = _oracle_batch(rho, T, E, dtm)
+    ro_p = _oracle_batch(rho * (1 + 2.0 ** -52), T, E, dtm)[2]
+    cond = np.abs(ro_p - ro).max() / ro.sum(1).max()
+    tol = max(1e-10, 4 * cond)
     r = torch.tensor(rho.T.copy(), device="cuda")
     t = torch.tensor(T_in, device="cuda")
     e = torch.tensor(E_in, device="cuda")
@@ -176,9 +184,11 @@ def test_chem_batch_matches_oracle(dtm, tail):
     assert (nch.cpu().numpy() == nch_o).all()
     assert st["active_cells"] == int((nch_o > 1).sum()) and st["substeps_total"] == int(nch_o.sum())
     rg = r.cpu().numpy().T
-    assert np.abs(rg - ro).max() / ro.sum(1).max() < 1e-10
+    err = np.abs(rg - ro).max() / ro.sum(1).max()
+    say("  active", st["active_cells"], "substeps", st["substeps_total"], "max nchem", st["nchem_max_seen"],
+        "rel err", err, "conditioning", cond)
+    assert err < tol
     assert np.abs(t.cpu().numpy() - To).max() < 1e-10 * 3000
-    say("  active", st["active_cells"], "substeps", st["substeps_total"], "max nchem", st["nchem_max_seen"])
 
 
 def test_surrogate_full_fallback_equals_fp64():
@@ -299,13 +309,16 @@ def test_autotune_caches(tmp_path):
     cands = [(64, 2, 1), (256, 4, 2), (128, 8, 4)]
     orig = autotune.tune
     first = autotune.load_or_tune(g, inputs, 7e-6, cache=cache)
-    assert not first["cached"] and first["candidates"] == len(autotune.BLOCKS) * len(autotune.BLOCKS_PER_CU) * len(autotune.WAVES)
+    # the tail thresholds at the default launch, then the launch grid (the default launch once)
+    grid = len(autotune.BLOCKS) * len(autotune.BLOCKS_PER_CU) * len(autotune.WAVES)
+    assert not first["cached"] and first["candidates"] == len(autotune.TAILS) + grid - 1
     assert all(torch.equal(a, b) for a, b in zip(inputs, keep))
     second = autotune.load_or_tune(g, inputs, 7e-6, cache=cache)
-    assert second["cached"] and {k: second[k] for k in ("block", "blocks_per_cu", "waves")} == \
-        {k: first[k] for k in ("block", "blocks_per_cu", "waves")}
-    best, table = orig(g, inputs, 7e-6, candidates=cands)
-    assert len(table) == 3 and best["ms"] == min(table.values())
+    keys = ("block", "blocks_per_cu", "waves", "tail")
+    assert second["cached"] and {k: second[k] for k in keys} == {k: first[k] for k in keys}
+    assert g.get_schedule() == {k: first[k] for k in keys}
+    best, table = orig(g, inputs, 7e-6, candidates=cands, tails=(0, 2))
+    assert len(table) == 4 and best["ms"] == min(table.values())
     say("  tuned", first, "table", table)
 
 

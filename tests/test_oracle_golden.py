@@ -509,3 +509,15 @@ def test_partial_t_matches_finite_difference():
         assert abs(fd - pt[cell + (s,)]) <= 2e-6 * abs(pt[cell + (s,)]), (s, fd, pt[cell + (s,)])
     o.set("CS", cs0)
     o.set("T", T0)
+
+
+@pytest.mark.parametrize("d", [3.0, 5.0])
+def test_constant_division_is_correctly_rounded(d):
+    """The device kernels divide the NASA-9 integral terms by 3 and 5 as x * RN(1/d) refined by one
+    FMA residual step. That equals IEEE division on 2e7 sampled operands over the exponent range
+    and on the 1e7 mantissas next to 1.0 and 2.0; a plain product with RN(1/d) does not."""
+    lib = load()
+    assert lib.orc_check_div_const(d, 20_000_000, 12345) == 0
+    # the check can fail: x * RN(1/3) alone differs from x / 3 on a sizeable fraction of operands
+    x = np.random.default_rng(0).uniform(1.0, 2.0, 100_000)
+    assert np.count_nonzero(x * (1.0 / d) != x / d) > 1000
