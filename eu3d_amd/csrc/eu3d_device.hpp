@@ -137,14 +137,15 @@ __device__ __forceinline__ MixFit mix_fit(const double (&Y)[NS], bool c4) {
 }
 
 // T from internal energy (Flowfield::Get_temp, src/Flowfield.cpp:1542-1566): Newton on
-// sum_s Y_s e_s(T) = E, at most 10 iterations, stop when |dT| < 1e-6 K. E in J/kg.
-__device__ __forceinline__ double newton_T(const MixFit& m, double T, double E, int* iters = nullptr) {
-    double T0 = T, Tn = T;
+// sum_s Y_s e_s(T) = E, at most 10 iterations, stop when |dT| < 1e-6 K. E in J/kg. p holds the
+// powers of the starting temperature (callers that already have them pass them in).
+__device__ __forceinline__ double newton_T_from(const MixFit& m, TPow p, double E, int* iters = nullptr) {
+    double Tn = p.T;
     const double Ek = E * 1e-3;
     int it = 0;
     for (int count = 0; count < 10; count++) {
-        T0 = Tn;
-        const TPow p = tpow(T0);
+        if (count > 0) p = tpow(Tn);
+        const double T0 = p.T;
         const bool lo = T0 < 1000;
         const double* c = lo ? m.lo : m.hi;
         const double h = -c[0] * p.Tm1 + c[1] * p.lnT + c[2] * p.T + c[3] * p.T2 / 2 + div_const<3>(c[4] * p.T3) +
@@ -160,6 +161,9 @@ __device__ __forceinline__ double newton_T(const MixFit& m, double T, double E, 
     if (iters) *iters = it;
     return Tn;
 }
+__device__ __forceinline__ double newton_T(const MixFit& m, double T, double E, int* iters = nullptr) {
+    return newton_T_from(m, tpow(T), E, iters);
+}
 
 // c_p of the mixture [kJ/kg/K] from the mixture fit.
 __device__ __forceinline__ double mix_cp(const MixFit& m, double T) {
@@ -174,10 +178,10 @@ struct Prim {
     double Y[NS];
 };
 
-// Update_after_CS for one cell: CS[NC], previous T and E -> primitives. Quirk F3 solves T
-// from the previous E; FIXED recomputes E from CS first.
-__device__ __forceinline__ Prim prim_from_cs(const double (&cs)[NC], double T_old, double E_old, uint32_t q,
-                                              int* iters = nullptr) {
+// Update_after_CS for one cell: CS[NC], previous T (as its powers p_old) and E -> primitives.
+// Quirk F3 solves T from the previous E; FIXED recomputes E from CS first.
+__device__ __forceinline__ Prim prim_from_cs_p(const double (&cs)[NC], const TPow& p_old, double E_old, uint32_t q,
+                                                int* iters = nullptr) {
     Prim o;
     double D = 0;
 #pragma unroll
@@ -191,7 +195,7 @@ __device__ __forceinline__ Prim prim_from_cs(const double (&cs)[NC], double T_ol
     o.W = cs[NS + 2] * rD;
     o.E = (cs[NS + 3] - 0.5 * D * (o.U * o.U + o.V * o.V + o.W * o.W)) * rD;
     const MixFit m = mix_fit(o.Y, q & Q_C4);
-    o.T = newton_T(m, T_old, (q & Q_F3) ? E_old : o.E, iters);
+    o.T = newton_T_from(m, p_old, (q & Q_F3) ? E_old : o.E, iters);
     double sy = 0;
 #pragma unroll
     for (int s = 0; s < NS; s++) sy += o.Y[s] * RMW(s);
@@ -203,6 +207,11 @@ __device__ __forceinline__ Prim prim_from_cs(const double (&cs)[NC], double T_ol
     o.Gamma = o.Cp / (o.Cp - R_UNIV * sy * 1000);
     o.C = sqrt(o.Gamma * o.P * rD);
     return o;
+}
+
+__device__ __forceinline__ Prim prim_from_cs(const double (&cs)[NC], double T_old, double E_old, uint32_t q,
+                                              int* iters = nullptr) {
+    return prim_from_cs_p(cs, tpow(T_old), E_old, q, iters);
 }
 
 // dT/drho_s (Flowfield::GetPartial_T, src/Flowfield.cpp:1578-1595); quirk C7 keeps h_s in kJ/kg.
@@ -506,12 +515,16 @@ __device__ __forceinline__ void kinetics_g(const TPow& p, const double (&gx)[NS]
 }
 
 template <bool TRAP, bool DIAG, bool C3 = false>
-__device__ __forceinline__ void kinetics(double T, const double (&Mc)[NS], const double (&Dr)[NS],
-                                         const double (&pts)[NS], int rtb, bool c4, KinOut& o) {
-    const TPow p = tpow(T);
+__device__ __forceinline__ void kinetics_p(const TPow& p, const double (&Mc)[NS], const double (&Dr)[NS],
+                                           const double (&pts)[NS], int rtb, bool c4, KinOut& o) {
     double gx[NS], hr[NS];
     species_thermo<false>(p, c4, gx, hr);
     kinetics_g<TRAP, DIAG, C3>(p, gx, Mc, Dr, pts, rtb, o);
+}
+template <bool TRAP, bool DIAG, bool C3 = false>
+__device__ __forceinline__ void kinetics(double T, const double (&Mc)[NS], const double (&Dr)[NS],
+                                         const double (&pts)[NS], int rtb, bool c4, KinOut& o) {
+    kinetics_p<TRAP, DIAG, C3>(tpow(T), Mc, Dr, pts, rtb, c4, o);
 }
 
 // Point-implicit trapezoid update (src/Reaction.cpp:481,587): Di' [kg/m^3] from Di [g/cm^3].

@@ -581,6 +581,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
         const size_t c = IDX(g, i, j, k);
         const double* CS0 = FLD(F, EU3D_FIELD_CS);
         const double T0 = FLD(F, EU3D_FIELD_T)[c], E0 = FLD(F, EU3D_FIELD_E)[c];
+        const TPow p = tpow(T0);
         KinOut ko;
         {
             double rho[NS], D = 0;
@@ -606,7 +607,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
             }
             o.Rgas = R_UNIV * 1000 * sy;
             o.Cp = mix_cp(mix_fit(Y, q & Q_C4), T0) * 1000;
-            const TPow p = tpow(T0);
             double gx[NS], hr[NS], pt[NS];
             species_thermo<true>(p, q & Q_C4, gx, hr);
             partial_T_h(o, q, hr, pt);
@@ -627,7 +627,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
         for (int s = 0; s < NS; s++) cs[s] = cs[s] + (R[s * g.n3 + c] + ko.w[s] * 1e3 * dt) / (1.0 - dt * ko.md[s]);
 #pragma unroll
         for (int s = NS; s < NC; s++) cs[s] = cs[s] + R[s * g.n3 + c];
-        const Prim n = prim_from_cs(cs, T0, E0, q);
+        const Prim n = prim_from_cs_p(cs, p, E0, q);
         double* CS = FLD(F, EU3D_FIELD_CS);
 #pragma unroll
         for (int s = 0; s < NC; s++) CS[s * g.n3 + c] = cs[s];
@@ -804,8 +804,9 @@ __device__ __forceinline__ void chem_inputs(const ChemCell& c, double (&Mc)[NS],
 
 // Sub-step tail: the trapezoid update from the kinetics k, then Flowfield::Explicit(cell) and
 // Update_after_CS(cell) (src/Flowfield.cpp:1189-1203,1287-1319). Leaves the conserved state in cs.
+// tp: powers of c.T (the sub-step's starting temperature, shared with the kinetics).
 __device__ __forceinline__ Prim chem_apply(ChemCell& c, const KinOut& k, const double (&Dr)[NS], double dt,
-                                           uint32_t q, double (&cs)[NC]) {
+                                           uint32_t q, double (&cs)[NC], const TPow& tp) {
     double Dn[NS];
     trap_update(k, Dr, dt, Dn);
     double D = 0.0;
@@ -818,7 +819,7 @@ __device__ __forceinline__ Prim chem_apply(ChemCell& c, const KinOut& k, const d
     cs[NS + 1] = c.V * D;
     cs[NS + 2] = c.W * D;
     cs[NS + 3] = D * c.E + 0.5 * (c.U * c.U + c.V * c.V + c.W * c.W) * D;
-    const Prim o = prim_from_cs(cs, c.T, c.E, q);
+    const Prim o = prim_from_cs_p(cs, tp, c.E, q);
 #pragma unroll
     for (int s = 0; s < NS; s++) c.Di[s] = cs[s];
     c.U = o.U;
@@ -968,7 +969,8 @@ __global__ void __launch_bounds__(256) k_trap_first(G3 g, Fields F, uint32_t q, 
 #pragma unroll
         for (int s = 0; s < NS; s++) Y[s] = FLD(F, EU3D_FIELD_YI)[s * g.n3 + c];
         KinOut ko;
-        kinetics<true, false, C3>(cell.T, Mc, Dr, Dr, (q & Q_C2) ? 1 : 2, q & Q_C4, ko);
+        const TPow tp = tpow(cell.T);
+        kinetics_p<true, false, C3>(tp, Mc, Dr, Dr, (q & Q_C2) ? 1 : 2, q & Q_C4, ko);
         const double dtm = scc->dt;
         nch = chem_nchem(ko, Dr, Y, dtm, nmax);
         FLD(F, EU3D_FIELD_NCHEM)[c] = (double)nch;
@@ -976,7 +978,7 @@ __global__ void __launch_bounds__(256) k_trap_first(G3 g, Fields F, uint32_t q, 
         to_net = nthr > 0 && nch > nthr && cell.T >= T_cold;
         if (!predict_only && nch > 0 && !to_net) {
             double cs[NC];
-            const Prim o = chem_apply(cell, ko, Dr, dtm / nch, q, cs);
+            const Prim o = chem_apply(cell, ko, Dr, dtm / nch, q, cs, tp);
             GridCells acc{g, F};
             acc.store((unsigned)c, cs, o, ko);
             active = nch > 1;
@@ -1030,8 +1032,9 @@ k_chem_persist(Acc acc, const unsigned* __restrict__ list, Scalars* sc, uint32_t
             double Mc[NS], Dr[NS], cs[NC];
             chem_inputs(cell, Mc, Dr);
             KinOut ko;
-            kinetics<true, false, C3>(cell.T, Mc, Dr, Dr, (q & Q_C2) ? 1 : 2, q & Q_C4, ko);
-            const Prim o = chem_apply(cell, ko, Dr, dtm / nch, q, cs);
+            const TPow tp = tpow(cell.T);
+            kinetics_p<true, false, C3>(tp, Mc, Dr, Dr, (q & Q_C2) ? 1 : 2, q & Q_C4, ko);
+            const Prim o = chem_apply(cell, ko, Dr, dtm / nch, q, cs, tp);
             if (--rem == 0) {
                 acc.store(id, cs, o, ko);
                 have = false;
@@ -1360,7 +1363,8 @@ __global__ void __launch_bounds__(256) k_batch_first(BatchCells acc, int* nchem_
 #pragma unroll
         for (int s = 0; s < NS; s++) Y[s] = cell.Di[s] / D;
         KinOut ko;
-        kinetics<true, false, C3>(cell.T, Mc, Dr, Dr, (q & Q_C2) ? 1 : 2, q & Q_C4, ko);
+        const TPow tp = tpow(cell.T);
+        kinetics_p<true, false, C3>(tp, Mc, Dr, Dr, (q & Q_C2) ? 1 : 2, q & Q_C4, ko);
         nch = chem_nchem(ko, Dr, Y, dtm, nmax);
         nchem_keep[id] = nch;
         if (nchem_out) nchem_out[id] = nch;
@@ -1368,7 +1372,7 @@ __global__ void __launch_bounds__(256) k_batch_first(BatchCells acc, int* nchem_
         to_net = nthr > 0 && nch > nthr && cell.T >= T_cold;
         if (nch > 0 && !to_net) {
             double cs[NC];
-            const Prim o = chem_apply(cell, ko, Dr, dtm / nch, q, cs);
+            const Prim o = chem_apply(cell, ko, Dr, dtm / nch, q, cs, tp);
             acc.store((unsigned)id, cs, o, ko);
             active = nch > 1;
         }
@@ -1405,14 +1409,15 @@ __global__ void __launch_bounds__(256) k_trap_first_list(G3 g, Fields F, uint32_
     #pragma unroll
             for (int s = 0; s < NS; s++) Y[s] = FLD(F, EU3D_FIELD_YI)[s * g.n3 + c];
             KinOut ko;
-            kinetics<true, false, C3>(cell.T, Mc, Dr, Dr, (q & Q_C2) ? 1 : 2, q & Q_C4, ko);
+            const TPow tp = tpow(cell.T);
+            kinetics_p<true, false, C3>(tp, Mc, Dr, Dr, (q & Q_C2) ? 1 : 2, q & Q_C4, ko);
             const double dtm = sc->dtm;
             nch = chem_nchem(ko, Dr, Y, dtm, nmax);
             FLD(F, EU3D_FIELD_NCHEM)[c] = (double)nch;
             if (nch <= 0) atomicOr(&sc->nonfinite, 2);
             if (nch > 0) {
                 double cs[NC];
-                const Prim o = chem_apply(cell, ko, Dr, dtm / nch, q, cs);
+                const Prim o = chem_apply(cell, ko, Dr, dtm / nch, q, cs, tp);
                 GridCells acc{g, F};
                 acc.store((unsigned)c, cs, o, ko);
                 active = nch > 1;
@@ -1449,13 +1454,14 @@ __global__ void __launch_bounds__(256) k_batch_first_list(BatchCells acc, int* n
     #pragma unroll
             for (int s = 0; s < NS; s++) Y[s] = cell.Di[s] / D;
             KinOut ko;
-            kinetics<true, false, C3>(cell.T, Mc, Dr, Dr, (q & Q_C2) ? 1 : 2, q & Q_C4, ko);
+            const TPow tp = tpow(cell.T);
+            kinetics_p<true, false, C3>(tp, Mc, Dr, Dr, (q & Q_C2) ? 1 : 2, q & Q_C4, ko);
             nch = chem_nchem(ko, Dr, Y, dtm, nmax);
             nchem_keep[id] = nch;
             if (nch <= 0) atomicOr(&sc->nonfinite, 2);
             if (nch > 0) {
                 double cs[NC];
-                const Prim o = chem_apply(cell, ko, Dr, dtm / nch, q, cs);
+                const Prim o = chem_apply(cell, ko, Dr, dtm / nch, q, cs, tp);
                 acc.store((unsigned)id, cs, o, ko);
                 active = nch > 1;
             }
