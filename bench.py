@@ -36,7 +36,8 @@ KERNEL_BYTES_PER_CELL = {
     "flux_z": (16 + 13) * 8,
     "flux_xy": (16 + 13 + 13) * 8,
 }
-KERNEL_NAMES = {"imex_fused": "k_imex_fused", "flux_z": "k_flux_z<16>", "flux_xy": "k_flux_xy<16,16,XCD order>"}
+KERNEL_NAMES = {"imex_fused": "k_imex_fused", "flux_z": "k_flux_z<8, MUSCL_2 minmod>",
+                "flux_xy": "k_flux_xy<16,16,XCD order, MUSCL_2 minmod>"}
 PROFILE = "imex_fused_bubble3d_384x128x128_pmc.json"
 
 
@@ -113,7 +114,7 @@ def bench_batched_chemistry(ctx, n=1 << 22, dtm=1e-8, seed=2024):
         out[burnt] = {"cells": n, "dt": dtm, "ms": ms, "cells_per_s": n / (ms * 1e-3),
                       "substeps_per_s": st["substeps_total"] / (ms * 1e-3), "active_cells": st["active_cells"],
                       "substeps_total": st["substeps_total"], "nchem_max": st["nchem_max_seen"],
-                      "schedule": {k: sched[k] for k in ("block", "blocks_per_cu", "waves")},
+                      "schedule": {k: sched[k] for k in ("block", "blocks_per_cu", "waves", "tail")},
                       "schedule_from_cache": sched.get("cached", False)}
         del r0, t0, e0
     return out

@@ -440,11 +440,14 @@ __device__ __forceinline__ const double* prim_ptr(const Fields& F, const G3& g, 
 // XCD: workgroups are dispatched to the 8 XCDs round-robin by linear id, and each XCD has its own
 // L2. The remap hands every XCD a contiguous range of tiles (x fastest, then y, then z), so the
 // halo columns and rows a tile shares with its neighbours are fetched into one L2, not eight.
-template <int XY_X, int XY_Y, bool XCD>
-__global__ void __launch_bounds__(XY_X * XY_Y) k_flux_xy(G3 g, Fields F, int order, int limiter, uint32_t q,
+// MM: the MUSCL_2 + minmod configuration compiled in (order and limiter arguments ignored);
+// otherwise both are taken at run time.
+template <int XY_X, int XY_Y, bool XCD, bool MM>
+__global__ void __launch_bounds__(XY_X * XY_Y) k_flux_xy(G3 g, Fields F, int order_a, int limiter_a, uint32_t q,
                                                  const double* __restrict__ xn, const double* __restrict__ yn,
                                                  const Scalars* __restrict__ sc) {
     constexpr int XY_TX = XY_X + 3, XY_TY = XY_Y + 3;
+    const int order = MM ? 1 : order_a, limiter = MM ? 0 : limiter_a;
     __shared__ double sp[NV][XY_TY][XY_TX];
     int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
     if (XCD) {
@@ -523,9 +526,12 @@ __global__ void __launch_bounds__(XY_X * XY_Y) k_flux_xy(G3 g, Fields F, int ord
     }
 }
 
-template <int ZCH>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) k_flux_z(G3 g, Fields F, int order, int limiter, uint32_t q,
+// One wave per SIMD: the 4-plane window and the prefetched plane (80 values) stay in registers
+// with no spill (two waves per SIMD spilled and ran 17 % slower).
+template <int ZCH, bool MM>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) k_flux_z(G3 g, Fields F, int order_a, int limiter_a, uint32_t q,
                                                 const double* __restrict__ zn, const Scalars* __restrict__ sc) {
+    const int order = MM ? 1 : order_a, limiter = MM ? 0 : limiter_a;
     const int x = g.bc + blockIdx.x * ZX + threadIdx.x, y = g.bc + blockIdx.y * ZY + threadIdx.y;
     if (x >= g.ni + g.bc || y >= g.nj + g.bc) return;
     const int z0 = g.bc + blockIdx.z * ZCH, z1 = min(z0 + ZCH, g.nk + g.bc);
@@ -1598,24 +1604,32 @@ template <int X, int Y, bool XCD>
 static void launch_xy(eu3d_ctx* c, int diff, int limiter) {
     const G3& g = c->g;
     dim3 gxy((g.ni + X - 2) / (X - 1), (g.nj + Y - 2) / (Y - 1), g.nk);
-    k_flux_xy<X, Y, XCD><<<gxy, dim3(X, Y), 0, c->stream>>>(g, c->fl, diff, limiter, c->quirks, c->xnode, c->ynode, c->sc);
+    if (diff == 1 && limiter == 0)
+        k_flux_xy<X, Y, XCD, true><<<gxy, dim3(X, Y), 0, c->stream>>>(g, c->fl, diff, limiter, c->quirks, c->xnode,
+                                                                      c->ynode, c->sc);
+    else
+        k_flux_xy<X, Y, XCD, false><<<gxy, dim3(X, Y), 0, c->stream>>>(g, c->fl, diff, limiter, c->quirks, c->xnode,
+                                                                       c->ynode, c->sc);
 }
 
 template <int ZCH>
 static void launch_z(eu3d_ctx* c, int diff, int limiter) {
     const G3& g = c->g;
     dim3 gz((g.ni + ZX - 1) / ZX, (g.nj + ZY - 1) / ZY, (g.nk + ZCH - 1) / ZCH);
-    k_flux_z<ZCH><<<gz, dim3(ZX, ZY), 0, c->stream>>>(g, c->fl, diff, limiter, c->quirks, c->znode, c->sc);
+    if (diff == 1 && limiter == 0)
+        k_flux_z<ZCH, true><<<gz, dim3(ZX, ZY), 0, c->stream>>>(g, c->fl, diff, limiter, c->quirks, c->znode, c->sc);
+    else
+        k_flux_z<ZCH, false><<<gz, dim3(ZX, ZY), 0, c->stream>>>(g, c->fl, diff, limiter, c->quirks, c->znode, c->sc);
 }
 
 hipError_t launch_flux_z(eu3d_ctx* c, int diff, int limiter) {
     // z planes per column chunk: shorter chunks give more waves to hide latency with, at the cost
-    // of re-reading 3 planes of stencil per chunk (bubble3d 384x128x128: 16 planes 0.466 ms,
-    // 8 planes 0.492 ms, 32 planes 0.499 ms)
+    // of re-reading 3 planes of stencil per chunk (bubble3d 384x128x128, MUSCL_2 + minmod, one
+    // wave per SIMD: 8 planes 0.391 ms, 16 planes 0.401 ms, 32 planes 0.403 ms)
     switch (c->zchunk) {
-    case 8: launch_z<8>(c, diff, limiter); break;
+    case 16: launch_z<16>(c, diff, limiter); break;
     case 32: launch_z<32>(c, diff, limiter); break;
-    default: launch_z<16>(c, diff, limiter);
+    default: launch_z<8>(c, diff, limiter);
     }
     return hipGetLastError();
 }
