@@ -72,7 +72,7 @@ class _Mech(C.Structure):
 class StepParams(C.Structure):
     _fields_ = [("cfl", C.c_double), ("final_time", C.c_double), ("bc_kinds", C.c_int * 6),
                 ("diff", C.c_int), ("limiter", C.c_int), ("rk", C.c_int), ("chem", C.c_int),
-                ("nchem_max", C.c_int)]
+                ("nchem_max", C.c_int), ("skip_derive", C.c_int)]
 
 
 class ChemStats(C.Structure):
@@ -294,8 +294,12 @@ class Context:
         return self.lib.eu3d_check_finite(self.h)
 
     def step(self, nsteps: int = 1, cfl: float = 0.5, final_time: float = 1e30, bc_kinds=(1,) * 6,
-             diff: int = 1, limiter: int = 0, rk: int = 1, chem: int = CHEM_IMEX, nchem_max: int = 0):
-        p = StepParams(cfl, final_time, (C.c_int * 6)(*bc_kinds), diff, limiter, rk, chem, nchem_max)
+             diff: int = 1, limiter: int = 0, rk: int = 1, chem: int = CHEM_IMEX, nchem_max: int = 0,
+             skip_derive: bool = False):
+        """nsteps steps of eu3d_step. skip_derive leaves the derived fields (Di, Mc, Wav, Rgas, Cp,
+        Ma, Partion_T) unrefreshed, for callers stepping one call at a time."""
+        p = StepParams(cfl, final_time, (C.c_int * 6)(*bc_kinds), diff, limiter, rk, chem, nchem_max,
+                       1 if skip_derive else 0)
         self._chk(self.lib.eu3d_step(self.h, C.byref(p), nsteps), "step")
 
     def sync(self):

@@ -319,7 +319,7 @@ int eu3d_step(eu3d_ctx* c, const eu3d_step_params* p, int nsteps) {
     if (p->chem == EU3D_CHEM_TRAP_ADAPTIVE) c->stats.cells = (long long)c->g.ni * c->g.nj * c->g.nk;
     if (p->chem == EU3D_CHEM_SURROGATE && nsteps > 0) HIPCHK(c, surrogate_grid_stats(c), "surrogate stats");
     if (nsteps > 0) {
-        HIPCHK(c, launch_derive(c), "step derive");
+        if (!p->skip_derive) HIPCHK(c, launch_derive(c), "step derive");
         HIPCHK(c, launch_check_finite(c), "step finite");
     }
     return EU3D_OK;

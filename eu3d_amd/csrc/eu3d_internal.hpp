@@ -88,9 +88,10 @@ hipError_t launch_diagonalized(eu3d_ctx* c);
 hipError_t launch_update_imex(eu3d_ctx* c);
 hipError_t launch_trapezoid(eu3d_ctx* c);
 // nthr > 0: cells above T_cold needing more than nthr sub-steps keep their state and are listed
-// in slist / scount for the surrogate network instead of taking the FP64 sub-steps.
+// in slist / scount for the surrogate network instead of taking the FP64 sub-steps. adv: run the
+// step's Update_after_Adv (CS += RHS) and Update_after_CS inside the prediction pass.
 hipError_t launch_trap_adaptive(eu3d_ctx* c, int nchem_max, bool predict_only, int nthr = 0, double T_cold = 0.0,
-                                unsigned* slist = nullptr, unsigned* scount = nullptr);
+                                unsigned* slist = nullptr, unsigned* scount = nullptr, bool adv = false);
 hipError_t launch_check_finite(eu3d_ctx* c);
 hipError_t ensure_coop(eu3d_ctx* c, long long n);
 hipError_t launch_set_scalars(eu3d_ctx* c, double dt, double time);

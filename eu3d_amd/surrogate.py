@@ -53,11 +53,14 @@ class Harvest:
     def __init__(self):
         self.rho0, self.rho1, self.T0, self.dt = [], [], [], []
 
-    def add(self, rho0, rho1, T0, dt):
+    def add(self, rho0, rho1, T0, dt, keep=None):
         """Keep the pairs whose states before and after are finite (a few cells of the FP64
-        chemistry leave the finite range at large dt)."""
+        chemistry leave the finite range at large dt), and where the boolean device mask keep
+        holds, if given."""
         torch = _torch()
         ok = torch.isfinite(rho0).all(0) & torch.isfinite(rho1).all(0) & torch.isfinite(T0)
+        if keep is not None:
+            ok &= keep
         self.rho0.append(rho0[:, ok].contiguous())
         self.rho1.append(rho1[:, ok].contiguous())
         self.T0.append(T0[ok].contiguous())

@@ -47,15 +47,20 @@ def _timed(ctx_stream, fn):
     return out, a.elapsed_time(b)
 
 
-def harvest_batch(ctx, seeds, n, dtm, log=print):
-    """(rho0, rho1, T0, dt) pairs: one FP64 chemistry step of equilibrium-line states."""
+def harvest_batch(ctx, seeds, n, dtm, min_nchem=0, log=print):
+    """(rho0, rho1, T0, dt) pairs: one FP64 chemistry step of equilibrium-line states. With
+    min_nchem > 0 only the cells needing more FP64 sub-steps than that are kept: the cells the
+    hybrid sends through the network."""
+    torch = _torch()
     H = surrogate.Harvest()
     for sd in seeds:
         log(f"harvest seed {sd}: {n} states")
         (r, t, e), _ = chemistry.states(ctx, n, seed=sd, burnt="equilibrium", log=log)
         r0, t0 = r.clone(), t.clone()
-        ctx.chem_batch(n, r.data_ptr(), t.data_ptr(), e.data_ptr(), dtm, 10000)
-        H.add(r0, r, t0, dtm)
+        nch = torch.empty(n, dtype=torch.int32, device="cuda")
+        ctx.chem_batch(n, r.data_ptr(), t.data_ptr(), e.data_ptr(), dtm, 10000, nch.data_ptr())
+        H.add(r0, r, t0, dtm, keep=(nch > min_nchem) if min_nchem > 0 else None)
+        log(f"  kept {H.n} pairs so far")
     return H
 
 
@@ -82,13 +87,17 @@ def errors(T_a, T_b, r_a, r_b):
             "Y_Linf": float(np.abs(Ya - Yb).max()), "nonfinite": int((~fin).sum())}
 
 
-def eval_batch(n=1 << 22, dtm=1e-8, threshold=8, train_seeds=(101, 102, 103), n_train=1 << 21, epochs=12, log=print):
+def eval_batch(n=1 << 22, dtm=1e-8, threshold=8, train_seeds=(101, 102, 103), n_train=1 << 21, epochs=12,
+               stiff_only=False, log=print, **fit):
+    """stiff_only: train on the cells above the hybrid threshold only (the cells the network sees)."""
     torch = _torch()
     x = configs.make_nodes(8, 1e-4)
     ctx = Context(x, x, x, quirks=0)
-    H = harvest_batch(ctx, train_seeds, n_train, dtm, log=log)
+    H = harvest_batch(ctx, train_seeds, n_train, dtm, min_nchem=threshold if stiff_only else 0, log=log)
     log(f"training on {H.n} pairs")
-    m, p, tr = train_network(H, epochs=epochs, threshold=threshold, log=log)
+    m, p, tr = train_network(H, epochs=epochs, threshold=threshold, log=log, **fit)
+    tr["stiff_only"] = stiff_only
+    tr["checks"] = {"dT_max": p.dT_max, "y_neg_tol": p.y_neg_tol, "elem_tol": p.elem_tol, "hull_margin": fit.get("hull_margin", 0.25)}
     del H
     log(f"evaluation states: {n}")
     (r0, t0, e0), _ = chemistry.states(ctx, n, seed=2024, burnt="equilibrium", log=log)
@@ -116,6 +125,17 @@ def eval_batch(n=1 << 22, dtm=1e-8, threshold=8, train_seeds=(101, 102, 103), n_
                         "rejected_fraction_of_network": sst["fallback"] / max(1, sst["network"])}
     out["speedup"] = out["fp64"]["ms"] / best
     out["errors_vs_fp64"] = errors(t.cpu().numpy(), ref_t, r.cpu().numpy(), ref_r)
+    # errors over the cells the network answered (routed, not rejected), the rest being FP64 anyway
+    nch = torch.empty(n, dtype=torch.int32, device="cuda")
+    r2, t2, e2 = r0.clone(), t0.clone(), e0.clone()
+    ctx.chem_batch(n, r2.data_ptr(), t2.data_ptr(), e2.data_ptr(), dtm, 10000, nch.data_ptr())
+    routed = (nch.cpu().numpy() > threshold) & (t0.cpu().numpy() >= p.T_cold)
+    tn = t.cpu().numpy()
+    diff = np.abs(tn - ref_t) > 0
+    sel = routed & diff
+    if sel.any():
+        out["errors_network_cells"] = errors(tn[sel], ref_t[sel], r.cpu().numpy()[:, sel], ref_r[:, sel])
+        out["errors_network_cells"]["cells"] = int(sel.sum())
     ctx.close()
     return out
 
@@ -155,10 +175,11 @@ def eval_detonation(grid=(512, 256, 128), nsteps=150, threshold=4, train_grid=(5
     net = rej = cells = 0
     for s in range(nsteps):
         c = 0.05 + (0.5 - 0.05) * min(1.0, s / 20)
-        _, ms = _timed(plain.stream(), lambda: plain.step(1, cfl=c, bc_kinds=wl.bc_kinds, **kw))
+        last = s == nsteps - 1   # the derived fields are refreshed once, after the last step
+        _, ms = _timed(plain.stream(), lambda: plain.step(1, cfl=c, bc_kinds=wl.bc_kinds, skip_derive=not last, **kw))
         times["plain"] += ms
         kws = dict(kw, chem=4)
-        _, ms = _timed(sur.stream(), lambda: sur.step(1, cfl=c, bc_kinds=wl.bc_kinds, **kws))
+        _, ms = _timed(sur.stream(), lambda: sur.step(1, cfl=c, bc_kinds=wl.bc_kinds, skip_derive=not last, **kws))
         times["surrogate"] += ms
         st = sur.surrogate_stats()
         if (s + 1) % 10 == 0:
@@ -199,6 +220,10 @@ def main():
     ap.add_argument("--steps", type=int, default=150)
     ap.add_argument("--grid", type=str, default="512x256x128")
     ap.add_argument("--epochs", type=int, default=None)
+    ap.add_argument("--stiff-only", action="store_true", help="batch: train on the cells above the threshold only")
+    ap.add_argument("--y-neg-tol", type=float, default=None)
+    ap.add_argument("--dT-max", type=float, default=None)
+    ap.add_argument("--seeds", type=int, default=3, help="batch: training seeds of 2^21 states each")
     a = ap.parse_args()
     log = lambda s: print(s, flush=True)
     if a.workload == "batch":
@@ -207,7 +232,12 @@ def main():
             kw["threshold"] = a.threshold
         if a.epochs:
             kw["epochs"] = a.epochs
-        res = eval_batch(log=log, **kw)
+        if a.y_neg_tol is not None:
+            kw["y_neg_tol"] = a.y_neg_tol
+        if a.dT_max is not None:
+            kw["dT_max"] = a.dT_max
+        kw["train_seeds"] = tuple(101 + i for i in range(a.seeds))
+        res = eval_batch(log=log, stiff_only=a.stiff_only, **kw)
     else:
         kw = dict(grid=tuple(int(v) for v in a.grid.split("x")), nsteps=a.steps)
         if a.threshold is not None:

@@ -165,11 +165,15 @@ int eu3d_check_finite(eu3d_ctx *ctx);
 
 /* One full time step of Euler::Computing (src/Euler.cpp:266-333) on the device, with no host
  * round trip: CFL dt (when cfl > 0; else the stored dt), ghost fill, advection, chemistry.
- * rk: 1 = explicit Euler (TimeAdv::EE), 2 = two-stage SSP Runge-Kutta built from EE. */
+ * rk: 1 = explicit Euler (TimeAdv::EE), 2 = two-stage SSP Runge-Kutta built from EE.
+ * The steps do not need the derived fields (Di, Mc, Wav, Rgas, Cp, Ma, Partion_T); eu3d_step
+ * refreshes them once after its last step unless skip_derive is nonzero (for a caller stepping
+ * one call at a time, on every call but the last). */
 typedef struct {
     double cfl, final_time;
     int bc_kinds[6];
     int diff, limiter, rk, chem, nchem_max;
+    int skip_derive;
 } eu3d_step_params;
 int eu3d_step(eu3d_ctx *ctx, const eu3d_step_params *p, int nsteps);
 

@@ -134,6 +134,39 @@ def test_steps_fixed(chem):
     compare(g, o, ["CS", "T", "D", "P", "E", "YI"])
 
 
+def test_steps_trap_adaptive_ref():
+    """The step's adaptive-trapezoid path fuses Update_after_Adv into the prediction pass; under
+    REF quirk F1 that update adds only the species components of RHS."""
+    say("case steps trap adaptive REF")
+    wl = small_det()
+    g, o = make_pair(wl, Q_REF)
+    kw = dict(nsteps=2, cfl=0.05, bc_kinds=wl.bc_kinds, rk=1, chem=3, nchem_max=200)
+    g.step(**kw)
+    o.step(**kw)
+    compare(g, o, ["CS", "T", "D", "P", "E", "YI"], allow_nan=True)
+
+
+def test_step_skip_derive_same_state():
+    """Stepping one call at a time with skip_derive on all but the last call leaves every field,
+    the derived ones included, as one call of all the steps does."""
+    say("case skip_derive")
+    from eu3d_amd.api import Context
+    from tests.common import init_state
+    wl = small_det()
+    a = Context(wl.xnode, wl.ynode, wl.znode, quirks=Q_FIXED)
+    b = Context(wl.xnode, wl.ynode, wl.znode, quirks=Q_FIXED)
+    init_state(a, wl)
+    init_state(b, wl)
+    kw = dict(cfl=0.05, bc_kinds=wl.bc_kinds, rk=1, chem=3, nchem_max=200)
+    a.step(3, **kw)
+    b.step(1, skip_derive=True, **kw)
+    b.step(1, skip_derive=True, **kw)
+    b.step(1, **kw)
+    for f in ("CS", "T", "E", "D", "P", "YI", "DI", "MC", "WAV", "RGAS", "CP", "MA", "PT", "CMS"):
+        x, y = a.get(f), b.get(f)
+        assert np.array_equal(x, y, equal_nan=True), f
+
+
 def test_steps_rk2_zn1d():
     say("case rk2 zn1d")
     wl = small_zn()
