@@ -279,8 +279,13 @@ static int chem_status(eu3d_ctx* c) {
     HIPCHK(c, hipMemcpyAsync(&s, c->sc, sizeof(Scalars), hipMemcpyDeviceToHost, c->stream), "chem stats");
     HIPCHK(c, hipStreamSynchronize(c->stream), "chem sync");
     c->stats.active_cells = (long long)s.chem_active + (long long)s.coop_count;
-    c->stats.substeps_total = (long long)s.chem_substeps;
-    c->stats.nchem_max_seen = (int)s.chem_nmax;
+    unsigned long long sub = 0, mx = 0;
+    for (int k = 0; k < STAT_SLOTS; k++) {
+        sub += s.stat[k][0];
+        mx = s.stat[k][1] > mx ? s.stat[k][1] : mx;
+    }
+    c->stats.substeps_total = (long long)sub;
+    c->stats.nchem_max_seen = (int)mx;
     if (s.nonfinite & 2) return fail(c, EU3D_ERR_NCHEM, "Nchem <= 0 (src/Reaction.cpp:664-668)");
     return EU3D_OK;
 }

@@ -915,8 +915,11 @@ __device__ __forceinline__ void chem_count(Scalars* sc, int nch, bool active, un
     }
     if (lane == leader) {
         if (am) base = atomicAdd(&sc->chem_active, (unsigned long long)__popcll(am));
-        atomicAdd(&sc->chem_substeps, (unsigned long long)tot);
-        atomicMax(&sc->chem_nmax, (unsigned long long)mx);
+        const unsigned wave = ((((blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x) * blockDim.z +
+                                threadIdx.z) * blockDim.y + threadIdx.y) * blockDim.x + threadIdx.x;
+        unsigned long long* st = sc->stat[(wave >> 6) % STAT_SLOTS];
+        atomicAdd(&st[0], (unsigned long long)tot);
+        atomicMax(&st[1], (unsigned long long)mx);
     }
     base = __shfl(base, leader);
     if (active) list[base + __popcll(am & ((1ull << lane) - 1ull))] = id;
@@ -954,10 +957,11 @@ __device__ __forceinline__ void chem_count_tail(Scalars* sc, int nch, bool activ
 // needing more than nthr sub-steps keeps its state and goes to the network list instead.
 // ADV: the step's Update_after_Adv (CS += RHS, src/Flowfield.cpp:1140-1157) and Update_after_CS
 // run first in registers, and the inputs of the prediction come from that state instead of the
-// fields it would have written; a cell that takes no sub-step stores it. One wave per SIMD: at
-// two the fused variant spills (3-D detonation 512x256x128: 12.58 against 12.71 ms per step).
+// fields it would have written. A cell whose prediction fails (Nchem <= 0, the reference's abort)
+// is left to k_adv_nochem, which stores that state; holding it here through the kinetics would
+// cost registers on every cell.
 template <bool C3, bool ADV>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ADV ? 2 : 1, ADV ? 2 : 1)))
 k_trap_first(G3 g, Fields F, uint32_t q, const Scalars* __restrict__ scc, Scalars* sc, unsigned* list, int nmax,
              int predict_only, int nthr, double T_cold, unsigned* slist, unsigned* scount, TailRoute rt) {
     int i, j, k;
@@ -967,7 +971,7 @@ k_trap_first(G3 g, Fields F, uint32_t q, const Scalars* __restrict__ scc, Scalar
     bool active = false, to_net = false;
     if (in) {
         ChemCell cell;
-        double Mc[NS], Dr[NS], Y[NS];
+        double Mc[NS];
         if (ADV) {
             double cs0[NC];
             load_cs(F, g, c, cs0);
@@ -981,8 +985,6 @@ k_trap_first(G3 g, Fields F, uint32_t q, const Scalars* __restrict__ scc, Scalar
             for (int s = 0; s < NS; s++) {
                 cell.Di[s] = cs0[s];
                 Mc[s] = (cs0[s] * RMW(s) * 1000) * 1e-6;
-                Dr[s] = cs0[s] * 1e-3;
-                Y[s] = o0.Y[s];
             }
             cell.U = o0.U;
             cell.V = o0.V;
@@ -991,19 +993,35 @@ k_trap_first(G3 g, Fields F, uint32_t q, const Scalars* __restrict__ scc, Scalar
             cell.T = o0.T;
         } else {
 #pragma unroll
-            for (int s = 0; s < NS; s++) cell.Di[s] = FLD(F, EU3D_FIELD_DI)[s * g.n3 + c];
+            for (int s = 0; s < NS; s++) {
+                cell.Di[s] = FLD(F, EU3D_FIELD_DI)[s * g.n3 + c];
+                Mc[s] = FLD(F, EU3D_FIELD_MC)[s * g.n3 + c] * 1e-6;
+            }
             cell.U = FLD(F, EU3D_FIELD_U)[c];
             cell.V = FLD(F, EU3D_FIELD_V)[c];
             cell.W = FLD(F, EU3D_FIELD_W)[c];
             cell.E = FLD(F, EU3D_FIELD_E)[c];
             cell.T = FLD(F, EU3D_FIELD_T)[c];
-            react_units(F, g, c, Mc, Dr);
-#pragma unroll
-            for (int s = 0; s < NS; s++) Y[s] = FLD(F, EU3D_FIELD_YI)[s * g.n3 + c];
         }
         KinOut ko;
         const TPow tp = tpow(cell.T);
-        kinetics_p<true, false, C3>(tp, Mc, Dr, Dr, (q & Q_C2) ? 1 : 2, q & Q_C4, ko);
+        // the trapezoid's rates do not read Dr; it and Y are formed after the kinetics so that they
+        // are not held through them (Y as Update_after_CS forms it: Di / sum Di)
+        kinetics_p<true, false, C3>(tp, Mc, Mc, Mc, (q & Q_C2) ? 1 : 2, q & Q_C4, ko);
+        double Dr[NS], Y[NS];
+        if (ADV) {
+            double D = 0;
+#pragma unroll
+            for (int s = 0; s < NS; s++) D += cell.Di[s];
+            const double rD = 1.0 / D;
+#pragma unroll
+            for (int s = 0; s < NS; s++) Y[s] = cell.Di[s] * rD;
+        } else {
+#pragma unroll
+            for (int s = 0; s < NS; s++) Y[s] = FLD(F, EU3D_FIELD_YI)[s * g.n3 + c];
+        }
+#pragma unroll
+        for (int s = 0; s < NS; s++) Dr[s] = cell.Di[s] * 1e-3;
         const double dtm = scc->dt;
         nch = chem_nchem(ko, Dr, Y, dtm, nmax);
         FLD(F, EU3D_FIELD_NCHEM)[c] = (double)nch;
@@ -1015,26 +1033,29 @@ k_trap_first(G3 g, Fields F, uint32_t q, const Scalars* __restrict__ scc, Scalar
             GridCells acc{g, F};
             acc.store((unsigned)c, cs, o, ko);
             active = nch > 1;
-        } else if (ADV) {
-            // no sub-step (Nchem <= 0): store the state after Update_after_CS, recomputed from
-            // memory (nothing of this cell has been written yet) rather than held through the kinetics
-            asm volatile("" ::: "memory");
-            double cs1[NC];
-            load_cs(F, g, c, cs1);
-            const double* R = FLD(F, EU3D_FIELD_RHS);
-            const int nadd = (q & Q_F1) ? NS : NC;
-#pragma unroll
-            for (int s = 0; s < NC; s++)
-                if (s < nadd) cs1[s] = cs1[s] + R[s * g.n3 + c];
-            const Prim o1 = prim_from_cs(cs1, FLD(F, EU3D_FIELD_T)[c], FLD(F, EU3D_FIELD_E)[c], q);
-            double* CS = FLD(F, EU3D_FIELD_CS);
-#pragma unroll
-            for (int s = 0; s < NC; s++) CS[s * g.n3 + c] = cs1[s];
-            store_prims(F, g, c, cs1, o1);
         }
     }
     chem_count_tail(sc, in && !to_net ? nch : 0, active, list, (unsigned)c, rt);
     if (nthr > 0) wave_append(slist, scount, to_net, (unsigned)c);
+}
+
+// After the fused prediction pass: Update_after_Adv + Update_after_CS for the cells whose
+// prediction failed (Nchem <= 0), which that pass did not store.
+__global__ void __launch_bounds__(256) k_adv_nochem(G3 g, Fields F, uint32_t q) {
+    int i, j, k;
+    if (!interior_ijk(g, i, j, k)) return;
+    const size_t c = IDX(g, i, j, k);
+    if (FLD(F, EU3D_FIELD_NCHEM)[c] > 0) return;
+    double cs[NC];
+    load_cs(F, g, c, cs);
+    const double* R = FLD(F, EU3D_FIELD_RHS);
+    const int nadd = (q & Q_F1) ? NS : NC;
+    for (int s = 0; s < nadd; s++) cs[s] = cs[s] + R[s * g.n3 + c];
+    const Prim o = prim_from_cs(cs, FLD(F, EU3D_FIELD_T)[c], FLD(F, EU3D_FIELD_E)[c], q);
+    double* CS = FLD(F, EU3D_FIELD_CS);
+#pragma unroll
+    for (int s = 0; s < NC; s++) CS[s * g.n3 + c] = cs[s];
+    store_prims(F, g, c, cs, o);
 }
 
 // Persistent chemistry kernel: every lane pulls cells from the compacted list through a
@@ -1524,8 +1545,10 @@ __global__ void k_chem_reset(Scalars* sc, double* dtm_slot, double dtm, int use_
     sc->coop_head = 0;
     sc->chem_head = 0;
     sc->chem_active = 0;
-    sc->chem_substeps = 0;
-    sc->chem_nmax = 0;
+    for (int k = 0; k < STAT_SLOTS; k++) {
+        sc->stat[k][0] = 0;
+        sc->stat[k][1] = 0;
+    }
     *dtm_slot = use_dt ? sc->dt : dtm;
 }
 
@@ -1774,6 +1797,7 @@ hipError_t launch_trap_adaptive(eu3d_ctx* c, int nchem_max, bool predict_only, i
     else
         k_trap_first<false, false><<<gr, bl, 0, c->stream>>>(G(c), c->fl, c->quirks, c->sc, c->sc, c->active, nmax, po,
                                                              nthr, T_cold, slist, scount, rt);
+    if (adv) k_adv_nochem<<<gr, bl, 0, c->stream>>>(G(c), c->fl, c->quirks);
     if (!predict_only) {
         GridCells acc{c->g, c->fl};
         launch_persist(c, acc, c->active, slot);
