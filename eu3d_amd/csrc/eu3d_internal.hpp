@@ -99,7 +99,10 @@ hipError_t launch_check_finite(eu3d_ctx* c);
 hipError_t ensure_coop(eu3d_ctx* c, long long n);
 hipError_t launch_set_scalars(eu3d_ctx* c, double dt, double time);
 hipError_t launch_step(eu3d_ctx* c, const eu3d_step_params* p, bool first);
-hipError_t launch_surrogate_grid(eu3d_ctx* c, int nchem_max);
+// adv: the step's Update_after_Adv + Update_after_CS have not run yet (fused into the passes)
+hipError_t launch_surrogate_grid(eu3d_ctx* c, int nchem_max, bool adv = false);
+// Update_after_Adv + Update_after_CS of the listed cells (count on the device)
+hipError_t launch_adv_list(eu3d_ctx* c, const unsigned* list, const unsigned* count, unsigned max_count);
 hipError_t surrogate_grid_stats(eu3d_ctx* c);
 hipError_t launch_trap_list(eu3d_ctx* c, const unsigned* list, const unsigned* count, unsigned max_count,
                             unsigned* work, int nchem_max);

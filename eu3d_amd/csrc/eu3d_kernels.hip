@@ -1039,13 +1039,9 @@ k_trap_first(G3 g, Fields F, uint32_t q, const Scalars* __restrict__ scc, Scalar
     if (nthr > 0) wave_append(slist, scount, to_net, (unsigned)c);
 }
 
-// After the fused prediction pass: Update_after_Adv + Update_after_CS for the cells whose
-// prediction failed (Nchem <= 0), which that pass did not store.
-__global__ void __launch_bounds__(256) k_adv_nochem(G3 g, Fields F, uint32_t q) {
-    int i, j, k;
-    if (!interior_ijk(g, i, j, k)) return;
-    const size_t c = IDX(g, i, j, k);
-    if (FLD(F, EU3D_FIELD_NCHEM)[c] > 0) return;
+// Update_after_Adv (CS += RHS; REF quirk F1 adds the species only) and Update_after_CS of one
+// cell, stored: for the cells the fused prediction pass leaves unstored.
+__device__ __forceinline__ void adv_update_cell(const G3& g, const Fields& F, uint32_t q, size_t c) {
     double cs[NC];
     load_cs(F, g, c, cs);
     const double* R = FLD(F, EU3D_FIELD_RHS);
@@ -1056,6 +1052,24 @@ __global__ void __launch_bounds__(256) k_adv_nochem(G3 g, Fields F, uint32_t q) 
 #pragma unroll
     for (int s = 0; s < NC; s++) CS[s * g.n3 + c] = cs[s];
     store_prims(F, g, c, cs, o);
+}
+
+// After the fused prediction pass: the cells whose prediction failed (Nchem <= 0).
+__global__ void __launch_bounds__(256) k_adv_nochem(G3 g, Fields F, uint32_t q) {
+    int i, j, k;
+    if (!interior_ijk(g, i, j, k)) return;
+    const size_t c = IDX(g, i, j, k);
+    if (FLD(F, EU3D_FIELD_NCHEM)[c] > 0) return;
+    adv_update_cell(g, F, q, c);
+}
+
+// After the fused prediction pass with surrogate routing: the cells listed for the network
+// (count on the device), grid-stride.
+__global__ void __launch_bounds__(256) k_adv_list(G3 g, Fields F, uint32_t q, const unsigned* __restrict__ list,
+                                                  const unsigned* __restrict__ count) {
+    const unsigned n = *count;
+    for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x)
+        adv_update_cell(g, F, q, list[t]);
 }
 
 // Persistent chemistry kernel: every lane pulls cells from the compacted list through a
@@ -1739,6 +1753,12 @@ hipError_t launch_set_scalars(eu3d_ctx* c, double dt, double time) {
     return hipGetLastError();
 }
 
+hipError_t launch_adv_list(eu3d_ctx* c, const unsigned* list, const unsigned* count, unsigned max_count) {
+    const unsigned nb = std::min<unsigned>((max_count + 255) / 256, (unsigned)c->num_cu * 8);
+    if (nb) k_adv_list<<<nb, 256, 0, c->stream>>>(G(c), c->fl, c->quirks, list, count);
+    return hipGetLastError();
+}
+
 hipError_t launch_update_after_adv(eu3d_ctx* c) {
     k_add_rhs<<<interior_grid(c->g), dim3(BX, BY), 0, c->stream>>>(G(c), c->fl, (c->quirks & Q_F1) ? NS : NC);
     return launch_update_after_cs(c);
@@ -1928,10 +1948,7 @@ hipError_t launch_step(eu3d_ctx* c, const eu3d_step_params* p, bool first) {
     case EU3D_CHEM_TRAP_ADAPTIVE:   // Update_after_Adv fused into the prediction pass
         e = launch_trap_adaptive(c, p->nchem_max, false, 0, 0.0, nullptr, nullptr, true);
         break;
-    case EU3D_CHEM_SURROGATE:
-        if ((e = launch_update_after_adv(c)) != hipSuccess) return e;
-        e = launch_surrogate_grid(c, p->nchem_max);
-        break;
+    case EU3D_CHEM_SURROGATE: e = launch_surrogate_grid(c, p->nchem_max, true); break;
     default: e = launch_update_after_adv(c);
     }
     return e;

@@ -1197,7 +1197,7 @@ int eu3d_get_surrogate_stats(eu3d_ctx* c, eu3d_surrogate_stats* st) {
 namespace eu3d {
 // Surrogate chemistry over the interior of a grid context (the chemistry half of a step with
 // EU3D_CHEM_SURROGATE): features, network, apply with checks, FP64 fallback on the rejects.
-hipError_t launch_surrogate_grid(eu3d_ctx* c, int nchem_max) {
+hipError_t launch_surrogate_grid(eu3d_ctx* c, int nchem_max, bool adv) {
     eu3d_mlp* m = c->mlp;
     const long long n = (long long)c->g.ni * c->g.nj * c->g.nk;
     if (!m || ensure_infer(m, (int)n) != hipSuccess || sur_ensure(c, n) != EU3D_OK) return hipErrorInvalidValue;
@@ -1205,14 +1205,21 @@ hipError_t launch_surrogate_grid(eu3d_ctx* c, int nchem_max) {
     const eu3d_surrogate_params& sp = m->sp;
     k_zero_u32<<<1, 64, 0, c->stream>>>(b.cnt, 64);
     if (sp.nchem_threshold <= 0) {   // every cell through the network
+        if (adv) {
+            hipError_t e = launch_update_after_adv(c);
+            if (e != hipSuccess) return e;
+        }
         const dim3 grid((c->g.ni + 63) / 64, (c->g.nj + 3) / 4, c->g.nk), blk(64, 4);
         k_sur_feat_grid<<<grid, blk, 0, c->stream>>>(sp, c->g, c->fl, c->sc, m->ifeat);
         hipError_t e = forward_fused(m, (int)n, m->ifeat, m->iout, c->stream);
         if (e != hipSuccess) return e;
         k_sur_apply_grid<<<grid, blk, 0, c->stream>>>(sp, c->g, c->fl, c->sc, m->iout, c->quirks, b.cnt, b.rej);
     } else {   // stiff cells through the network, the rest through the FP64 sub-steps
-        hipError_t e = launch_trap_adaptive(c, nchem_max, false, sp.nchem_threshold, sp.T_cold, b.net, b.cnt + CNT_NET);
+        hipError_t e = launch_trap_adaptive(c, nchem_max, false, sp.nchem_threshold, sp.T_cold, b.net, b.cnt + CNT_NET,
+                                            adv);
         if (e != hipSuccess) return e;
+        // the network cells' state after Update_after_CS, which the fused pass does not store
+        if (adv && (e = launch_adv_list(c, b.net, b.cnt + CNT_NET, (unsigned)n)) != hipSuccess) return e;
         const unsigned nb = std::min<unsigned>((unsigned)((n + 255) / 256), (unsigned)c->num_cu * 8);
         k_sur_feat_grid_list<<<nb, 256, 0, c->stream>>>(sp, c->g, c->fl, c->sc, b.net, b.cnt + CNT_NET, m->ifeat);
         if ((e = forward_fused(m, (int)n, m->ifeat, m->iout, c->stream, b.cnt + CNT_NET)) != hipSuccess) return e;
