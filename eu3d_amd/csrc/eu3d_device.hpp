@@ -136,6 +136,37 @@ __device__ __forceinline__ MixFit mix_fit(const double (&Y)[NS], bool c4) {
     return m;
 }
 
+// One range of the mixture fit, in mix_fit's order: c = sum_s w_s (a1..a7, b1 of s).
+__device__ __forceinline__ void mix_fit_range(const double (&Y)[NS], bool c4, bool lo, double (&c)[8]) {
+#pragma unroll
+    for (int k = 0; k < 8; k++) c[k] = 0.0;
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        const double w = Y[s] * RI(s);
+#pragma unroll
+        for (int k = 0; k < 7; k++) c[k] += w * (lo ? C0[k][s] : C1[k][s]);
+        c[7] += w * ((lo || c4) ? C0[7][s] : C1[7][s]);
+    }
+}
+
+// The mixture fit formed per range on demand for the whole wavefront: a range is summed only
+// when some lane of the wavefront needs it (with the usual wavefront on one side of 1000 K, half
+// of mix_fit's work). The values equal mix_fit's.
+struct LazyMix {
+    MixFit m;
+    bool have_lo = false, have_hi = false;
+    __device__ __forceinline__ void need(const double (&Y)[NS], bool c4, bool lo) {
+        if (!have_lo && __ballot(lo) != 0) {
+            mix_fit_range(Y, c4, true, m.lo);
+            have_lo = true;
+        }
+        if (!have_hi && __ballot(!lo) != 0) {
+            mix_fit_range(Y, c4, false, m.hi);
+            have_hi = true;
+        }
+    }
+};
+
 // T from internal energy (Flowfield::Get_temp, src/Flowfield.cpp:1542-1566): Newton on
 // sum_s Y_s e_s(T) = E, at most 10 iterations, stop when |dT| < 1e-6 K. E in J/kg. p holds the
 // powers of the starting temperature (callers that already have them pass them in).
@@ -164,12 +195,43 @@ __device__ __forceinline__ double newton_T_from(const MixFit& m, TPow p, double 
 __device__ __forceinline__ double newton_T(const MixFit& m, double T, double E, int* iters = nullptr) {
     return newton_T_from(m, tpow(T), E, iters);
 }
+// The same Newton on a LazyMix of the mass fractions Y.
+__device__ __forceinline__ double newton_T_lazy(LazyMix& L, const double (&Y)[NS], bool c4, TPow p, double E,
+                                                int* iters = nullptr) {
+    double Tn = p.T;
+    const double Ek = E * 1e-3;
+    int it = 0;
+    for (int count = 0; count < 10; count++) {
+        if (count > 0) p = tpow(Tn);
+        const double T0 = p.T;
+        const bool lo = T0 < 1000;
+        L.need(Y, c4, lo);
+        const double* c = lo ? L.m.lo : L.m.hi;
+        const double h = -c[0] * p.Tm1 + c[1] * p.lnT + c[2] * p.T + c[3] * p.T2 / 2 + div_const<3>(c[4] * p.T3) +
+                         c[5] * p.T4 / 4 + div_const<5>(c[6] * p.T5) + c[7];
+        const double cp = c[0] * p.Tm2 + c[1] * p.Tm1 + c[2] + c[3] * p.T + c[4] * p.T2 + c[5] * p.T3 +
+                          c[6] * p.T4;
+        const double f = (h - Ek) - L.m.rmix * T0;
+        const double fp = cp - L.m.rmix;
+        Tn = T0 - f / fp;
+        it++;
+        if (fabs(Tn - T0) < 1e-6) break;
+    }
+    if (iters) *iters = it;
+    return Tn;
+}
 
-// c_p of the mixture [kJ/kg/K] from the mixture fit.
-__device__ __forceinline__ double mix_cp(const MixFit& m, double T) {
-    const TPow p = tpow(T);
-    const double* c = (T < 1000) ? m.lo : m.hi;
+// c_p of the mixture [kJ/kg/K] from the mixture fit, at p.T.
+__device__ __forceinline__ double mix_cp_p(const MixFit& m, const TPow& p) {
+    const double* c = (p.T < 1000) ? m.lo : m.hi;
     return c[0] * p.Tm2 + c[1] * p.Tm1 + c[2] + c[3] * p.T + c[4] * p.T2 + c[5] * p.T3 + c[6] * p.T4;
+}
+__device__ __forceinline__ double mix_cp(const MixFit& m, double T) { return mix_cp_p(m, tpow(T)); }
+// c_p of the mixture of mass fractions Y at p.T, summing only the ranges the wavefront needs.
+__device__ __forceinline__ double mix_cp_Y(const double (&Y)[NS], bool c4, const TPow& p) {
+    LazyMix L;
+    L.need(Y, c4, p.T < 1000);
+    return mix_cp_p(L.m, p);
 }
 
 // Primitive state of one cell, as Flowfield::Update_after_CS leaves it (src/Flowfield.cpp:1246-1284).
@@ -179,7 +241,11 @@ struct Prim {
 };
 
 // Update_after_CS for one cell: CS[NC], previous T (as its powers p_old) and E -> primitives.
-// Quirk F3 solves T from the previous E; FIXED recomputes E from CS first.
+// Quirk F3 solves T from the previous E; FIXED recomputes E from CS first. LAZY forms the
+// mixture fit per range on demand (LazyMix): it pays where a wavefront's cells are neighbours in
+// space and mostly on one side of 1000 K (the grid step), not where they are unrelated cells of
+// a work queue (there both ranges are usually needed and the branches cost more than they save).
+template <bool LAZY = false>
 __device__ __forceinline__ Prim prim_from_cs_p(const double (&cs)[NC], const TPow& p_old, double E_old, uint32_t q,
                                                 int* iters = nullptr) {
     Prim o;
@@ -194,8 +260,20 @@ __device__ __forceinline__ Prim prim_from_cs_p(const double (&cs)[NC], const TPo
     o.V = cs[NS + 1] * rD;
     o.W = cs[NS + 2] * rD;
     o.E = (cs[NS + 3] - 0.5 * D * (o.U * o.U + o.V * o.V + o.W * o.W)) * rD;
-    const MixFit m = mix_fit(o.Y, q & Q_C4);
-    o.T = newton_T_from(m, p_old, (q & Q_F3) ? E_old : o.E, iters);
+    double cp;
+    if constexpr (LAZY) {
+        LazyMix L;
+        L.m.rmix = 0.0;
+#pragma unroll
+        for (int s = 0; s < NS; s++) L.m.rmix += o.Y[s] * RI(s);
+        o.T = newton_T_lazy(L, o.Y, q & Q_C4, p_old, (q & Q_F3) ? E_old : o.E, iters);
+        L.need(o.Y, q & Q_C4, o.T < 1000);
+        cp = mix_cp(L.m, o.T);
+    } else {
+        const MixFit m = mix_fit(o.Y, q & Q_C4);
+        o.T = newton_T_from(m, p_old, (q & Q_F3) ? E_old : o.E, iters);
+        cp = mix_cp(m, o.T);
+    }
     double sy = 0;
 #pragma unroll
     for (int s = 0; s < NS; s++) sy += o.Y[s] * RMW(s);
@@ -203,7 +281,7 @@ __device__ __forceinline__ Prim prim_from_cs_p(const double (&cs)[NC], const TPo
     o.Rgas = R_UNIV * 1000 / o.Wav;
     o.P = o.T * D * o.Rgas;
     o.H = o.P * rD + o.E;
-    o.Cp = mix_cp(m, o.T) * 1000;
+    o.Cp = cp * 1000;
     o.Gamma = o.Cp / (o.Cp - R_UNIV * sy * 1000);
     o.C = sqrt(o.Gamma * o.P * rD);
     return o;

@@ -612,7 +612,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
                 Dr[s] = rho[s] * 1e-3;
             }
             o.Rgas = R_UNIV * 1000 * sy;
-            o.Cp = mix_cp(mix_fit(Y, q & Q_C4), T0) * 1000;
+            o.Cp = mix_cp_Y(Y, q & Q_C4, p) * 1000;
             double gx[NS], hr[NS], pt[NS];
             species_thermo<true>(p, q & Q_C4, gx, hr);
             partial_T_h(o, q, hr, pt);
@@ -633,7 +633,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
         for (int s = 0; s < NS; s++) cs[s] = cs[s] + (R[s * g.n3 + c] + ko.w[s] * 1e3 * dt) / (1.0 - dt * ko.md[s]);
 #pragma unroll
         for (int s = NS; s < NC; s++) cs[s] = cs[s] + R[s * g.n3 + c];
-        const Prim n = prim_from_cs_p(cs, p, E0, q);
+        const Prim n = prim_from_cs_p<true>(cs, p, E0, q);
         double* CS = FLD(F, EU3D_FIELD_CS);
 #pragma unroll
         for (int s = 0; s < NC; s++) CS[s * g.n3 + c] = cs[s];
@@ -680,7 +680,7 @@ __global__ void __launch_bounds__(256) k_derive(G3 g, Fields F, uint32_t q) {
     o.T = FLD(F, EU3D_FIELD_T)[c];
     o.Wav = 1.0 / sy;
     o.Rgas = R_UNIV * 1000 / o.Wav;
-    o.Cp = mix_cp(mix_fit(Y, q & Q_C4), o.T) * 1000;
+    o.Cp = mix_cp_Y(Y, q & Q_C4, tpow(o.T)) * 1000;
     FLD(F, EU3D_FIELD_WAV)[c] = o.Wav;
     FLD(F, EU3D_FIELD_RGAS)[c] = o.Rgas;
     FLD(F, EU3D_FIELD_CP)[c] = o.Cp;
