@@ -585,23 +585,26 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
     double v = 1e10;
     if (interior_ijk(g, i, j, k)) {
         const size_t c = IDX(g, i, j, k);
-        const double* CS0 = FLD(F, EU3D_FIELD_CS);
         const double T0 = FLD(F, EU3D_FIELD_T)[c], E0 = FLD(F, EU3D_FIELD_E)[c];
         const TPow p = tpow(T0);
         KinOut ko;
+        // the conserved state stays in registers through the kinetics (226 VGPRs, still two waves
+        // per SIMD): a second read after the kinetics missed L2 often enough to cost 2 %
+        double cs[NC];
+        load_cs(F, g, c, cs);
         {
             double rho[NS], D = 0;
 #pragma unroll
             for (int s = 0; s < NS; s++) {
-                rho[s] = CS0[s * g.n3 + c];
+                rho[s] = cs[s];
                 D += rho[s];
             }
             Prim o;
             o.D = D;
             const double rD = 1.0 / D;
-            o.U = CS0[NS * g.n3 + c] * rD;
-            o.V = CS0[(NS + 1) * g.n3 + c] * rD;
-            o.W = CS0[(NS + 2) * g.n3 + c] * rD;
+            o.U = cs[NS] * rD;
+            o.V = cs[NS + 1] * rD;
+            o.W = cs[NS + 2] * rD;
             o.T = T0;
             double Y[NS], Mc[NS], Dr[NS], sy = 0;
 #pragma unroll
@@ -622,11 +625,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
             kinetics_g<false, true>(p, gx, Mc, Dr, pt, 2, ko);
         }
         __builtin_amdgcn_sched_barrier(0);
-        // the conserved state is read again here rather than held in registers through the
-        // kinetics (its second read mostly hits L2)
-        asm volatile("" ::: "memory");
-        double cs[NC];
-        load_cs(F, g, c, cs);
         const double dt = sc->dt;
         const double* R = FLD(F, EU3D_FIELD_RHS);
 #pragma unroll
