@@ -48,6 +48,28 @@ __device__ __forceinline__ TPow tpow(double T) {
     return p;
 }
 
+// Powers of T from those of a nearby temperature p.T (|T / p.T - 1| < 2^-12), without a log or a
+// division: ln T = ln p.T + log1p(x), x = (T - p.T) / p.T, by its series to x^4 (truncation below
+// 1e-17 relative), and 1/T by two Newton steps from 1/p.T. Each agrees with tpow(T) to about an
+// ulp; farther temperatures take tpow.
+__device__ __forceinline__ TPow tpow_near(const TPow& p, double T) {
+    const double x = (T - p.T) * p.Tm1;
+    if (!(fabs(x) < 0x1p-12)) return tpow(T);
+    TPow q;
+    q.T = T;
+    double r = p.Tm1;
+    r = __builtin_fma(r, __builtin_fma(-T, r, 1.0), r);
+    r = __builtin_fma(r, __builtin_fma(-T, r, 1.0), r);
+    q.Tm1 = r;
+    q.Tm2 = r * r;
+    q.T2 = T * T;
+    q.T3 = q.T2 * T;
+    q.T4 = q.T2 * q.T2;
+    q.T5 = q.T4 * T;
+    q.lnT = p.lnT + x * (1.0 + x * (-0.5 + x * (1.0 / 3 - 0.25 * x)));
+    return q;
+}
+
 // c_p/R of species s (Reaction::GetCpi, src/Reaction.cpp:893-904); range by T < 1000.
 __device__ __forceinline__ double cp_R(int s, bool lo, const TPow& p) {
     const double a0 = lo ? C0[0][s] : C1[0][s], a1 = lo ? C0[1][s] : C1[1][s];

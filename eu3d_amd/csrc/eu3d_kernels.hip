@@ -1187,9 +1187,12 @@ struct TailLds {
 };
 
 // Newton on the mixture energy (Flowfield::Get_temp, src/Flowfield.cpp:1542-1566), newton_T's
-// arithmetic with the mixture fit in LDS and the powers of the starting T given.
-__device__ __forceinline__ double tail_newton(const TailLds& S, TPow p, double E) {
+// arithmetic with the mixture fit in LDS and the powers of the starting T given; the iterates'
+// powers come from the start's by tpow_near (anchored to the start, so nothing accumulates
+// over a chain of sub-steps).
+__device__ __forceinline__ double tail_newton(const TailLds& S, const TPow& p0, double E) {
     const double Ek = E * 1e-3;
+    TPow p = p0;
     double Tn = p.T;
     for (int count = 0; count < 10; count++) {
         const double T0 = p.T;
@@ -1202,7 +1205,7 @@ __device__ __forceinline__ double tail_newton(const TailLds& S, TPow p, double E
         const double fp = cp - S.rmix;
         Tn = T0 - f / fp;
         if (fabs(Tn - T0) < 1e-6) break;
-        p = tpow(Tn);
+        p = tpow_near(p0, Tn);
     }
     return Tn;
 }
@@ -1291,11 +1294,13 @@ __global__ void __launch_bounds__(64) k_chem_tail(Acc acc, const unsigned* __res
         if (!have) continue;
         const TPow p = tpow(T);
         const bool lo = T < 1000;
-        // species lanes: reaction units and G/RT (Reaction::GetHi/GetSi, src/Reaction.cpp:906-948)
-        double Dr = 0.0;
-        if (hl < NS) {
-            S.mc[hl] = (di * rmw * 1000) * 1e-6;
-            Dr = di * 1e-3;
+        // species lanes: reaction units and G/RT (Reaction::GetHi/GetSi, src/Reaction.cpp:906-948);
+        // reaction lanes: k_f. Every lane evaluates both (lanes past NS or NR on species or reaction
+        // 0, stored nowhere) so that the two independent chains share one basic block.
+        const double Dr = di * 1e-3;
+        const double mcs = (di * rmw * 1000) * 1e-6;
+        double gxs;
+        {
             double a[9];
 #pragma unroll
             for (int k = 0; k < 9; k++) a[k] = lo ? clo[k] : chi[k];
@@ -1303,19 +1308,26 @@ __global__ void __launch_bounds__(64) k_chem_tail(Acc acc, const unsigned* __res
                              div_const<3>(a[4] * p.T3) + a[5] * p.T4 / 4 + div_const<5>(a[6] * p.T5) + a[7];
             const double sr = -a[0] * p.Tm2 / 2 - a[1] * p.Tm1 + a[2] * p.lnT + a[3] * p.T + a[4] * p.T2 / 2 +
                               div_const<3>(a[5] * p.T3) + a[6] * p.T4 / 4 + a[8];
-            S.gx[hl] = h * p.Tm1 - sr;
+            gxs = h * p.Tm1 - sr;
         }
-        __syncthreads();
-        // reaction lanes: rates and the per-species terms (src/Reaction.cpp:548-585)
-        if (hl < NR) {
-            const double RT = R_UNIV * T;
-            constexpr double P0R = P0_PA * 1e-6;
+        const double RT = R_UNIV * T;
+        constexpr double P0R = P0_PA * 1e-6;
+        double kf;
+        {
             const double rRuT = 1.0 / (RU_CAL * T);
             const double tfac = tmode == 1 ? T : tmode == 2 ? p.Tm1 : tmode == 3 ? p.Tm2 : 1.0;
             const double ex = (b_exp ? bf * p.lnT : 0.0) - eaf * rRuT;
             const double ekf = exp(ex);
-            double kf = af * tfac;
+            kf = af * tfac;
             if (use_exp) kf *= ekf;
+        }
+        if (hl < NS) {
+            S.mc[hl] = mcs;
+            S.gx[hl] = gxs;
+        }
+        __syncthreads();
+        // reaction lanes: rates and the per-species terms (src/Reaction.cpp:548-585)
+        if (hl < NR) {
             double mc[NS];
 #pragma unroll
             for (int s = 0; s < NS; s++) mc[s] = S.mc[s];
