@@ -1179,8 +1179,9 @@ __constant__ RxnSparse RXS = make_rxn_sparse();
 struct TailLds {
     double mc[NS + 1];               // reaction units [mol/cm^3]; slot NS pads the sparse lists
     double gx[NS + 1];               // G_s / (R T)
-    double part[3][NS + 1][NR];      // per-reaction terms of w, Q, P; row NS is the sink
-    double fin[3 * NS];              // w, Q, P after the sums and scaling
+    double pw[NS + 1][NR];           // per-reaction terms of w; row NS is the sink
+    double2 pqp[NS + 1][NR];         // per-reaction terms of Q and P, side by side
+    double fin[NS];                  // w of the last sub-step, scaled
     double cs[NS];                   // updated partial densities
     double mix[2][8];                // mixture fit, low and high range: a1..a7, b1
     double rmix;                     // sum_s Y_s R_s
@@ -1245,11 +1246,13 @@ __global__ void __launch_bounds__(64) k_chem_tail(Acc acc, const unsigned* __res
         nbj[j] = RXS.nb[rR][j];
     }
     const int xr = RXS.xr[rR];
-    const int qt = hl / NS, qs = hl - (hl / NS) * NS;   // sum lane: quantity, species
-    const double mwq = MW[qs];
+    const double mws = MW[sS];
     const double rmw = RMW(sS);
     const int kd = kM & 7;
-    for (int i = hl; i < 3 * (NS + 1) * NR; i += 32) (&S.part[0][0][0])[i] = 0.0;
+    for (int i = hl; i < (NS + 1) * NR; i += 32) {
+        (&S.pw[0][0])[i] = 0.0;
+        (&S.pqp[0][0])[i] = make_double2(0.0, 0.0);
+    }
     if (hl == 0) {
         S.mc[NS] = 1.0;
         S.gx[NS] = 0.0;
@@ -1355,36 +1358,33 @@ __global__ void __launch_bounds__(64) k_chem_tail(Acc acc, const unsigned* __res
                 const double qb = nf ? (double)nf * kb * rb : 0.0;
                 const double pb = nb ? (double)nb * kb * rb : 0.0;
                 const double pf = nf ? (double)nf * kf * rf : 0.0;
-                S.part[0][s][hl] = dn != 0 ? (double)dn * tb * rr : 0.0;
-                if (C3) {
-                    S.part[1][s][hl] = qf + qb * tb;
-                    S.part[2][s][hl] = pb + pf * tb;
-                } else {
-                    S.part[1][s][hl] = (qf + qb) * tb;
-                    S.part[2][s][hl] = (pb + pf) * tb;
-                }
+                S.pw[s][hl] = dn != 0 ? (double)dn * tb * rr : 0.0;
+                if (C3) S.pqp[s][hl] = make_double2(qf + qb * tb, pb + pf * tb);
+                else S.pqp[s][hl] = make_double2((qf + qb) * tb, (pb + pf) * tb);
             }
         }
         __syncthreads();
-        // sum lanes: w, Q, P of one species over the reactions, in reaction order
-        if (hl < 3 * NS) {
-            const double* row = &S.part[qt][qs][0];
-            double a = 0.0;
-#pragma unroll
-            for (int r = 0; r < NR; r++) a += row[r];
-            if (qt < 2) a *= mwq;
-            else {
-                const double m = S.mc[qs];
-                a = (m == 0) ? 0.0 : a / m;
-            }
-            S.fin[hl] = a;
-        }
-        __syncthreads();
-        // species lanes: point-implicit trapezoid update (src/Reaction.cpp:481,587)
+        // species lanes: Q and P of the species summed over the reactions in reaction order (two
+        // chains side by side), then the point-implicit trapezoid update (src/Reaction.cpp:481,587);
+        // w only on the last sub-step, for the store
         if (hl < NS) {
-            const double Qs = S.fin[NS + hl], Ps = S.fin[2 * NS + hl];
+            double Qs = 0.0, Ps = 0.0;
+#pragma unroll
+            for (int r = 0; r < NR; r++) {
+                const double2 v = S.pqp[hl][r];
+                Qs += v.x;
+                Ps += v.y;
+            }
+            Qs *= mws;
+            Ps = (mcs == 0) ? 0.0 : Ps / mcs;
             di = ((1.0 - dt / 2.0 * Ps) * Dr + dt * Qs) / (1.0 + dt / 2.0 * Ps) * 1e3;
             S.cs[hl] = di;
+            if (rem == 1) {
+                double w = 0.0;
+#pragma unroll
+                for (int r = 0; r < NR; r++) w += S.pw[hl][r];
+                S.fin[hl] = w * mws;
+            }
         }
         __syncthreads();
         // every lane: the conserved state (Flowfield::Explicit for the cell)
