@@ -126,14 +126,14 @@ def _states(n, seed=11):  # device copies of configs.chem_states
             torch.tensor(E, device="cuda"), rho, T, E)
 
 
-def _oracle_batch(rho, T, E, dtm):
+def _oracle_batch(rho, T, E, dtm, quirks=0):
     """The oracle's Euler::Trapezoid(type 1) cell loop on states at rest, laid out along x."""
     from eu3d_amd import configs
     from oracle import OracleFlow
     n = T.shape[0]
     x = configs.make_nodes(n, 1e-4)
     y = configs.make_nodes(1, 1e-4)
-    o = OracleFlow(x, y, y, quirks=0)
+    o = OracleFlow(x, y, y, quirks=quirks)
     sh = o.shape3
     cs = np.zeros(sh + (13,))
     cs[..., 0:9] = rho[0]            # ghosts: any valid state
@@ -152,11 +152,17 @@ def _oracle_batch(rho, T, E, dtm):
     return T_in, E_in, o.get("CS")[2:-2, 2, 2, :9], o.get("T")[2:-2, 2, 2], o.get("NCHEM")[2:-2, 2, 2]
 
 
-@pytest.mark.parametrize("dtm,tail", [(1e-8, 512), (7e-6, 512), (7e-6, 1)])
-def test_chem_batch_matches_oracle(dtm, tail):
+Q_C3 = 1 << 7   # the reference's trapezoid production/destruction grouping (include/eu3d.h)
+
+
+@pytest.mark.parametrize("dtm,tail,quirks", [(1e-8, 512, 0), (7e-6, 512, 0), (7e-6, 1, 0), (7e-6, 512, Q_C3),
+                                             (7e-6, 1, Q_C3)])
+def test_chem_batch_matches_oracle(dtm, tail, quirks):
     """Batched FP64 chemistry (compaction + persistent kernel) equals the oracle's
     Euler::Trapezoid(type 1) cell loop on the same states (7e-6 s: up to 4 sub-steps). With
-    tail = 1 every multi-sub-step cell runs in the tail kernel instead.
+    tail = 1 every multi-sub-step cell runs in the tail kernel instead. Quirk C3 alone (the
+    reference's grouping of the trapezoid's production and destruction terms, without the C2
+    third-body quirk that would zero the rates) runs the other instance of both kernels.
 
     The bound is 1e-10 relative, or four times the problem's own conditioning where that is
     larger: at 7e-6 s (700 times the configured chemistry step) a 1-ulp change of the inputs moves
@@ -165,18 +171,22 @@ def test_chem_batch_matches_oracle(dtm, tail):
     torch = _t()
     from eu3d_amd import configs
     from eu3d_amd.api import Context
-    say("case chem batch", dtm)
+    say("case chem batch", dtm, "tail", tail, "quirks", quirks)
     n = 3000
     rho, T, E, _ = configs.chem_states(n, seed=11)
-    T_in, E_in, ro, To, nch_o = _oracle_batch(rho, T, E, dtm)
-    ro_p = _oracle_batch(rho * (1 + 2.0 ** -52), T, E, dtm)[2]
-    cond = np.abs(ro_p - ro).max() / ro.sum(1).max()
+    T_in, E_in, ro, To, nch_o = _oracle_batch(rho, T, E, dtm, quirks)
+    ro_p = _oracle_batch(rho * (1 + 2.0 ** -52), T, E, dtm, quirks)[2]
+    # under quirk C3 a few cells leave the finite range at this dt in the oracle too: the
+    # non-finite pattern must coincide and the bound applies to the rest
+    fin = np.isfinite(ro).all(1) & np.isfinite(ro_p).all(1) & np.isfinite(To)
+    scale = ro[fin].sum(1).max()
+    cond = np.abs(ro_p[fin] - ro[fin]).max() / scale
     tol = max(1e-10, 4 * cond)
     r = torch.tensor(rho.T.copy(), device="cuda")
     t = torch.tensor(T_in, device="cuda")
     e = torch.tensor(E_in, device="cuda")
     x = configs.make_nodes(8, 1e-4)
-    g = Context(x, x, x, quirks=0)
+    g = Context(x, x, x, quirks=quirks)
     g.set_schedule(256, 4, 2, tail)
     nch = torch.empty(n, dtype=torch.int32, device="cuda")
     st = g.chem_batch(n, r.data_ptr(), t.data_ptr(), e.data_ptr(), dtm, 10000, nch.data_ptr())
@@ -184,11 +194,13 @@ def test_chem_batch_matches_oracle(dtm, tail):
     assert (nch.cpu().numpy() == nch_o).all()
     assert st["active_cells"] == int((nch_o > 1).sum()) and st["substeps_total"] == int(nch_o.sum())
     rg = r.cpu().numpy().T
-    err = np.abs(rg - ro).max() / ro.sum(1).max()
+    assert (np.isfinite(rg).all(1) == np.isfinite(ro).all(1)).all()
+    assert (np.isfinite(t.cpu().numpy()) == np.isfinite(To)).all()
+    err = np.abs(rg[fin] - ro[fin]).max() / scale
     say("  active", st["active_cells"], "substeps", st["substeps_total"], "max nchem", st["nchem_max_seen"],
         "rel err", err, "conditioning", cond)
     assert err < tol
-    assert np.abs(t.cpu().numpy() - To).max() < 1e-10 * 3000
+    assert np.abs(t.cpu().numpy()[fin] - To[fin]).max() < 1e-10 * 3000
 
 
 def test_surrogate_full_fallback_equals_fp64():
