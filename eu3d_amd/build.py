@@ -49,9 +49,9 @@ def build_host():
     host = os.path.join(HERE, "host")
     flags = ["-std=c++17", "-O2", "-Wall", "-Wextra", "-fPIC"]
     run([CXX, *flags, "-shared", "-o", HOST_LIB + ".tmp", os.path.join(host, "eu3d_host.cpp"),
-         "-L", HERE, "-leu3d", "-Wl,-rpath,$ORIGIN"])
+         os.path.join(host, "eu3d_run.cpp"), "-L", HERE, "-leu3d", "-Wl,-rpath,$ORIGIN"])
     os.replace(HOST_LIB + ".tmp", HOST_LIB)
-    run([CXX, *flags, "-o", HOST_RUN + ".tmp", os.path.join(host, "eu3d_run.cpp"),
+    run([CXX, *flags, "-o", HOST_RUN + ".tmp", os.path.join(host, "eu3d_run_main.cpp"),
          "-L", HERE, "-leu3d_host", "-leu3d", "-Wl,-rpath,$ORIGIN"])
     os.replace(HOST_RUN + ".tmp", HOST_RUN)
 

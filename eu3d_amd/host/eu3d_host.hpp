@@ -218,3 +218,7 @@ void WriteTecplot(const std::string& filename, const Mesh& m, const std::vector<
                   double wait_time);
 
 }  // namespace eu3d_host
+
+// The deck-driven driver of the eu3d_run program, in-process: argv as the program takes it
+// (argv[0] is ignored). Writes its report to stdout; returns the program's exit status.
+extern "C" int eu3d_run_main(int argc, char** argv);

@@ -1,5 +1,6 @@
 // eu3d_run: the reference's driver flow (Euler constructor + Computing) from its control deck,
-// on the GPU through the host layer.
+// on the GPU through the host layer. The flow is eu3d_run_main in libeu3d_host.so, callable
+// in-process with a command line; the eu3d_run program (eu3d_run_main.cpp) only forwards to it.
 //
 //   eu3d_run <ctrl deck> --ic <file> [--bc k0,k1,k2,k3,k4,k5] [--steps N] [--fused RK]
 //            [--quirks ref|fixed] [--out DIR] [--dump FILE]
@@ -65,6 +66,7 @@ static int print_decks(const std::string& ctrl) {
     json_array("mw", me.mw);
     json_array("mr", me.mr, true);
     std::printf("}}\n");
+    std::fflush(stdout);
     return 0;
 }
 
@@ -75,7 +77,7 @@ static std::vector<double> read_block(std::ifstream& in, size_t n, const char* w
     return v;
 }
 
-int main(int argc, char** argv) {
+extern "C" int eu3d_run_main(int argc, char** argv) {
     if (argc < 2) {
         std::fprintf(stderr, "usage: %s <ctrl deck> --ic FILE [--bc k,k,k,k,k,k] [--steps N] [--fused RK] "
                              "[--quirks ref|fixed] [--out DIR] [--dump FILE]\n", argv[0]);
@@ -134,6 +136,7 @@ int main(int argc, char** argv) {
             e.Computing(e.ctrl.count, have_bc ? bc : nullptr, steps);
         }
         std::printf("iterations %d time %.9e dt %.9e\n", e.iteration, e.TwoDim.time(), e.TwoDim.dt());
+        std::fflush(stdout);
         if (!dump.empty()) {
             std::ofstream d(dump, std::ios::binary);
             const std::vector<double> cs = e.TwoDim.field(EU3D_FIELD_CS), t = e.TwoDim.field(EU3D_FIELD_T);

@@ -77,6 +77,8 @@ def load():
     lib.orc_set_num_threads.argtypes = [C.c_int]
     lib.orc_check_div_const.argtypes = [C.c_double, C.c_long, C.c_ulonglong]
     lib.orc_check_div_const.restype = C.c_long
+    lib.orc_check_div_fast.argtypes = [C.c_double, C.c_long, C.c_ulonglong]
+    lib.orc_check_div_fast.restype = C.c_long
     _lib = lib
     return lib
 

@@ -91,6 +91,7 @@ void orc_field_from_prims(orc_flow *f, const orc_mech *m);
 
 int orc_num_threads(void);
 long orc_check_div_const(double d, long n, unsigned long long seed);
+long orc_check_div_fast(double rel_err, long n, unsigned long long seed);
 void orc_set_num_threads(int n);
 
 #ifdef __cplusplus

@@ -1,8 +1,10 @@
 """The C++ host layer (eu3d_amd/host): deck, grid and mechanism readers on the CPU, and the
 deck-driven driver (the reference's Euler flow over the C ABI) against the oracle on the GPU."""
+import ctypes as C
 import json
 import os
 import subprocess
+import tempfile
 
 import numpy as np
 import pytest
@@ -11,6 +13,7 @@ from tests.common import compare, say
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 RUN = os.path.join(ROOT, "eu3d_amd", "eu3d_run")
+HOST_LIB = os.path.join(ROOT, "eu3d_amd", "libeu3d_host.so")
 REF_INPUT = "/root/reference/input"
 
 
@@ -50,10 +53,41 @@ reaction\t{reaction}
     return str(c)
 
 
+def _driver(args):
+    """eu3d_run_main (the eu3d_run program's flow in libeu3d_host.so) called in this process, with
+    its stdout captured: no child process is started from a process whose GPU runtime is active."""
+    lib = C.CDLL(HOST_LIB)
+    lib.eu3d_run_main.argtypes = [C.c_int, C.POINTER(C.c_char_p)]
+    argv = (C.c_char_p * (len(args) + 2))(b"eu3d_run", *[a.encode() for a in args], None)
+    libc = C.CDLL(None)
+    with tempfile.TemporaryFile() as f:
+        libc.fflush(None)
+        saved = os.dup(1)
+        os.dup2(f.fileno(), 1)
+        try:
+            rc = lib.eu3d_run_main(len(args) + 1, argv)
+            libc.fflush(None)
+        finally:
+            os.dup2(saved, 1)
+            os.close(saved)
+        f.seek(0)
+        out = f.read().decode()
+    return rc, out
+
+
 def _print_decks(ctrl):
+    rc, out = _driver([ctrl, "--print-decks"])
+    assert rc == 0, out
+    return json.loads(out)
+
+
+def test_driver_program_prints_decks(tmp_path):
+    """The eu3d_run program forwards its command line to eu3d_run_main (CPU only: parse, no GPU)."""
+    _need_driver()
+    ctrl = _decks(tmp_path, GRID_1D)
     r = subprocess.run([RUN, ctrl, "--print-decks"], capture_output=True, text=True, timeout=60)
     assert r.returncode == 0, r.stderr
-    return json.loads(r.stdout)
+    assert json.loads(r.stdout) == _print_decks(ctrl)
 
 
 GRID_1D = {"total_ni": 160, "xa": 0.0, "xb": 159 * 5e-5, "total_nj": 2, "ya": 0.0, "yb": 5e-5,
@@ -144,10 +178,10 @@ def _zn_prims(ni, n_driver=10):
 
 def _run(ctrl, ic, args, tmp_path):
     dump = str(tmp_path / "dump.bin")
-    cmd = [RUN, ctrl, "--ic", ic, "--out", str(tmp_path), "--dump", dump, *args]
-    say("run", " ".join(cmd[1:]))
-    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
-    assert r.returncode == 0, r.stdout + r.stderr
+    cmd = [ctrl, "--ic", ic, "--out", str(tmp_path), "--dump", dump, *args]
+    say("run", " ".join(cmd))
+    rc, out = _driver(cmd)
+    assert rc == 0, out
     return dump
 
 

@@ -521,3 +521,14 @@ def test_constant_division_is_correctly_rounded(d):
     # the check can fail: x * RN(1/3) alone differs from x / 3 on a sizeable fraction of operands
     x = np.random.default_rng(0).uniform(1.0, 2.0, 100_000)
     assert np.count_nonzero(x * (1.0 / d) != x / d) > 1000
+
+
+def test_fast_division_equals_ieee():
+    """The flux kernels' division (reciprocal estimate, two Newton steps, one residual correction)
+    equals IEEE division on sampled operands even from a reciprocal estimate 1e-4 off; with no
+    Newton step the same estimate misses on almost every pair (the check can fail)."""
+    lib = load()
+    assert lib.orc_check_div_fast(1e-4, 5_000_000, 777) == 0
+    b = np.random.default_rng(1).uniform(1.0, 2.0, 100_000)
+    a = np.random.default_rng(2).uniform(1.0, 2.0, 100_000)
+    assert np.count_nonzero(a * ((1.0 / b) * (1 + 1e-4)) != a / b) > 90_000
