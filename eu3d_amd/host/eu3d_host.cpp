@@ -2,8 +2,10 @@
 // operation goes through the C ABI of libeu3d.so.
 #include "eu3d_host.hpp"
 
+#include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <filesystem>
 #include <fstream>
 #include <iostream>
 #include <sstream>
@@ -412,19 +414,36 @@ Euler::Euler(const Control& c, const Mesh& m, const Mechanism& me, uint32_t quir
 
 void Euler::set_surrogate(eu3d_mlp* m) { ctx.check(eu3d_set_surrogate(ctx.get(), m), "set_surrogate"); }
 
+double Euler::now() {
+    if (record) ctx.check(eu3d_sync(ctx.get()), "sync");
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
 void Euler::Trapezoid(int type) {
+    const double t0 = record ? now() : 0;
     TwoDim.Update_after_Adv();
+    const double t1 = record ? now() : 0;
     if (type == 0) {
         React.Trapezoid(TwoDim);
         TwoDim.Explicit();
     } else {
         React.TrapezoidAdaptive(TwoDim, nchem_max);
     }
+    if (record) {
+        phase.tr1 += t1 - t0;
+        phase.tr2 += now() - t1;   // prediction and sub-steps run as one adaptive pass
+    }
 }
 
 void Euler::IMEX() {
+    const double t0 = record ? now() : 0;
     React.Diagonalized(TwoDim);
+    const double t1 = record ? now() : 0;
     TwoDim.Update_IMEX(React);
+    if (record) {
+        phase.tr1 += t1 - t0;
+        phase.tr2 += now() - t1;
+    }
 }
 
 void Euler::DNN(int type) {
@@ -441,18 +460,30 @@ void Euler::Computing(int count, const int* bc_kinds, int max_steps) {
         if (ctrl.cfl == 0) TwoDim.set_dt(ctrl.dt);
         else TwoDim.CFLcondition(ctrl.cfl, Final_Time);
         const double dt = TwoDim.dt();
+        const double t1 = record ? now() : 0;
         if (time <= num * Final_Time / count && time + dt > num * Final_Time / count) {
             Output(num);
             num++;
         }
+        const double t2 = record ? now() : 0;
         if (bc_kinds) TwoDim.FieldBoundary(bc_kinds);
         else TwoDim.FieldBoundary_3dRSBI();
+        const double t3 = record ? now() : 0;
         TwoDim.Mpi_Boundary();
+        const double t4 = record ? now() : 0;
         TwoDim.Advection(ctrl.TimeAdv_Sch, ctrl.Diff_Sch);
+        const double t5 = record ? now() : 0;
         switch (ctrl.Reaction_Sch) {
         case 0: Trapezoid(ctrl.type); break;
         case 1: IMEX(); break;
         case 2: DNN(ctrl.type); break;
+        }
+        if (record) {
+            phase.output += t2 - t1;
+            phase.boundary += t3 - t2;
+            phase.mpi_boundary += t4 - t3;
+            phase.advection += t5 - t4;
+            phase.reaction += now() - t5;
         }
         time += dt;
         iteration++;
@@ -481,6 +512,66 @@ void Euler::ComputingFused(int nsteps, const int bc_kinds[6], int rk, int nchem_
     ctx.check(eu3d_step(ctx.get(), &p, nsteps), "eu3d_step");
     iteration += nsteps;
     if (TwoDim.IsNan()) throw Error(EU3D_ERR_NONFINITE, "D or T is nan");
+}
+
+// src/Euler.cpp:984-1021: the phase times of the run, and the loop time for comparing runs.
+void Euler::Output_Total_Time(double file_reader, double mesh_t, double input, double reaction_read,
+                              double field_initial, double computing, double output) const {
+    namespace fs = std::filesystem;
+    const std::string base = outdir + "/output_1/time_record/";
+    fs::create_directories(base + "total");
+    fs::create_directories(base + "compare");
+    std::ofstream t(base + "total/Total_Time_0.txt");
+    t << "myid:" << '\t' << 0 << '\n';
+    t << "FileReader:" << '\t' << file_reader << '\n';
+    t << "Mpi_Initial:" << '\t' << 0.0 << '\n';
+    t << "MeshProcess:" << '\t' << mesh_t << '\n';
+    t << "InputRead:" << '\t' << input << '\n';
+    t << "ReactionRead:" << '\t' << reaction_read << '\n';
+    t << "FieldInitial:" << '\t' << field_initial << '\n';
+    t << "MPI_Barrier:" << '\t' << 0.0 << '\n';
+    t << "Computing:" << '\t' << computing << '\n';
+    t << "Output:" << '\t' << output << '\n';
+    if (!t) throw Error(EU3D_ERR_ARG, "cannot write " + base + "total/Total_Time_0.txt");
+    std::ofstream c(base + "compare/" + ctrl.react + "_mpi_1_omp_1.txt");
+    c << computing << '\t' << 0.0;
+    std::cout << "Loop time: " << computing << std::endl;
+}
+
+// src/Euler.cpp:1022-1099: the loop's phases, per reaction scheme, and the per-rank files.
+void Euler::Output_Compute_Time() const {
+    namespace fs = std::filesystem;
+    const std::string base = outdir + "/output_1/time_record/";
+    fs::create_directories(base + "compute");
+    fs::create_directories(base + "all");
+    std::ofstream o(base + "compute/Compute_Time_0.txt");
+    o << "myid:" << '\t' << 0 << '\n';
+    o << "FieldBoundary:" << '\t' << phase.boundary << '\n';
+    o << "Mpi_Boundary:" << '\t' << phase.mpi_boundary << '\n';
+    o << "Advection:" << '\t' << phase.advection << '\n';
+    o << "Reaction:" << '\t' << phase.reaction << '\n';
+    o << "Output:" << '\t' << phase.output << '\n';
+    switch (ctrl.Reaction_Sch) {
+    case 0:
+        o << "Update_after_Advection:" << '\t' << phase.tr1 << '\n';
+        o << "Trapezoid_Prediction:" << '\t' << phase.prediction << '\n';
+        o << "Trapezoid:" << '\t' << phase.tr2 << '\n';
+        o << "DLB:" << '\t' << 0.0 << '\n';
+        o << "Wait:" << '\t' << 0.0 << '\n';
+        break;
+    case 1:
+        o << "Diagnalized:" << '\t' << phase.tr1 << '\n';
+        o << "Update_IMEX:" << '\t' << phase.tr2 << '\n';
+        o << "FileBoundary:" << phase.boundary << " " << 0.0 << " " << 0.0 << " " << 0.0 << " " << 0.0 << " " << 0.0
+          << " " << 0.0 << '\n';
+        break;
+    }
+    if (!o) throw Error(EU3D_ERR_ARG, "cannot write " + base + "compute/Compute_Time_0.txt");
+    std::ofstream(base + "all/All_wait.txt") << 0.0 << '\n';
+    std::ofstream(base + "all/All_react.txt") << phase.tr1 + phase.tr2 + phase.prediction << '\n';
+    std::ofstream(base + "all/All_advert.txt") << phase.advection << '\n';
+    std::ofstream(base + "all/All_DLB.txt") << 0.0 << '\t' << 0.0 << '\t' << 0.0 << '\t' << 0.0 << '\t' << 0.0 << '\t'
+                                            << 0.0 << '\t' << 0.0 << '\n';
 }
 
 void Euler::Output(int num) {

@@ -188,6 +188,21 @@ class Euler {
     void Output(int num);
     void set_output_dir(const std::string& d) { outdir = d; }
     void set_surrogate(eu3d_mlp* m);
+    // Time records (src/Euler.cpp:984-1099). With record on, Computing times each phase of the
+    // loop (the device is synchronised around it, so record only when the numbers are wanted).
+    // Output_Total_Time / Output_Compute_Time then write <outdir>/output_1/time_record/... in
+    // the reference's formats, as its rank 0 of one rank: MPI phases, load balancing and waits
+    // are 0, and the boundary-fill split of the IMEX record holds the whole fill in its first
+    // entry.
+    void set_time_record(bool on) { record = on; }
+    struct PhaseTimes {
+        double boundary = 0, mpi_boundary = 0, advection = 0, reaction = 0, output = 0;
+        double tr1 = 0, tr2 = 0, prediction = 0;   // Update_after_Adv / Diagonalized, the rest
+    };
+    PhaseTimes phase;
+    void Output_Total_Time(double file_reader, double mesh, double input, double reaction_read,
+                           double field_initial, double computing, double output) const;
+    void Output_Compute_Time() const;
 
     Mesh mesh;
     Mechanism mech;
@@ -207,6 +222,8 @@ class Euler {
     Euler(const Setup& s, uint32_t quirks);
     std::string outdir = ".";
     int nchem_max = 0;
+    bool record = false;
+    double now();   // seconds, after the device has finished queued work when recording
 };
 
 // Tecplot block writer: variables X, Y, Z, T, D, P, U, V, Ma, Y1..Y7, L (Nchem), WaitTime over

@@ -3,7 +3,7 @@
 // in-process with a command line; the eu3d_run program (eu3d_run_main.cpp) only forwards to it.
 //
 //   eu3d_run <ctrl deck> --ic <file> [--bc k0,k1,k2,k3,k4,k5] [--steps N] [--fused RK]
-//            [--quirks ref|fixed] [--out DIR] [--dump FILE]
+//            [--quirks ref|fixed] [--out DIR] [--dump FILE] [--time-record]
 //   eu3d_run <ctrl deck> --print-decks     (parse only, no GPU: JSON of what the decks yield)
 //
 // The control deck names the grid deck, the mechanism and the thermo file as in the reference.
@@ -11,7 +11,9 @@
 // V, W (one value per cell each) then Yi (NS values per cell). --bc replaces
 // FieldBoundary_3dRSBI with the given ghost kinds. --fused runs the steps as one eu3d_step call
 // with RK stages (1 or 2) instead of the method-by-method loop. --dump writes CS (13 per cell)
-// and T after the run as raw float64 in reference layout.
+// and T after the run as raw float64 in reference layout. --time-record times the phases of the
+// method-by-method loop and writes the reference's time-record files under DIR/output_1/.
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -93,7 +95,7 @@ extern "C" int eu3d_run_main(int argc, char** argv) {
         }
     }
     int steps = 0, fused = 0;
-    bool have_bc = false;
+    bool have_bc = false, time_record = false;
     int bc[6] = {0, 0, 0, 0, 0, 0};
     uint32_t quirks = EU3D_QUIRKS_FIXED;
     for (int a = 2; a < argc; a++) {
@@ -108,6 +110,7 @@ extern "C" int eu3d_run_main(int argc, char** argv) {
         else if (k == "--out") out = val();
         else if (k == "--dump") dump = val();
         else if (k == "--quirks") quirks = val() == "ref" ? EU3D_QUIRKS_REF : EU3D_QUIRKS_FIXED;
+        else if (k == "--time-record") time_record = true;
         else if (k == "--bc") {
             std::stringstream ss(val());
             std::string t;
@@ -119,21 +122,38 @@ extern "C" int eu3d_run_main(int argc, char** argv) {
         }
     }
     try {
+        auto clock = []() {
+            return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+        };
+        const double t0 = clock();
         Euler e(ctrl, quirks);
         e.set_output_dir(out);
+        e.set_time_record(time_record);
+        const double t1 = clock();
         const size_t n3 = e.mesh.cells_with_ghosts();
         std::ifstream in(ic, std::ios::binary);
         if (!in.is_open()) throw Error(EU3D_ERR_ARG, "cannot read initial condition " + ic);
         std::vector<double> T = read_block(in, n3, "T"), P = read_block(in, n3, "P"), U = read_block(in, n3, "U"),
                             V = read_block(in, n3, "V"), W = read_block(in, n3, "W"),
                             Y = read_block(in, n3 * e.mech.NS, "Yi");
+        const double t2 = clock();
         e.TwoDim.SetPrimitives(T, P, U, V, W, Y);
         e.TwoDim.FieldInitial();
+        e.TwoDim.IsNan();   // waits for the device
+        const double t3 = clock();
         if (fused) {
             if (!have_bc) throw Error(EU3D_ERR_ARG, "--fused needs --bc");
             e.ComputingFused(steps, bc, fused);
         } else {
             e.Computing(e.ctrl.count, have_bc ? bc : nullptr, steps);
+        }
+        e.TwoDim.IsNan();
+        const double t4 = clock();
+        if (time_record) {
+            // the decks, grid and mechanism are read together by the constructor: FileReader
+            // holds that time, MeshProcess and ReactionRead 0
+            e.Output_Total_Time(t1 - t0, 0.0, t2 - t1, 0.0, t3 - t2, t4 - t3, 0.0);
+            e.Output_Compute_Time();
         }
         std::printf("iterations %d time %.9e dt %.9e\n", e.iteration, e.TwoDim.time(), e.TwoDim.dt());
         std::fflush(stdout);

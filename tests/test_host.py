@@ -240,7 +240,8 @@ def test_driver_fused_rk2(tmp_path):
 
 @pytest.mark.gpu
 def test_driver_stepwise_imex_bubble(tmp_path):
-    """Euler::Computing with IMEX (Diagonalized + Update_IMEX) on a small shock-bubble case."""
+    """Euler::Computing with IMEX (Diagonalized + Update_IMEX) on a small shock-bubble case, with
+    the reference's time records (src/Euler.cpp:984-1099) written beside the results."""
     _need_driver()
     from tests.common import small_bubble
     n = (24, 12, 10)
@@ -249,6 +250,26 @@ def test_driver_stepwise_imex_bubble(tmp_path):
     ctrl = _decks(tmp_path, grid, reaction="IMEX", cfl=0.3)
     decks = _print_decks(ctrl)
     wl = small_bubble()
-    dump = _run(ctrl, _ic_file(tmp_path, wl.prims), ["--bc", "1,1,2,2,2,2", "--steps", "5"], tmp_path)
+    dump = _run(ctrl, _ic_file(tmp_path, wl.prims), ["--bc", "1,1,2,2,2,2", "--steps", "5", "--time-record"],
+                tmp_path)
     o = _oracle(decks, wl.prims, 5, cfl=0.3, bc_kinds=(1, 1, 2, 2, 2, 2), rk=1, chem=1)
     _check_dump(dump, o)
+    rec = tmp_path / "output_1" / "time_record"
+
+    def keyed(path):
+        return dict(line.split(":", 1) for line in path.read_text().splitlines())
+
+    total = keyed(rec / "total" / "Total_Time_0.txt")
+    assert list(total) == ["myid", "FileReader", "Mpi_Initial", "MeshProcess", "InputRead", "ReactionRead",
+                           "FieldInitial", "MPI_Barrier", "Computing", "Output"]
+    comp = keyed(rec / "compute" / "Compute_Time_0.txt")
+    assert list(comp)[:6] == ["myid", "FieldBoundary", "Mpi_Boundary", "Advection", "Reaction", "Output"]
+    assert list(comp)[6:8] == ["Diagnalized", "Update_IMEX"] and "FileBoundary" in comp
+    loop = float(total["Computing"])
+    parts = sum(float(comp[k]) for k in ("FieldBoundary", "Mpi_Boundary", "Advection", "Reaction", "Output"))
+    assert loop > 0 and 0 < parts <= loop
+    assert abs(float(comp["Diagnalized"]) + float(comp["Update_IMEX"]) - float(comp["Reaction"])) <= 0.5 * loop
+    cmp_ = (rec / "compare" / "IMEX_mpi_1_omp_1.txt").read_text().split("\t")
+    assert float(cmp_[0]) == loop
+    for f in ("All_wait.txt", "All_react.txt", "All_advert.txt", "All_DLB.txt"):
+        assert (rec / "all" / f).exists()
