@@ -84,9 +84,24 @@ def cpu_baseline(grid, steps):
     t0 = time.perf_counter()
     o.step(steps, **kw)
     el = time.perf_counter() - t0
-    return {"value": wl.ncells * steps / el, "unit": "cell-updates/s", "cores": cores, "kind": "port",
-            "sample": f"bubble3d {ni}x{nj}x{nk} (same generator; bubble radius 25 cells scaled with ni/384), "
-                      f"{steps} IMEX steps after 1 warm-up step, {el:.1f} s"}
+    res = {"value": wl.ncells * steps / el, "unit": "cell-updates/s", "cores": cores, "kind": "port",
+           "sample": f"bubble3d {ni}x{nj}x{nk} (same generator; bubble radius 25 cells scaled with ni/384), "
+                     f"{steps} IMEX steps after 1 warm-up step, {el:.1f} s"}
+    # one core, on a quarter-size section of the same generator
+    oracle.set_threads(1)
+    w1 = configs.bubble3d(96, 64, 64, radius_cells=25.0 * 96 / 384)
+    o1 = oracle.OracleFlow(w1.xnode, w1.ynode, w1.znode, quirks=0)
+    for k in ("T", "P", "U", "V", "W", "YI"):
+        o1.set(k, w1.prims[k])
+    o1.field_initial()
+    o1.step(1, **dict(kw, bc_kinds=w1.bc_kinds))
+    t0 = time.perf_counter()
+    o1.step(1, **dict(kw, bc_kinds=w1.bc_kinds))
+    el1 = time.perf_counter() - t0
+    oracle.set_threads(cores)
+    res["single_core"] = {"value": w1.ncells / el1, "unit": "cell-updates/s", "cores": 1,
+                          "sample": f"bubble3d 96x64x64, 1 IMEX step after 1 warm-up step, {el1:.1f} s"}
+    return res
 
 
 def bench_batched_chemistry(ctx, n=1 << 22, dtm=1e-8, seed=2024):
