@@ -877,6 +877,47 @@ struct GridCells {
     }
 };
 
+// Grid accessor of eu3d_step's adaptive trapezoid: stores what the step reads next (the conserved
+// state, the primitives of the advection and the CFL bound, Y, the chemical source) and not the
+// derived fields (Di, Mc, Wav, Rgas, Cp, Ma), which eu3d_step refreshes after its last step;
+// Di is read from the conserved state, which holds the same values.
+struct GridCellsStep {
+    G3 g;
+    Fields F;
+    __device__ void load(unsigned id, ChemCell& c, int& n) const {
+        const size_t cc = id;
+#pragma unroll
+        for (int s = 0; s < NS; s++) c.Di[s] = FLD(F, EU3D_FIELD_CS)[s * g.n3 + cc];
+        c.U = FLD(F, EU3D_FIELD_U)[cc];
+        c.V = FLD(F, EU3D_FIELD_V)[cc];
+        c.W = FLD(F, EU3D_FIELD_W)[cc];
+        c.E = FLD(F, EU3D_FIELD_E)[cc];
+        c.T = FLD(F, EU3D_FIELD_T)[cc];
+        n = (int)FLD(F, EU3D_FIELD_NCHEM)[cc];
+    }
+    __device__ void store(unsigned id, const double (&cs)[NC], const Prim& o, const KinOut& k) const {
+        const size_t cc = id;
+        double* CS = FLD(F, EU3D_FIELD_CS);
+#pragma unroll
+        for (int s = 0; s < NC; s++) CS[s * g.n3 + cc] = cs[s];
+        FLD(F, EU3D_FIELD_D)[cc] = o.D;
+        FLD(F, EU3D_FIELD_T)[cc] = o.T;
+        FLD(F, EU3D_FIELD_U)[cc] = o.U;
+        FLD(F, EU3D_FIELD_V)[cc] = o.V;
+        FLD(F, EU3D_FIELD_W)[cc] = o.W;
+        FLD(F, EU3D_FIELD_E)[cc] = o.E;
+        FLD(F, EU3D_FIELD_P)[cc] = o.P;
+        FLD(F, EU3D_FIELD_H)[cc] = o.H;
+        FLD(F, EU3D_FIELD_GAMMA)[cc] = o.Gamma;
+        FLD(F, EU3D_FIELD_C)[cc] = o.C;
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+            FLD(F, EU3D_FIELD_YI)[s * g.n3 + cc] = o.Y[s];
+            FLD(F, EU3D_FIELD_CMS)[s * g.n3 + cc] = k.w[s] * 1e3;
+        }
+    }
+};
+
 // Flat batch accessor: rho[s * n + id], T[id], E[id]; the cells are at rest.
 struct BatchCells {
     long long n;
@@ -1028,8 +1069,13 @@ k_trap_first(G3 g, Fields F, uint32_t q, const Scalars* __restrict__ scc, Scalar
         if (!predict_only && nch > 0 && !to_net) {
             double cs[NC];
             const Prim o = chem_apply(cell, ko, Dr, dtm / nch, q, cs, tp);
-            GridCells acc{g, F};
-            acc.store((unsigned)c, cs, o, ko);
+            if (ADV) {
+                GridCellsStep acc{g, F};
+                acc.store((unsigned)c, cs, o, ko);
+            } else {
+                GridCells acc{g, F};
+                acc.store((unsigned)c, cs, o, ko);
+            }
             active = nch > 1;
         }
     }
@@ -1828,7 +1874,11 @@ hipError_t launch_trap_adaptive(eu3d_ctx* c, int nchem_max, bool predict_only, i
         k_trap_first<false, false><<<gr, bl, 0, c->stream>>>(G(c), c->fl, c->quirks, c->sc, c->sc, c->active, nmax, po,
                                                              nthr, T_cold, slist, scount, rt);
     if (adv) k_adv_nochem<<<gr, bl, 0, c->stream>>>(G(c), c->fl, c->quirks);
-    if (!predict_only) {
+    if (!predict_only && adv) {   // the step: the derived fields are refreshed after its last step
+        GridCellsStep acc{c->g, c->fl};
+        launch_persist(c, acc, c->active, slot);
+        launch_coop(c, acc, slot);
+    } else if (!predict_only) {
         GridCells acc{c->g, c->fl};
         launch_persist(c, acc, c->active, slot);
         launch_coop(c, acc, slot);
