@@ -96,8 +96,8 @@ __device__ __forceinline__ double cp_R(int s, bool lo, const TPow& p) {
 
 // x / D, correctly rounded, for the constant divisors D = 3 and 5 of the h/R and s/R integrals:
 // the product with RN(1/D) refined by one FMA residual step (Markstein's correction). It equals
-// IEEE division on every finite, normal operand sampled by the oracle's check (6e8 operands
-// across the exponent range, both divisors) at three instructions instead of a full division.
+// IEEE division on every operand of the oracle's check (3e7 per divisor: random over 2^-300..2^300
+// and the mantissas next to 1 and 2) at three instructions instead of a full division.
 template <int D>
 __device__ __forceinline__ double div_const(double x) {
     static_assert(D == 3 || D == 5, "checked for the divisors 3 and 5 only");
