@@ -172,16 +172,45 @@ __global__ void k_reduce_splits(const float* __restrict__ part, float* __restric
     out[i] = s;
 }
 
-// Column sums of a bf16 [rows x ld] matrix (bias gradients), deterministic two-level:
-// block b sums a row range per column into part[b * ncol + col].
-__global__ void k_colsum_bf16(const bf16_t* __restrict__ X, int rows, int ld, int ncol, int rows_per_block,
-                              float* __restrict__ part) {
-    const int col = threadIdx.x + blockIdx.y * blockDim.x;
-    if (col >= ncol) return;
-    const int r0 = blockIdx.x * rows_per_block, r1 = min(rows, r0 + rows_per_block);
+// First level of a two-level split-K sum: out[g * n + i] = sum of partials g*per .. g*per+per-1.
+__global__ void k_reduce_splits_grouped(const float* __restrict__ part, float* __restrict__ out, int n, int splits,
+                                        int per) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, g = blockIdx.y;
+    if (i >= n) return;
+    const int k1 = min(splits, (g + 1) * per);
     float s = 0.f;
-    for (int r = r0; r < r1; r++) s += bf2f(X[(size_t)r * ld + col]);
-    part[(size_t)blockIdx.x * ncol + col] = s;
+    for (int k = g * per; k < k1; k++) s += part[(size_t)k * n + i];
+    out[(size_t)g * n + i] = s;
+}
+
+// Column sums of a bf16 [rows x ncol] matrix (bias gradients; ncol a multiple of 8, at most 256),
+// deterministic two-level: block b sums rows [b * rows_per_block, ...) per column into
+// part[b * ncol + col]. A row is read by ncol / 8 threads with 16-byte loads; the block's
+// 2048 / ncol row groups stride over its rows, and their sums are combined in LDS in group order.
+__global__ void __launch_bounds__(256) k_colsum_bf16(const bf16_t* __restrict__ X, int rows, int ncol,
+                                                     int rows_per_block, float* __restrict__ part) {
+    __shared__ float red[2048];
+    const int tpr = ncol / 8, groups = 256 / tpr;
+    const int t = threadIdx.x, cg = t % tpr, rg = t / tpr;
+    const int r0 = blockIdx.x * rows_per_block, r1 = min(rows, r0 + rows_per_block);
+    float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    for (int r = r0 + rg; r < r1; r += groups) {
+        const uint4 v = *reinterpret_cast<const uint4*>(X + (size_t)r * ncol + cg * 8);
+        const unsigned w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            s[2 * j] += __uint_as_float(w[j] << 16);
+            s[2 * j + 1] += __uint_as_float(w[j] & 0xffff0000u);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 8; j++) red[rg * ncol + cg * 8 + j] = s[j];
+    __syncthreads();
+    if (t < ncol) {
+        float a = 0.f;
+        for (int g = 0; g < groups; g++) a += red[g * ncol + t];
+        part[(size_t)blockIdx.x * ncol + t] = a;
+    }
 }
 
 // Loss gradient: dY = 2 (Y - T) / (n * nout) on the nout real outputs, zero on the padding;
@@ -405,7 +434,9 @@ hipError_t gemm(hipStream_t s, const GemmArgs& a0, int splits = 1) {
     return hipGetLastError();
 }
 
-int splits_for(int K) { return std::max(1, std::min(64, K / 2048)); }
+// split-K count of the weight-gradient GEMMs (M = N = 256, K = rows): enough K chunks for about
+// eight 128 x 128 tiles per CU at 2^20 rows
+int splits_for(int K) { return std::max(1, std::min(512, K / 2048)); }
 
 hipError_t ensure_ws(eu3d_mlp* m, int n) {
     if (n <= m->cap) return hipSuccess;
@@ -994,8 +1025,12 @@ int eu3d_mlp_train_step(eu3d_mlp* m, int n, const void* feats_bf16, const float*
     hipStream_t s = m->stream;
     if (ensure_ws(m, n) != hipSuccess) return EU3D_ERR_HIP;
     // partials: split-K dW [sp x 256 x 256] followed by the bias column sums [nb x 256]
-    const int sp = splits_for(n), rpb = 4096, nb = (n + rpb - 1) / rpb;
-    if (ensure_part(m, (size_t)sp * NH * NH + (size_t)nb * NH) != hipSuccess) return EU3D_ERR_HIP;
+    // partials: split-K dW [sp x 256 x 256], the bias column sums [nb x 256], then the first-level
+    // sums of the split-K partials in groups of 32 [ceil(sp / 32) x 256 x 256]
+    const int sp = splits_for(n), rpb = 512, nb = (n + rpb - 1) / rpb, kgrp = 32;
+    const int ng = (sp + kgrp - 1) / kgrp;
+    if (ensure_part(m, (size_t)sp * NH * NH + (size_t)nb * NH + (size_t)ng * NH * NH) != hipSuccess)
+        return EU3D_ERR_HIP;
     if (hipMemcpyAsync(m->act[0], feats_bf16, (size_t)n * NIN * 2, hipMemcpyDeviceToDevice, s) != hipSuccess)
         return EU3D_ERR_HIP;
     if (forward(m, n, true) != hipSuccess) return EU3D_ERR_HIP;
@@ -1021,10 +1056,17 @@ int eu3d_mlp_train_step(eu3d_mlp* m, int n, const void* feats_bf16, const float*
         if (gemm<false, false, EPI_F32>(s, g, sp) != hipSuccess) return EU3D_ERR_HIP;
         const int nw = m->dout[l] * m->din[l];
         const int kch = ((n + sp - 1) / sp + BK - 1) / BK * BK, nsp = (n + kch - 1) / kch;
-        k_reduce_splits<<<(nw + 255) / 256, 256, 0, s>>>(m->part, m->gw[l], nw, nsp);
+        if (nsp > kgrp) {   // two levels: many split-K partials
+            float* grp = m->part + (size_t)sp * NH * NH + (size_t)nb * NH;
+            const int ngr = (nsp + kgrp - 1) / kgrp;
+            k_reduce_splits_grouped<<<dim3((nw + 255) / 256, ngr), 256, 0, s>>>(m->part, grp, nw, nsp, kgrp);
+            k_reduce_splits<<<(nw + 255) / 256, 256, 0, s>>>(grp, m->gw[l], nw, ngr);
+        } else {
+            k_reduce_splits<<<(nw + 255) / 256, 256, 0, s>>>(m->part, m->gw[l], nw, nsp);
+        }
         // db = column sums of dZ
         float* cpart = m->part + (size_t)sp * NH * NH;
-        k_colsum_bf16<<<dim3(nb, (m->dout[l] + 255) / 256), 256, 0, s>>>(dz, n, m->dout[l], m->dout[l], rpb, cpart);
+        k_colsum_bf16<<<nb, 256, 0, s>>>(dz, n, m->dout[l], rpb, cpart);
         k_reduce_splits<<<(m->dout[l] + 255) / 256, 256, 0, s>>>(cpart, m->gb[l], m->dout[l], nb);
         if (l > 0) {
             // dZ_{l-1} = (dZ_l [n x dout] * W_l^T) .* GELU'(Z_{l-1}); B operand element
