@@ -207,12 +207,16 @@ struct LazyMix {
 // T from internal energy (Flowfield::Get_temp, src/Flowfield.cpp:1542-1566): Newton on
 // sum_s Y_s e_s(T) = E, at most 10 iterations, stop when |dT| < 1e-6 K. E in J/kg. p holds the
 // powers of the starting temperature (callers that already have them pass them in).
-__device__ __forceinline__ double newton_T_from(const MixFit& m, TPow p, double E, int* iters = nullptr) {
+// The iterates' powers of T come from the previous iterate's by tpow_near (no log or division
+// while the steps are small); *p_last receives the powers of the last iterate, from which the
+// caller forms those of the result the same way.
+__device__ __forceinline__ double newton_T_from(const MixFit& m, TPow p, double E, int* iters = nullptr,
+                                                TPow* p_last = nullptr) {
     double Tn = p.T;
     const double Ek = E * 1e-3;
     int it = 0;
     for (int count = 0; count < 10; count++) {
-        if (count > 0) p = tpow(Tn);
+        if (count > 0) p = tpow_near(p, Tn);
         const double T0 = p.T;
         const bool lo = T0 < 1000;
         const double* c = lo ? m.lo : m.hi;
@@ -227,6 +231,7 @@ __device__ __forceinline__ double newton_T_from(const MixFit& m, TPow p, double 
         if (fabs(Tn - T0) < 1e-6) break;
     }
     if (iters) *iters = it;
+    if (p_last) *p_last = p;
     return Tn;
 }
 __device__ __forceinline__ double newton_T(const MixFit& m, double T, double E, int* iters = nullptr) {
@@ -234,12 +239,12 @@ __device__ __forceinline__ double newton_T(const MixFit& m, double T, double E, 
 }
 // The same Newton on a LazyMix of the mass fractions Y.
 __device__ __forceinline__ double newton_T_lazy(LazyMix& L, const double (&Y)[NS], bool c4, TPow p, double E,
-                                                int* iters = nullptr) {
+                                                int* iters = nullptr, TPow* p_last = nullptr) {
     double Tn = p.T;
     const double Ek = E * 1e-3;
     int it = 0;
     for (int count = 0; count < 10; count++) {
-        if (count > 0) p = tpow(Tn);
+        if (count > 0) p = tpow_near(p, Tn);
         const double T0 = p.T;
         const bool lo = T0 < 1000;
         L.need(Y, c4, lo);
@@ -255,6 +260,7 @@ __device__ __forceinline__ double newton_T_lazy(LazyMix& L, const double (&Y)[NS
         if (fabs(Tn - T0) < 1e-6) break;
     }
     if (iters) *iters = it;
+    if (p_last) *p_last = p;
     return Tn;
 }
 
@@ -303,13 +309,15 @@ __device__ __forceinline__ Prim prim_from_cs_p(const double (&cs)[NC], const TPo
         L.m.rmix = 0.0;
 #pragma unroll
         for (int s = 0; s < NS; s++) L.m.rmix += o.Y[s] * RI(s);
-        o.T = newton_T_lazy(L, o.Y, q & Q_C4, p_old, (q & Q_F3) ? E_old : o.E, iters);
+        TPow pl;
+        o.T = newton_T_lazy(L, o.Y, q & Q_C4, p_old, (q & Q_F3) ? E_old : o.E, iters, &pl);
         L.need(o.Y, q & Q_C4, o.T < 1000);
-        cp = mix_cp(L.m, o.T);
+        cp = mix_cp_p(L.m, tpow_near(pl, o.T));
     } else {
         const MixFit m = mix_fit(o.Y, q & Q_C4);
-        o.T = newton_T_from(m, p_old, (q & Q_F3) ? E_old : o.E, iters);
-        cp = mix_cp(m, o.T);
+        TPow pl;
+        o.T = newton_T_from(m, p_old, (q & Q_F3) ? E_old : o.E, iters, &pl);
+        cp = mix_cp_p(m, tpow_near(pl, o.T));
     }
     double sy = 0;
 #pragma unroll
