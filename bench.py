@@ -52,6 +52,9 @@ def parse():
                     help="skip the batched-chemistry and surrogate-network sections")
     ap.add_argument("--cpu-grid", type=str, default="384x128x128")
     ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 256 MiB of float64 three times (the known-size kernel that scales the "
+                         "FETCH_SIZE / WRITE_SIZE counters to bytes) and exit")
     return ap.parse_args()
 
 
@@ -189,6 +192,14 @@ def main():
     from eu3d_amd import configs
     from eu3d_amd.api import Context
     assert torch.cuda.is_available(), "bench.py needs the GPU"
+    if a.copy_calibration:
+        x = torch.ones(256 * 1024 * 1024 // 8, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copied_bytes_per_launch": x.numel() * 8, "launches": 3}))
+        return
     ni, nj, nk = (int(v) for v in a.grid.split("x"))
     wl = configs.bubble3d(ni, nj, nk)
     g = Context(wl.xnode, wl.ynode, wl.znode, quirks=0)

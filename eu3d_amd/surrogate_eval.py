@@ -15,8 +15,10 @@ Two workloads, each a JSON report:
   - the fraction of cells sent through the network and the fraction rejected.
   Training pairs are harvested from plain runs of a smaller transverse section of the same
   configuration.
+- plain-detonation: the plain FP64 detonation step alone (its time per step over the last 40 of
+  140 steps), the driver of the detonation kernel profile.
 
-Usage: python -m eu3d_amd.surrogate_eval batch|detonation [options]
+Usage: python -m eu3d_amd.surrogate_eval batch|detonation|plain-detonation [options]
 """
 from __future__ import annotations
 
@@ -213,9 +215,37 @@ def eval_detonation(grid=(512, 256, 128), nsteps=150, threshold=4, train_grid=(5
     return out
 
 
+def time_plain_detonation(grid=(512, 256, 128), nsteps=140, window=40, log=print):
+    """The plain FP64 detonation step alone: adaptive trapezoid chemistry, the CFL ramp of
+    eval_detonation, one step per call with the derived fields refreshed after the last. Returns
+    the mean ms/step of the last `window` steps (HIP events on the context stream)."""
+    _torch()
+    wl = configs.det3d(*grid)
+    g = Context(wl.xnode, wl.ynode, wl.znode, quirks=0)
+    for k in ("T", "P", "U", "V", "W", "YI"):
+        g.set(k, wl.prims[k])
+    g.field_initial()
+    kw = dict(wl.step)
+    kw.pop("cfl")
+    ts = []
+    for s in range(nsteps):
+        c = 0.05 + (0.5 - 0.05) * min(1.0, s / 20)
+        last = s == nsteps - 1
+        _, ms = _timed(g.stream(), lambda: g.step(1, cfl=c, bc_kinds=wl.bc_kinds, skip_derive=not last, **kw))
+        ts.append(ms)
+        if (s + 1) % 20 == 0:
+            log(f"step {s + 1}: {ms:.2f} ms")
+    T = g.get("T")[2:-2, 2:-2, 2:-2]
+    out = {"grid": list(grid), "steps": nsteps, "ms_per_step_last": float(np.mean(ts[-window:])),
+           "window": window, "cells_per_s": float(np.prod(grid) / (np.mean(ts[-window:]) * 1e-3)),
+           "T_finite": bool(np.isfinite(T).all())}
+    g.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("workload", choices=("batch", "detonation"))
+    ap.add_argument("workload", choices=("batch", "detonation", "plain-detonation"))
     ap.add_argument("--threshold", type=int, default=None)
     ap.add_argument("--steps", type=int, default=150)
     ap.add_argument("--grid", type=str, default="512x256x128")
@@ -238,6 +268,8 @@ def main():
             kw["dT_max"] = a.dT_max
         kw["train_seeds"] = tuple(101 + i for i in range(a.seeds))
         res = eval_batch(log=log, stiff_only=a.stiff_only, **kw)
+    elif a.workload == "plain-detonation":
+        res = time_plain_detonation(grid=tuple(int(v) for v in a.grid.split("x")), nsteps=a.steps, log=log)
     else:
         kw = dict(grid=tuple(int(v) for v in a.grid.split("x")), nsteps=a.steps)
         if a.threshold is not None:
