@@ -316,10 +316,17 @@ __device__ __forceinline__ void store_w(bf16_t (*ws)[WPITCH], const WPiece& r, i
     if (p1 < N * 4) *reinterpret_cast<uint4*>(&ws[p1 >> 2][(p1 & 3) * 8]) = r.b;
 }
 
-template <int K, int N, bool LAST>
+// Training outputs of one hidden layer: the pre-activation Z and the activation GELU(Z), BF16
+// [n x 256] each (the backward pass reads both).
+struct TrainOut {
+    bf16_t* z;
+    bf16_t* a;
+};
+
+template <int K, int N, bool LAST, bool TRAIN = false>
 __device__ __forceinline__ void fused_layer(bf16_t (*act)[FPITCH], bf16_t (*ws)[256][WPITCH], const bf16_t* __restrict__ wt,
                                             const float* __restrict__ bias, int tid, int lane, int wr, long long row0,
-                                            int n, float* __restrict__ Y) {
+                                            int n, float* __restrict__ Y, TrainOut tr = {}) {
     constexpr int NT = N / 16, KS = K / 32;
     const int fr = lane & 15, fk = (lane >> 4) * 8;
     f32x4_t acc[NT];
@@ -347,7 +354,7 @@ __device__ __forceinline__ void fused_layer(bf16_t (*act)[FPITCH], bf16_t (*ws)[
         }
     }
     // D fragment: lane holds rows 4 * (lane / 16) + r, column lane % 16 of each 16 x 16 tile
-    if (!LAST) {
+    if constexpr (!LAST && !TRAIN) {
 #pragma unroll
         for (int j = 0; j < NT; j++) {
             const int col = 16 * j + fr;
@@ -356,6 +363,37 @@ __device__ __forceinline__ void fused_layer(bf16_t (*act)[FPITCH], bf16_t (*ws)[
             for (int r = 0; r < 4; r++) {
                 const bf16_t zb = f2bf(acc[j][r] + bv);
                 act[wr + 4 * (lane >> 4) + r][col] = f2bf(gelu(bf2f(zb)));
+            }
+        }
+    } else if constexpr (!LAST) {
+        // Z into the wavefront's own rows of act; then, in 16-byte pieces (row-contiguous, so the
+        // global stores coalesce), Z out to HBM, GELU(Z) back into act and out to HBM
+#pragma unroll
+        for (int j = 0; j < NT; j++) {
+            const int col = 16 * j + fr;
+            const float bv = bias[col];
+#pragma unroll
+            for (int r = 0; r < 4; r++) act[wr + 4 * (lane >> 4) + r][col] = f2bf(acc[j][r] + bv);
+        }
+        static_assert(N == 256, "16 rows x 256 columns: 8 pieces per lane");
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
+            const int c = lane + 64 * t, r = c >> 5, cc = (c & 31) * 8;
+            uint4 zv = *reinterpret_cast<const uint4*>(&act[wr + r][cc]);
+            const unsigned zw[4] = {zv.x, zv.y, zv.z, zv.w};
+            unsigned gw[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const bf16_t lo = f2bf(gelu(bf2f((bf16_t)(zw[q] & 0xffffu))));
+                const bf16_t hi = f2bf(gelu(bf2f((bf16_t)(zw[q] >> 16))));
+                gw[q] = (unsigned)lo | ((unsigned)hi << 16);
+            }
+            const uint4 gv = make_uint4(gw[0], gw[1], gw[2], gw[3]);
+            *reinterpret_cast<uint4*>(&act[wr + r][cc]) = gv;
+            const long long row = row0 + wr + r;
+            if (row < n) {
+                *reinterpret_cast<uint4*>(tr.z + (size_t)row * N + cc) = zv;
+                *reinterpret_cast<uint4*>(tr.a + (size_t)row * N + cc) = gv;
             }
         }
     } else {
@@ -367,9 +405,13 @@ __device__ __forceinline__ void fused_layer(bf16_t (*act)[FPITCH], bf16_t (*ws)[
     }
 }
 
-// count (device, may be null): rows beyond *count are skipped, whole tiles at once.
+// count (device, may be null): rows beyond *count are skipped, whole tiles at once. TRAIN: the
+// training forward pass, which also writes each hidden layer's Z and GELU(Z) (tr[l], l = 0..3).
+template <bool TRAIN>
 __global__ void __launch_bounds__(FT) k_mlp_fused(MlpView v, int n_max, const unsigned* __restrict__ count,
-                                                  const bf16_t* __restrict__ X, float* __restrict__ Y) {
+                                                  const bf16_t* __restrict__ X, float* __restrict__ Y,
+                                                  TrainOut tr0 = {}, TrainOut tr1 = {}, TrainOut tr2 = {},
+                                                  TrainOut tr3 = {}) {
     __shared__ bf16_t act[FM][FPITCH];
     __shared__ bf16_t ws[2][256][WPITCH];
     const int tid = threadIdx.x, lane = tid & 63, wr = (tid >> 6) * 16;
@@ -383,10 +425,10 @@ __global__ void __launch_bounds__(FT) k_mlp_fused(MlpView v, int n_max, const un
             if (row0 + r < n) val = *reinterpret_cast<const uint4*>(X + (size_t)(row0 + r) * 32 + kc);
             *reinterpret_cast<uint4*>(&act[r][kc]) = val;
         }
-        fused_layer<32, 256, false>(act, ws, v.wt[0], v.b[0], tid, lane, wr, row0, n, Y);
-        fused_layer<256, 256, false>(act, ws, v.wt[1], v.b[1], tid, lane, wr, row0, n, Y);
-        fused_layer<256, 256, false>(act, ws, v.wt[2], v.b[2], tid, lane, wr, row0, n, Y);
-        fused_layer<256, 256, false>(act, ws, v.wt[3], v.b[3], tid, lane, wr, row0, n, Y);
+        fused_layer<32, 256, false, TRAIN>(act, ws, v.wt[0], v.b[0], tid, lane, wr, row0, n, Y, tr0);
+        fused_layer<256, 256, false, TRAIN>(act, ws, v.wt[1], v.b[1], tid, lane, wr, row0, n, Y, tr1);
+        fused_layer<256, 256, false, TRAIN>(act, ws, v.wt[2], v.b[2], tid, lane, wr, row0, n, Y, tr2);
+        fused_layer<256, 256, false, TRAIN>(act, ws, v.wt[3], v.b[3], tid, lane, wr, row0, n, Y, tr3);
         fused_layer<256, 16, true>(act, ws, v.wt[4], v.b[4], tid, lane, wr, row0, n, Y);
     }
 }
@@ -516,7 +558,22 @@ hipError_t forward_fused(eu3d_mlp* m, int n, const bf16_t* X, float* Y, hipStrea
     // one 8-wave block per CU fits its LDS; a device count means a grid-stride over the tiles
     const unsigned tiles = (unsigned)((n + FM - 1) / FM);
     const unsigned nb = count ? std::min<unsigned>(tiles, (unsigned)m->num_cu) : tiles;
-    k_mlp_fused<<<nb, FT, 0, st>>>(v, n, count, X, Y);
+    k_mlp_fused<false><<<nb, FT, 0, st>>>(v, n, count, X, Y);
+    return hipGetLastError();
+}
+
+// The training forward pass through the fused kernel: outputs into yout, and each hidden layer's
+// Z and GELU(Z) into zs[l + 1] / act[l + 1] for the backward pass.
+hipError_t forward_fused_train(eu3d_mlp* m, int n) {
+    MlpView v;
+    for (int l = 0; l < 5; l++) {
+        v.wt[l] = m->wt[l];
+        v.b[l] = m->b[l];
+    }
+    TrainOut t[4];
+    for (int l = 0; l < 4; l++) t[l] = TrainOut{m->zs[l + 1], m->act[l + 1]};
+    const unsigned tiles = (unsigned)((n + FM - 1) / FM);
+    k_mlp_fused<true><<<tiles, FT, 0, m->stream>>>(v, n, nullptr, m->act[0], m->yout, t[0], t[1], t[2], t[3]);
     return hipGetLastError();
 }
 
@@ -1033,7 +1090,7 @@ int eu3d_mlp_train_step(eu3d_mlp* m, int n, const void* feats_bf16, const float*
         return EU3D_ERR_HIP;
     if (hipMemcpyAsync(m->act[0], feats_bf16, (size_t)n * NIN * 2, hipMemcpyDeviceToDevice, s) != hipSuccess)
         return EU3D_ERR_HIP;
-    if (forward(m, n, true) != hipSuccess) return EU3D_ERR_HIP;
+    if (forward_fused_train(m, n) != hipSuccess) return EU3D_ERR_HIP;
     // loss and dY (bf16, [n x 16]) into dz_a
     const int lb = 1024;
     k_mse_grad<<<lb, 256, 0, s>>>(m->yout, targets, n, NOUT, NOUT_REAL, 2.0f / ((float)n * NOUT_REAL), m->dz_a, m->part);
