@@ -323,13 +323,14 @@ struct TrainOut {
     bf16_t* a;
 };
 
-template <int K, int N, bool LAST, bool TRAIN = false>
-__device__ __forceinline__ void fused_layer(bf16_t (*act)[FPITCH], bf16_t (*ws)[256][WPITCH], const bf16_t* __restrict__ wt,
-                                            const float* __restrict__ bias, int tid, int lane, int wr, long long row0,
-                                            int n, float* __restrict__ Y, TrainOut tr = {}) {
+// acc = act[the wavefront's 16 rows][0:K] x W^T[0:N][0:K]^T on MFMA, the weights streamed in
+// 32-column chunks through the double-buffered stage ws.
+template <int K, int N>
+__device__ __forceinline__ void fused_mfma(const bf16_t (*act)[FPITCH], bf16_t (*ws)[256][WPITCH],
+                                           const bf16_t* __restrict__ wt, int tid, int lane, int wr,
+                                           f32x4_t (&acc)[N / 16]) {
     constexpr int NT = N / 16, KS = K / 32;
     const int fr = lane & 15, fk = (lane >> 4) * 8;
-    f32x4_t acc[NT];
 #pragma unroll
     for (int j = 0; j < NT; j++) acc[j] = (f32x4_t){0.f, 0.f, 0.f, 0.f};
     // chunk c waits in register set c % 2 from its load (two chunks ahead) to its LDS store
@@ -353,6 +354,16 @@ __device__ __forceinline__ void fused_layer(bf16_t (*act)[FPITCH], bf16_t (*ws)[
             acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[j], 0, 0, 0);
         }
     }
+}
+
+template <int K, int N, bool LAST, bool TRAIN = false>
+__device__ __forceinline__ void fused_layer(bf16_t (*act)[FPITCH], bf16_t (*ws)[256][WPITCH], const bf16_t* __restrict__ wt,
+                                            const float* __restrict__ bias, int tid, int lane, int wr, long long row0,
+                                            int n, float* __restrict__ Y, TrainOut tr = {}) {
+    constexpr int NT = N / 16;
+    const int fr = lane & 15;
+    f32x4_t acc[NT];
+    fused_mfma<K, N>(act, ws, wt, tid, lane, wr, acc);
     // D fragment: lane holds rows 4 * (lane / 16) + r, column lane % 16 of each 16 x 16 tile
     if constexpr (!LAST && !TRAIN) {
 #pragma unroll
@@ -430,6 +441,57 @@ __global__ void __launch_bounds__(FT) k_mlp_fused(MlpView v, int n_max, const un
         fused_layer<256, 256, false, TRAIN>(act, ws, v.wt[2], v.b[2], tid, lane, wr, row0, n, Y, tr2);
         fused_layer<256, 256, false, TRAIN>(act, ws, v.wt[3], v.b[3], tid, lane, wr, row0, n, Y, tr3);
         fused_layer<256, 16, true>(act, ws, v.wt[4], v.b[4], tid, lane, wr, row0, n, Y);
+    }
+}
+
+// One wavefront's 16 rows x 256 columns between HBM and LDS in 16-byte pieces (8 per lane),
+// rows at or beyond n skipped (loads give zeros).
+__device__ __forceinline__ void rows_in(bf16_t (*act)[FPITCH], const bf16_t* __restrict__ g, long long row0, int wr,
+                                        int lane, int n) {
+#pragma unroll
+    for (int t = 0; t < 8; t++) {
+        const int c = lane + 64 * t, r = c >> 5, cc = (c & 31) * 8;
+        const long long row = row0 + wr + r;
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (row < n) v = *reinterpret_cast<const uint4*>(g + (size_t)row * 256 + cc);
+        *reinterpret_cast<uint4*>(&act[wr + r][cc]) = v;
+    }
+}
+__device__ __forceinline__ void rows_out(const bf16_t (*act)[FPITCH], bf16_t* __restrict__ g, long long row0, int wr,
+                                         int lane, int n) {
+#pragma unroll
+    for (int t = 0; t < 8; t++) {
+        const int c = lane + 64 * t, r = c >> 5, cc = (c & 31) * 8;
+        const long long row = row0 + wr + r;
+        if (row < n) *reinterpret_cast<uint4*>(g + (size_t)row * 256 + cc) = *reinterpret_cast<const uint4*>(&act[wr + r][cc]);
+    }
+}
+
+// Backward through one hidden layer: dZ_{l-1} = (dZ_l W_l) .* GELU'(Z_{l-1}) for a tile of FM rows,
+// all [n x 256] BF16. Each wavefront stages its own 16 rows of dZ_l in LDS, the weights (W_l as
+// [in x out], K-contiguous rows) stream through the forward kernel's double-buffered stage, then
+// the wavefront's rows of Z_{l-1} replace its dZ_l rows in LDS for the GELU' epilogue, whose
+// result leaves with 16-byte stores. Same arithmetic as the GEMM's DGELU epilogue.
+__global__ void __launch_bounds__(FT) k_mlp_dx(const bf16_t* __restrict__ dz, const bf16_t* __restrict__ wn,
+                                               const bf16_t* __restrict__ Z, bf16_t* __restrict__ out, int n) {
+    __shared__ bf16_t act[FM][FPITCH];
+    __shared__ bf16_t ws[2][256][WPITCH];
+    const int tid = threadIdx.x, lane = tid & 63, wr = (tid >> 6) * 16, fr = lane & 15;
+    for (long long row0 = (long long)blockIdx.x * FM; row0 < n; row0 += (long long)gridDim.x * FM) {
+        rows_in(act, dz, row0, wr, lane, n);
+        f32x4_t acc[16];
+        fused_mfma<256, 256>(act, ws, wn, tid, lane, wr, acc);
+        rows_in(act, Z, row0, wr, lane, n);   // the wavefront read only its own rows above
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+            const int col = 16 * j + fr;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                bf16_t& e = act[wr + 4 * (lane >> 4) + r][col];
+                e = f2bf(acc[j][r] * dgelu(bf2f(e)));
+            }
+        }
+        rows_out(act, out, row0, wr, lane, n);
     }
 }
 
@@ -1128,12 +1190,17 @@ int eu3d_mlp_train_step(eu3d_mlp* m, int n, const void* feats_bf16, const float*
         if (l > 0) {
             // dZ_{l-1} = (dZ_l [n x dout] * W_l^T) .* GELU'(Z_{l-1}); B operand element
             // (n = i, k = o) = W[i][o] = wn[i * dout + o]
-            GemmArgs d{};
-            d.M = n; d.N = m->din[l]; d.K = m->dout[l];
-            d.A = dz; d.lda = m->dout[l];
-            d.B = m->wn[l]; d.ldb = m->dout[l];
-            d.Cb = dz_next; d.Z = m->zs[l]; d.ldc = m->din[l];
-            if (gemm<true, true, EPI_DGELU>(s, d) != hipSuccess) return EU3D_ERR_HIP;
+            if (m->dout[l] == NH && m->din[l] == NH) {   // hidden layers: the fused row-tile kernel
+                k_mlp_dx<<<(n + FM - 1) / FM, FT, 0, s>>>(dz, m->wn[l], m->zs[l], dz_next, n);
+                if (hipGetLastError() != hipSuccess) return EU3D_ERR_HIP;
+            } else {
+                GemmArgs d{};
+                d.M = n; d.N = m->din[l]; d.K = m->dout[l];
+                d.A = dz; d.lda = m->dout[l];
+                d.B = m->wn[l]; d.ldb = m->dout[l];
+                d.Cb = dz_next; d.Z = m->zs[l]; d.ldc = m->din[l];
+                if (gemm<true, true, EPI_DGELU>(s, d) != hipSuccess) return EU3D_ERR_HIP;
+            }
             std::swap(dz, dz_next);
         }
     }
