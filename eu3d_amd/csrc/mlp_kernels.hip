@@ -72,36 +72,67 @@ struct GemmArgs {
     int ldc;
 };
 
+// One operand tile (ROWS x BK) moves in two phases, so that the loads of the next K step are in
+// flight while the MFMAs of the current one run: stage_load takes the thread's two 16-byte
+// pieces into registers, stage_store writes them to S[row][k] (the M- or N-contiguous case
+// transposing through 8 two-byte stores).
+constexpr int STAGE_PIECES = 2;   // (ROWS * BK / 8) / 256 for ROWS = 128, BK = 32
+struct StageRegs {
+    uint4 v[STAGE_PIECES];
+};
 template <bool KC, int ROWS>
-__device__ __forceinline__ void stage(bf16_t (*S)[LDK], const bf16_t* G, int ld, int r0, int nrows, int k0, int kend,
-                                      int tid) {
-    if (KC) {   // G[r * ld + k]; 8-element chunks along k
-        for (int c = tid; c < ROWS * (BK / 8); c += 256) {
+__device__ __forceinline__ StageRegs stage_load(const bf16_t* G, int ld, int r0, int nrows, int k0, int kend, int tid) {
+    static_assert(ROWS * BK / 8 == STAGE_PIECES * 256, "two pieces per thread");
+    StageRegs R;
+#pragma unroll
+    for (int p = 0; p < STAGE_PIECES; p++) {
+        const int c = tid + 256 * p;
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (KC) {   // G[r * ld + k]; 8-element pieces along k
             const int r = c / (BK / 8), kc = (c % (BK / 8)) * 8;
             const int gr = r0 + r, gk = k0 + kc;
-            uint4 v = make_uint4(0, 0, 0, 0);
             if (gr < nrows) {
-                if (gk + 8 <= kend && ((((uintptr_t)(G + (size_t)gr * ld + gk)) & 15) == 0)) {
-                    v = *reinterpret_cast<const uint4*>(G + (size_t)gr * ld + gk);
+                const bf16_t* src = G + (size_t)gr * ld + gk;
+                if (gk + 8 <= kend && (((uintptr_t)src) & 15) == 0) {
+                    v = *reinterpret_cast<const uint4*>(src);
                 } else {
                     bf16_t tmp[8];
-                    for (int e = 0; e < 8; e++) tmp[e] = (gk + e < kend) ? G[(size_t)gr * ld + gk + e] : (bf16_t)0;
+                    for (int e = 0; e < 8; e++) tmp[e] = (gk + e < kend) ? src[e] : (bf16_t)0;
                     memcpy(&v, tmp, 16);
                 }
             }
-            *reinterpret_cast<uint4*>(&S[r][kc]) = v;
-        }
-    } else {    // G[k * ld + r]; 8-element chunks along r, transposed into S[r][k]
-        for (int c = tid; c < BK * (ROWS / 8); c += 256) {
-            const int k = c / (ROWS / 8), rc = (c % (ROWS / 8)) * 8;
+        } else {    // G[k * ld + r]; 8-element pieces along r, k fastest over the lanes
+            const int k = c % BK, rc = (c / BK) * 8;
             const int gk = k0 + k, gr = r0 + rc;
-            bf16_t tmp[8];
-            if (gk < kend && gr + 8 <= nrows && ((((uintptr_t)(G + (size_t)gk * ld + gr)) & 15) == 0)) {
-                uint4 v = *reinterpret_cast<const uint4*>(G + (size_t)gk * ld + gr);
-                memcpy(tmp, &v, 16);
-            } else {
-                for (int e = 0; e < 8; e++) tmp[e] = (gk < kend && gr + e < nrows) ? G[(size_t)gk * ld + gr + e] : (bf16_t)0;
+            if (gk < kend) {
+                const bf16_t* src = G + (size_t)gk * ld + gr;
+                if (gr + 8 <= nrows && (((uintptr_t)src) & 15) == 0) {
+                    v = *reinterpret_cast<const uint4*>(src);
+                } else {
+                    bf16_t tmp[8];
+                    for (int e = 0; e < 8; e++) tmp[e] = (gr + e < nrows) ? src[e] : (bf16_t)0;
+                    memcpy(&v, tmp, 16);
+                }
             }
+        }
+        R.v[p] = v;
+    }
+    return R;
+}
+template <bool KC, int ROWS>
+__device__ __forceinline__ void stage_store(bf16_t (*S)[LDK], const StageRegs& R, int tid) {
+#pragma unroll
+    for (int p = 0; p < STAGE_PIECES; p++) {
+        const int c = tid + 256 * p;
+        if (KC) {
+            const int r = c / (BK / 8), kc = (c % (BK / 8)) * 8;
+            *reinterpret_cast<uint4*>(&S[r][kc]) = R.v[p];
+        } else {
+            // k fastest over the lanes: each two-byte store of a wavefront covers 32 consecutive k
+            // of two rows (with r fastest, the rows 8 apart fell in one LDS bank)
+            const int k = c % BK, rc = (c / BK) * 8;
+            bf16_t tmp[8];
+            memcpy(tmp, &R.v[p], 16);
             for (int e = 0; e < 8; e++) S[rc + e][k] = tmp[e];
         }
     }
@@ -121,10 +152,16 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs a) {
     for (int i = 0; i < 4; i++)
 #pragma unroll
         for (int j = 0; j < 4; j++) acc[i][j] = (f32x4_t){0.f, 0.f, 0.f, 0.f};
+    StageRegs ra = stage_load<AKC, BM>(a.A, a.lda, m0, a.M, kbeg, kend, tid);
+    StageRegs rb = stage_load<BKC, BN>(a.B, a.ldb, n0, a.N, kbeg, kend, tid);
     for (int k0 = kbeg; k0 < kend; k0 += BK) {
-        stage<AKC, BM>(As, a.A, a.lda, m0, a.M, k0, kend, tid);
-        stage<BKC, BN>(Bs, a.B, a.ldb, n0, a.N, k0, kend, tid);
+        stage_store<AKC, BM>(As, ra, tid);
+        stage_store<BKC, BN>(Bs, rb, tid);
         __syncthreads();
+        if (k0 + BK < kend) {   // the next K step's loads, in flight during this step's MFMAs
+            ra = stage_load<AKC, BM>(a.A, a.lda, m0, a.M, k0 + BK, kend, tid);
+            rb = stage_load<BKC, BN>(a.B, a.ldb, n0, a.N, k0 + BK, kend, tid);
+        }
         bf16x8_t af[4], bfr[4];
 #pragma unroll
         for (int i = 0; i < 4; i++) af[i] = *reinterpret_cast<const bf16x8_t*>(&As[wm + 16 * i + fr][fk]);
@@ -1143,13 +1180,12 @@ int eu3d_mlp_train_step(eu3d_mlp* m, int n, const void* feats_bf16, const float*
     if (!m || n <= 0 || !feats_bf16 || !targets) return EU3D_ERR_ARG;
     hipStream_t s = m->stream;
     if (ensure_ws(m, n) != hipSuccess) return EU3D_ERR_HIP;
-    // partials: split-K dW [sp x 256 x 256] followed by the bias column sums [nb x 256]
     // partials: split-K dW [sp x 256 x 256], the bias column sums [nb x 256], then the first-level
-    // sums of the split-K partials in groups of 32 [ceil(sp / 32) x 256 x 256]
+    // sums in groups of 32, of the split-K partials [ceil(sp / 32) x 256 x 256] or (later, same
+    // space) of the column sums [ceil(nb / 32) x 256]
     const int sp = splits_for(n), rpb = 512, nb = (n + rpb - 1) / rpb, kgrp = 32;
-    const int ng = (sp + kgrp - 1) / kgrp;
-    if (ensure_part(m, (size_t)sp * NH * NH + (size_t)nb * NH + (size_t)ng * NH * NH) != hipSuccess)
-        return EU3D_ERR_HIP;
+    const size_t ng = (size_t)std::max((sp + kgrp - 1) / kgrp * NH * NH, (nb + kgrp - 1) / kgrp * NH);
+    if (ensure_part(m, (size_t)sp * NH * NH + (size_t)nb * NH + ng) != hipSuccess) return EU3D_ERR_HIP;
     if (hipMemcpyAsync(m->act[0], feats_bf16, (size_t)n * NIN * 2, hipMemcpyDeviceToDevice, s) != hipSuccess)
         return EU3D_ERR_HIP;
     if (forward_fused_train(m, n) != hipSuccess) return EU3D_ERR_HIP;
@@ -1186,7 +1222,14 @@ int eu3d_mlp_train_step(eu3d_mlp* m, int n, const void* feats_bf16, const float*
         // db = column sums of dZ
         float* cpart = m->part + (size_t)sp * NH * NH;
         k_colsum_bf16<<<nb, 256, 0, s>>>(dz, n, m->dout[l], rpb, cpart);
-        k_reduce_splits<<<(m->dout[l] + 255) / 256, 256, 0, s>>>(cpart, m->gb[l], m->dout[l], nb);
+        if (nb > kgrp) {   // two levels, as for the weights: one thread per column cannot sum nb rows fast
+            float* grp = m->part + (size_t)sp * NH * NH + (size_t)nb * NH;
+            const int ngr = (nb + kgrp - 1) / kgrp;
+            k_reduce_splits_grouped<<<dim3((m->dout[l] + 255) / 256, ngr), 256, 0, s>>>(cpart, grp, m->dout[l], nb, kgrp);
+            k_reduce_splits<<<(m->dout[l] + 255) / 256, 256, 0, s>>>(grp, m->gb[l], m->dout[l], ngr);
+        } else {
+            k_reduce_splits<<<(m->dout[l] + 255) / 256, 256, 0, s>>>(cpart, m->gb[l], m->dout[l], nb);
+        }
         if (l > 0) {
             // dZ_{l-1} = (dZ_l [n x dout] * W_l^T) .* GELU'(Z_{l-1}); B operand element
             // (n = i, k = o) = W[i][o] = wn[i * dout + o]
