@@ -338,12 +338,15 @@ struct MlpView {
 struct WPiece {
     uint4 a, b;
 };
-template <int N, int K>
+// KSRC: the row length of wt when it is shorter than the chunk (16 < 32: zeros beyond it)
+template <int N, int K, int KSRC = K>
 __device__ __forceinline__ WPiece load_w(const bf16_t* __restrict__ wt, int c, int tid) {
     WPiece r{make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0)};
     const int p0 = tid, p1 = tid + FT;
-    if (p0 < N * 4) r.a = *reinterpret_cast<const uint4*>(wt + (size_t)(p0 >> 2) * K + c * 32 + (p0 & 3) * 8);
-    if (p1 < N * 4) r.b = *reinterpret_cast<const uint4*>(wt + (size_t)(p1 >> 2) * K + c * 32 + (p1 & 3) * 8);
+    if (p0 < N * 4 && c * 32 + (p0 & 3) * 8 < KSRC)
+        r.a = *reinterpret_cast<const uint4*>(wt + (size_t)(p0 >> 2) * KSRC + c * 32 + (p0 & 3) * 8);
+    if (p1 < N * 4 && c * 32 + (p1 & 3) * 8 < KSRC)
+        r.b = *reinterpret_cast<const uint4*>(wt + (size_t)(p1 >> 2) * KSRC + c * 32 + (p1 & 3) * 8);
     return r;
 }
 template <int N>
@@ -362,7 +365,7 @@ struct TrainOut {
 
 // acc = act[the wavefront's 16 rows][0:K] x W^T[0:N][0:K]^T on MFMA, the weights streamed in
 // 32-column chunks through the double-buffered stage ws.
-template <int K, int N>
+template <int K, int N, int KSRC = K>
 __device__ __forceinline__ void fused_mfma(const bf16_t (*act)[FPITCH], bf16_t (*ws)[256][WPITCH],
                                            const bf16_t* __restrict__ wt, int tid, int lane, int wr,
                                            f32x4_t (&acc)[N / 16]) {
@@ -371,18 +374,18 @@ __device__ __forceinline__ void fused_mfma(const bf16_t (*act)[FPITCH], bf16_t (
 #pragma unroll
     for (int j = 0; j < NT; j++) acc[j] = (f32x4_t){0.f, 0.f, 0.f, 0.f};
     // chunk c waits in register set c % 2 from its load (two chunks ahead) to its LDS store
-    WPiece r0 = load_w<N, K>(wt, 0, tid), r1{};
-    if (KS > 1) r1 = load_w<N, K>(wt, 1, tid);
+    WPiece r0 = load_w<N, K, KSRC>(wt, 0, tid), r1{};
+    if (KS > 1) r1 = load_w<N, K, KSRC>(wt, 1, tid);
     __syncthreads();   // the previous layer is done with ws and act
     store_w<N>(ws[0], r0, tid);
-    if (KS > 2) r0 = load_w<N, K>(wt, 2, tid);
+    if (KS > 2) r0 = load_w<N, K, KSRC>(wt, 2, tid);
 #pragma unroll
     for (int ks = 0; ks < KS; ks++) {   // unrolled: the register set of each chunk is known
         __syncthreads();   // chunk ks staged; the other buffer's readers are done
         if (ks + 1 < KS) store_w<N>(ws[(ks + 1) & 1], ((ks + 1) & 1) ? r1 : r0, tid);
         if (ks + 3 < KS) {
-            if ((ks + 1) & 1) r1 = load_w<N, K>(wt, ks + 3, tid);
-            else r0 = load_w<N, K>(wt, ks + 3, tid);
+            if ((ks + 1) & 1) r1 = load_w<N, K, KSRC>(wt, ks + 3, tid);
+            else r0 = load_w<N, K, KSRC>(wt, ks + 3, tid);
         }
         const bf16x8_t a = *reinterpret_cast<const bf16x8_t*>(&act[wr + fr][ks * 32 + fk]);
 #pragma unroll
@@ -483,14 +486,25 @@ __global__ void __launch_bounds__(FT) k_mlp_fused(MlpView v, int n_max, const un
 
 // One wavefront's 16 rows x 256 columns between HBM and LDS in 16-byte pieces (8 per lane),
 // rows at or beyond n skipped (loads give zeros).
+// NCOL 16: rows of 16 columns, padded with zeros to the 32 of one K chunk.
+template <int NCOL = 256>
 __device__ __forceinline__ void rows_in(bf16_t (*act)[FPITCH], const bf16_t* __restrict__ g, long long row0, int wr,
                                         int lane, int n) {
+    static_assert(NCOL == 256 || NCOL == 16, "row widths of the network");
+    if constexpr (NCOL == 256) {
 #pragma unroll
-    for (int t = 0; t < 8; t++) {
-        const int c = lane + 64 * t, r = c >> 5, cc = (c & 31) * 8;
+        for (int t = 0; t < 8; t++) {
+            const int c = lane + 64 * t, r = c >> 5, cc = (c & 31) * 8;
+            const long long row = row0 + wr + r;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (row < n) v = *reinterpret_cast<const uint4*>(g + (size_t)row * 256 + cc);
+            *reinterpret_cast<uint4*>(&act[wr + r][cc]) = v;
+        }
+    } else {
+        const int r = lane >> 2, cc = (lane & 3) * 8;
         const long long row = row0 + wr + r;
         uint4 v = make_uint4(0, 0, 0, 0);
-        if (row < n) v = *reinterpret_cast<const uint4*>(g + (size_t)row * 256 + cc);
+        if (row < n && cc < 16) v = *reinterpret_cast<const uint4*>(g + (size_t)row * 16 + cc);
         *reinterpret_cast<uint4*>(&act[wr + r][cc]) = v;
     }
 }
@@ -509,16 +523,31 @@ __device__ __forceinline__ void rows_out(const bf16_t (*act)[FPITCH], bf16_t* __
 // [in x out], K-contiguous rows) stream through the forward kernel's double-buffered stage, then
 // the wavefront's rows of Z_{l-1} replace its dZ_l rows in LDS for the GELU' epilogue, whose
 // result leaves with 16-byte stores. Same arithmetic as the GEMM's DGELU epilogue.
+// KIN: the width of dZ_l, 256 for the hidden layers, 16 for the output layer (its W_l rows are 16
+// long; the MFMA chunk of 32 is padded with zeros).
+template <int KIN>
 __global__ void __launch_bounds__(FT) k_mlp_dx(const bf16_t* __restrict__ dz, const bf16_t* __restrict__ wn,
                                                const bf16_t* __restrict__ Z, bf16_t* __restrict__ out, int n) {
     __shared__ bf16_t act[FM][FPITCH];
     __shared__ bf16_t ws[2][256][WPITCH];
     const int tid = threadIdx.x, lane = tid & 63, wr = (tid >> 6) * 16, fr = lane & 15;
     for (long long row0 = (long long)blockIdx.x * FM; row0 < n; row0 += (long long)gridDim.x * FM) {
-        rows_in(act, dz, row0, wr, lane, n);
+        rows_in<KIN>(act, dz, row0, wr, lane, n);
+        // the wavefront's Z rows, loaded now and held in registers through the MFMAs
+        uint4 zr[8];
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
+            const int c = lane + 64 * t, r = c >> 5, cc = (c & 31) * 8;
+            const long long row = row0 + wr + r;
+            zr[t] = row < n ? *reinterpret_cast<const uint4*>(Z + (size_t)row * 256 + cc) : make_uint4(0, 0, 0, 0);
+        }
         f32x4_t acc[16];
-        fused_mfma<256, 256>(act, ws, wn, tid, lane, wr, acc);
-        rows_in(act, Z, row0, wr, lane, n);   // the wavefront read only its own rows above
+        fused_mfma<(KIN < 32 ? 32 : KIN), 256, KIN>(act, ws, wn, tid, lane, wr, acc);
+#pragma unroll
+        for (int t = 0; t < 8; t++) {   // the wavefront read only its own rows above
+            const int c = lane + 64 * t, r = c >> 5, cc = (c & 31) * 8;
+            *reinterpret_cast<uint4*>(&act[wr + r][cc]) = zr[t];
+        }
 #pragma unroll
         for (int j = 0; j < 16; j++) {
             const int col = 16 * j + fr;
@@ -1233,8 +1262,9 @@ int eu3d_mlp_train_step(eu3d_mlp* m, int n, const void* feats_bf16, const float*
         if (l > 0) {
             // dZ_{l-1} = (dZ_l [n x dout] * W_l^T) .* GELU'(Z_{l-1}); B operand element
             // (n = i, k = o) = W[i][o] = wn[i * dout + o]
-            if (m->dout[l] == NH && m->din[l] == NH) {   // hidden layers: the fused row-tile kernel
-                k_mlp_dx<<<(n + FM - 1) / FM, FT, 0, s>>>(dz, m->wn[l], m->zs[l], dz_next, n);
+            if (m->din[l] == NH && (m->dout[l] == NH || m->dout[l] == NOUT)) {   // the fused row-tile kernel
+                if (m->dout[l] == NH) k_mlp_dx<NH><<<(n + FM - 1) / FM, FT, 0, s>>>(dz, m->wn[l], m->zs[l], dz_next, n);
+                else k_mlp_dx<NOUT><<<(n + FM - 1) / FM, FT, 0, s>>>(dz, m->wn[l], m->zs[l], dz_next, n);
                 if (hipGetLastError() != hipSuccess) return EU3D_ERR_HIP;
             } else {
                 GemmArgs d{};
