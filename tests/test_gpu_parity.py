@@ -185,3 +185,32 @@ def test_steps_imex_bubble_ref():
     g.step(**kw)
     o.step(**kw)
     compare(g, o, ["CS", "T", "D", "P", "E", "YI"])
+
+
+def test_steps_imex_ragged_grid():
+    """x rows of 70 cells (one full and one partial 64-cell segment) and 7 y rows (a partial
+    block of the 64 x 4 interior blocks): the lanes and rows past the interior compute nothing
+    that is stored."""
+    say("case imex ragged 70x7x5")
+    from eu3d_amd import configs
+    wl = configs.bubble3d(ni=70, nj=7, nk=5, dx=1e-4, radius_cells=3.0, shock_cells=5, center_cells=(30, 3, 2),
+                          seed=11)
+    g, o = make_pair(wl, Q_FIXED)
+    kw = dict(nsteps=2, cfl=0.1, bc_kinds=(0,) * 6, rk=1, chem=1)
+    g.step(**kw)
+    o.step(**kw)
+    assert g.check_finite() == 0
+    compare(g, o, ["CS", "T", "D", "P", "E", "YI"])
+
+
+@pytest.mark.parametrize("chem", [1, 3])
+def test_steps_zn1d_2000(chem):
+    """The reference's CPU-sized case: the 1-D ZND detonation tube of 2000 cells."""
+    say("case zn1d 2000 chem", chem)
+    wl = small_zn(2000)
+    g, o = make_pair(wl, Q_FIXED)
+    kw = dict(nsteps=3, cfl=0.05, bc_kinds=wl.bc_kinds, rk=1, chem=chem, nchem_max=200)
+    g.step(**kw)
+    o.step(**kw)
+    assert g.check_finite() == 0
+    compare(g, o, ["CS", "T", "D", "P", "E", "YI"])
