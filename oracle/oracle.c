@@ -8,6 +8,11 @@
  * parity is defined against that arithmetic. Built with -O2 -fno-fast-math
  * -ffp-contract=off so no FMA contraction or reassociation changes the rounding.
  * Never-written reference scratch (quirks F1b, I2) is taken as zero.
+ *
+ * Parity unpinned: what the reference leaves to uninitialised memory has no check to pin it:
+ * MD(c,c) for c >= NS under quirk I2 (taken as 0), the F1b face states for i in [ni, ni+bc)
+ * (taken as zero-filled) and F5's shallow copies (taken as deep copies). Every function has its
+ * own pins in tests/test_oracle_golden.py.
  */
 #include "oracle.h"
 

@@ -485,6 +485,59 @@ def test_ghost_kinds_closed_forms():
             assert np.array_equal(after[f][bc:bc + ni, J, K], before[f][bc:bc + ni, J, K])
 
 
+def _nasa_mix(T, Y):
+    """Mixture R, c_p, h and e [J/kg(/K)] from the NASA-9 fits, computed here in numpy (range by
+    T < 1000 as src/Reaction.cpp:893-926, FIXED: the high range keeps its own b1)."""
+    ri = R_UNIV / M.mw                                     # kJ/kg/K
+    lo = (T < 1000.0)[..., None, None]
+    a = np.where(lo, M.coeff0.T, M.coeff1.T)               # (..., ns, 9)
+    t = T[..., None]
+    cp = a[..., 0] / t ** 2 + a[..., 1] / t + a[..., 2] + a[..., 3] * t + a[..., 4] * t ** 2 + \
+        a[..., 5] * t ** 3 + a[..., 6] * t ** 4
+    h = -a[..., 0] / t + a[..., 1] * np.log(t) + a[..., 2] * t + a[..., 3] * t ** 2 / 2 + \
+        a[..., 4] * t ** 3 / 3 + a[..., 5] * t ** 4 / 4 + a[..., 6] * t ** 5 / 5 + a[..., 7]
+    R = (Y * ri).sum(-1) * 1000
+    return R, (Y * ri * cp).sum(-1) * 1000, (Y * ri * h).sum(-1) * 1000, (Y * ri * (h - t)).sum(-1) * 1000
+
+
+def test_rsbi_ghosts_closed_forms():
+    """FieldBoundary_3dRSBI (src/Flowfield.cpp:559-773) in closed form: every face mirrors
+    (ghost at distance g + 1 takes the cell at distance g from the face inside; no sign change);
+    x faces over the interior j, k, y faces over all i and the interior k, z faces over all i, j,
+    in that order; the y-low face copies U, V, W, P, T, Y and Mc and recomputes R, rho = P / (R T),
+    c_p, h, e and gamma = c_p / (c_p - R) from the NASA fits, the other faces copy rho, H, gamma
+    and E too."""
+    ni, nj, nk, bc = 6, 5, 4, 2
+    o = flow(ni=ni, nj=nj, nk=nk)
+    random_prims(o, seed=21)
+    o.fill_ghosts((0,) * 6)
+    f = {k: o.get(k) for k in PRIM_FIELDS + ("MC", "RGAS", "CP")}
+    NX, NY, NZ = ni + 2 * bc, nj + 2 * bc, nk + 2 * bc
+    I, J, K = slice(bc, ni + bc), slice(bc, nj + bc), slice(bc, nk + bc)
+    copied = ("U", "V", "W", "P", "D", "T", "H", "GAMMA", "E", "YI")
+    for g in range(bc):
+        for k in copied:   # x faces, interior j and k
+            assert np.array_equal(f[k][bc - 1 - g, J, K], f[k][bc + g, J, K]), ("x-", k, g)
+            assert np.array_equal(f[k][ni + bc + g, J, K], f[k][ni + bc - 1 - g, J, K]), ("x+", k, g)
+        for k in copied:   # y-high face, all i, interior k
+            assert np.array_equal(f[k][:, nj + bc + g, K], f[k][:, nj + bc - 1 - g, K]), ("y+", k, g)
+        for k in ("U", "V", "W", "P", "T", "YI", "MC"):   # y-low face copies
+            assert np.array_equal(f[k][:, bc - 1 - g, K], f[k][:, bc + g, K]), ("y-", k, g)
+        for k in copied:   # z faces, all i and j
+            assert np.array_equal(f[k][:, :, bc - 1 - g], f[k][:, :, bc + g]), ("z-", k, g)
+            assert np.array_equal(f[k][:, :, nk + bc + g], f[k][:, :, nk + bc - 1 - g]), ("z+", k, g)
+    # y-low recompute from the copied T, P, Y
+    sl = (slice(0, NX), slice(0, bc), K)
+    T, P, Y = f["T"][sl], f["P"][sl], f["YI"][sl]
+    R, cp, h, e = _nasa_mix(T, Y)
+    for name, want, got in (("RGAS", R, f["RGAS"][sl]), ("D", P / (T * R), f["D"][sl]), ("CP", cp, f["CP"][sl]),
+                            ("H", h, f["H"][sl]), ("E", e, f["E"][sl]),
+                            ("GAMMA", cp / (cp - R), f["GAMMA"][sl])):
+        assert np.allclose(got, want, rtol=1e-12, atol=0), (name, np.abs(got / want - 1).max())
+    # the recompute is not a copy: the source cells hold other values
+    assert not np.array_equal(f["D"][:, bc - 1, K], f["D"][:, bc, K])
+
+
 def test_partial_t_matches_finite_difference():
     """GetPartial_T (FIXED units): dT/drho_s at constant rho u and rho E equals the central finite
     difference of the energy Newton, (1/2 |u|^2 - e_s) / (rho c_v)."""
