@@ -244,6 +244,154 @@ __global__ void k_fill_z(G3 g, Fields F, int klo, int khi) {
     copy_prims(F, g, IDX(g, i, j, dk), IDX(g, i, j, sk), kind == EU3D_BC_REFLECT ? 3 : 0);
 }
 
+// ---------------------------------------------------------------- ghost fill, one launch
+// The three face passes above composed per ghost cell, for grids whose interior extents are all
+// at least bc (then every source of every face is an interior cell, and no cell this launch
+// writes is read by it). A ghost cell's final value is that of one interior cell, carried through
+// the passes that write it in the reference's order x, y, z: x copies over the interior (j, k), y
+// over all i and the interior k (RSBI on y- copies U, V, W, P, T, Y, Mc and recomputes the
+// thermodynamic state), z over all (i, j). Regions: z-ghost planes (all i, j), y-ghost rows
+// (interior k, all i), x-ghost columns (interior j, k).
+struct GhostCell {
+    double U, V, W, P, D, T, H, G, E, Y[NS];
+};
+__device__ __forceinline__ GhostCell gc_load(const Fields& F, const G3& g, size_t c) {
+    GhostCell r;
+    r.U = FLD(F, EU3D_FIELD_U)[c]; r.V = FLD(F, EU3D_FIELD_V)[c]; r.W = FLD(F, EU3D_FIELD_W)[c];
+    r.P = FLD(F, EU3D_FIELD_P)[c]; r.D = FLD(F, EU3D_FIELD_D)[c]; r.T = FLD(F, EU3D_FIELD_T)[c];
+    r.H = FLD(F, EU3D_FIELD_H)[c]; r.G = FLD(F, EU3D_FIELD_GAMMA)[c]; r.E = FLD(F, EU3D_FIELD_E)[c];
+#pragma unroll
+    for (int s = 0; s < NS; s++) r.Y[s] = FLD(F, EU3D_FIELD_YI)[s * g.n3 + c];
+    return r;
+}
+// the copy set of the face passes (copy_prims)
+__device__ __forceinline__ void gc_store(const Fields& F, const G3& g, size_t c, const GhostCell& r) {
+    FLD(F, EU3D_FIELD_U)[c] = r.U; FLD(F, EU3D_FIELD_V)[c] = r.V; FLD(F, EU3D_FIELD_W)[c] = r.W;
+    FLD(F, EU3D_FIELD_P)[c] = r.P; FLD(F, EU3D_FIELD_D)[c] = r.D; FLD(F, EU3D_FIELD_T)[c] = r.T;
+    FLD(F, EU3D_FIELD_H)[c] = r.H; FLD(F, EU3D_FIELD_GAMMA)[c] = r.G; FLD(F, EU3D_FIELD_E)[c] = r.E;
+#pragma unroll
+    for (int s = 0; s < NS; s++) FLD(F, EU3D_FIELD_YI)[s * g.n3 + c] = r.Y[s];
+}
+// RSBI y-: R, rho, c_p, h, e, gamma from (p, T, Y), in k_fill_y's arithmetic
+__device__ __forceinline__ void rsbi_state(GhostCell& r, uint32_t q, double& Rg, double& Cp) {
+    const double T = r.T;
+    const TPow p = tpow(T);
+    const bool lo = T < 1000;
+    double rg = 0, cp = 0, h = 0, e = 0;
+#pragma unroll
+    for (int sp = 0; sp < NS; sp++) {
+        const double y = r.Y[sp];
+        const double hi = RI(sp) * h_R(sp, lo, q & Q_C4, p);
+        rg += y * RI(sp);
+        cp += RI(sp) * cp_R(sp, lo, p) * y;
+        h += y * hi;
+        e += y * (hi - RI(sp) * T);
+    }
+    Rg = rg * 1000;
+    Cp = cp * 1000;
+    r.D = r.P / (T * Rg);
+    r.H = h * 1000;
+    r.E = e * 1000;
+    r.G = Cp / (Cp - Rg);
+}
+// the source index along one dimension of ghost index t, or -1 where that face is not filled
+__device__ __forceinline__ int gsrc(int t, int n, int bc, int klo, int khi, int& kind) {
+    if (t < bc) {
+        kind = klo;
+        return klo == EU3D_BC_NONE ? -1 : ghost_src(klo, true, t, n, bc);
+    }
+    kind = khi;
+    return khi == EU3D_BC_NONE ? -1 : ghost_src(khi, false, t - (n + bc), n, bc);
+}
+struct GhostKinds {
+    int k[6];
+};
+__device__ __forceinline__ bool ghost_dim(int t, int n, int bc) { return t < bc || t >= n + bc; }
+
+// (i, j, k) after the x pass, j and k interior
+__device__ __forceinline__ GhostCell gc_after_x(const Fields& F, const G3& g, const GhostKinds& K, int i, int j, int k) {
+    if (ghost_dim(i, g.ni, g.bc)) {
+        int kind;
+        const int si = gsrc(i, g.ni, g.bc, K.k[0], K.k[1], kind);
+        if (si >= 0) {
+            GhostCell r = gc_load(F, g, IDX(g, si, j, k));
+            if (kind == EU3D_BC_REFLECT) r.U = -r.U;
+            return r;
+        }
+    }
+    return gc_load(F, g, IDX(g, i, j, k));
+}
+// (i, j, k) after the x and y passes, k interior
+__device__ __forceinline__ GhostCell gc_after_xy(const Fields& F, const G3& g, const GhostKinds& K, uint32_t q, int i,
+                                                 int j, int k) {
+    if (ghost_dim(j, g.nj, g.bc)) {
+        int kind;
+        const int sj = gsrc(j, g.nj, g.bc, K.k[2], K.k[3], kind);
+        if (sj >= 0) {
+            GhostCell r = gc_after_x(F, g, K, i, sj, k);
+            if (kind == EU3D_BC_RSBI && j < g.bc) {
+                double Rg, Cp;
+                rsbi_state(r, q, Rg, Cp);
+            } else if (kind == EU3D_BC_REFLECT) {
+                r.V = -r.V;
+            }
+            return r;
+        }
+        return gc_load(F, g, IDX(g, i, j, k));
+    }
+    return gc_after_x(F, g, K, i, j, k);
+}
+
+__global__ void __launch_bounds__(256) k_fill_all(G3 g, Fields F, GhostKinds K, uint32_t q) {
+    const long long nR1 = (long long)g.NX * g.NY * (2 * g.bc), nR2 = (long long)g.NX * (2 * g.bc) * g.nk;
+    const long long nR3 = (long long)(2 * g.bc) * g.nj * g.nk;
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < nR1) {   // z-ghost planes, all (i, j)
+        const int i = (int)(t % g.NX), j = (int)((t / g.NX) % g.NY), kg = (int)(t / ((long long)g.NX * g.NY));
+        const int k = kg < g.bc ? kg : g.nk + kg;
+        int kind;
+        const int sk = gsrc(k, g.nk, g.bc, K.k[4], K.k[5], kind);
+        if (sk < 0) return;
+        GhostCell r = gc_after_xy(F, g, K, q, i, j, sk);
+        if (kind == EU3D_BC_REFLECT) r.W = -r.W;
+        gc_store(F, g, IDX(g, i, j, k), r);
+        return;
+    }
+    t -= nR1;
+    if (t < nR2) {   // y-ghost rows, interior k, all i
+        const int i = (int)(t % g.NX), jg = (int)((t / g.NX) % (2 * g.bc)), k = g.bc + (int)(t / ((long long)g.NX * 2 * g.bc));
+        const int j = jg < g.bc ? jg : g.nj + jg;
+        int kind;
+        const int sj = gsrc(j, g.nj, g.bc, K.k[2], K.k[3], kind);
+        if (sj < 0) return;
+        GhostCell r = gc_after_x(F, g, K, i, sj, k);
+        const size_t d = IDX(g, i, j, k);
+        if (kind == EU3D_BC_RSBI && j < g.bc) {
+            double Rg, Cp;
+            rsbi_state(r, q, Rg, Cp);
+            gc_store(F, g, d, r);
+            FLD(F, EU3D_FIELD_RGAS)[d] = Rg;
+            FLD(F, EU3D_FIELD_CP)[d] = Cp;
+            const size_t s = IDX(g, i, sj, k);   // Mc is copied from the mirror cell as it stands
+#pragma unroll
+            for (int sp = 0; sp < NS; sp++) FLD(F, EU3D_FIELD_MC)[sp * g.n3 + d] = FLD(F, EU3D_FIELD_MC)[sp * g.n3 + s];
+        } else {
+            if (kind == EU3D_BC_REFLECT) r.V = -r.V;
+            gc_store(F, g, d, r);
+        }
+        return;
+    }
+    t -= nR2;
+    if (t < nR3) {   // x-ghost columns, interior (j, k)
+        const int ig = (int)(t % (2 * g.bc)), j = g.bc + (int)((t / (2 * g.bc)) % g.nj);
+        const int k = g.bc + (int)(t / ((long long)2 * g.bc * g.nj));
+        const int i = ig < g.bc ? ig : g.ni + ig;
+        int kind;
+        if (gsrc(i, g.ni, g.bc, K.k[0], K.k[1], kind) < 0) return;
+        gc_store(F, g, IDX(g, i, j, k), gc_after_x(F, g, K, i, j, k));
+    }
+}
+
 // ---------------------------------------------------------------- advection
 // One face per thread: MUSCL on the 16 primitives from the 4-cell stencil, then the AUSM flux
 // (Flowfield::AUSM, src/Flowfield.cpp:1330-1502). Faces along dir cover [bc-1, n+bc); the other
@@ -1684,6 +1832,14 @@ hipError_t launch_partial_t(eu3d_ctx* c) {
 
 hipError_t launch_fill_ghosts(eu3d_ctx* c, const int kinds[6]) {
     const G3& g = c->g;
+    if (g.ni >= g.bc && g.nj >= g.bc && g.nk >= g.bc) {   // else the three passes (a source may be a ghost)
+        GhostKinds K;
+        for (int f = 0; f < 6; f++) K.k[f] = kinds[f];
+        const long long n = (long long)g.NX * g.NY * (2 * g.bc) + (long long)g.NX * (2 * g.bc) * g.nk +
+                            (long long)(2 * g.bc) * g.nj * g.nk;
+        k_fill_all<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(g, c->fl, K, c->quirks);
+        return hipGetLastError();
+    }
     k_fill_x<<<dim3((g.nj + 63) / 64, g.nk, 2 * g.bc), 64, 0, c->stream>>>(g, c->fl, kinds[0], kinds[1]);
     k_fill_y<<<dim3((g.NX + 63) / 64, g.nk, 2 * g.bc), 64, 0, c->stream>>>(g, c->fl, kinds[2], kinds[3], c->quirks);
     k_fill_z<<<dim3((g.NX + 63) / 64, g.NY, 2 * g.bc), 64, 0, c->stream>>>(g, c->fl, kinds[4], kinds[5]);

@@ -57,15 +57,28 @@ def test_advection_fused(quirks, limiter):
     compare(g, o, ["RHS"])
 
 
-@pytest.mark.parametrize("kinds", [(0,) * 6, (1, 1, 2, 2, 2, 2), (2, 1, 3, 3, 3, 3)])
+@pytest.mark.parametrize("kinds", [(0,) * 6, (1, 1, 2, 2, 2, 2), (2, 1, 3, 3, 3, 3), (2, 3, 0, 1, 3, 2),
+                                   (4, 0, 0, 4, 1, 4), (3, 3, 0, 2, 0, 0)])
 def test_fill_ghosts(kinds):
+    """All ghost cells, corners included, against the oracle's three face passes (the GPU fills a
+    grid like this one in one launch that composes them)."""
     say("case fill_ghosts", kinds)
     q = Q_REF if kinds[0] == 0 else Q_FIXED
     g, o = make_pair(small_bubble(), q)
     for obj in (g, o):
         obj.fill_ghosts(kinds)
     whole = lambda x: x
-    compare(g, o, ["U", "V", "W", "P", "D", "T", "H", "GAMMA", "E", "YI"], region=whole)
+    compare(g, o, ["U", "V", "W", "P", "D", "T", "H", "GAMMA", "E", "YI", "MC", "RGAS", "CP"], region=whole)
+
+
+def test_fill_ghosts_thin_grid():
+    """A grid one cell thick in y and z (sources there can be ghosts): the three face passes."""
+    say("case fill_ghosts thin")
+    wl = small_zn()
+    g, o = make_pair(wl, Q_FIXED)
+    for obj in (g, o):
+        obj.fill_ghosts((0, 2, 1, 1, 1, 1))
+    compare(g, o, ["U", "V", "W", "P", "D", "T", "H", "GAMMA", "E", "YI"], region=lambda x: x)
 
 
 @pytest.mark.parametrize("quirks", [Q_FIXED, Q_REF])
