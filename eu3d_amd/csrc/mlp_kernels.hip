@@ -363,16 +363,35 @@ struct TrainOut {
     bf16_t* a;
 };
 
-// acc = act[the wavefront's 16 rows][0:K] x W^T[0:N][0:K]^T on MFMA, the weights streamed in
-// 32-column chunks through the double-buffered stage ws.
+// How the eight wavefronts split a layer's FM x N output: RG row groups x CG column groups,
+// each wavefront RT x CT tiles of 16 x 16. For N = 256, 4 x 2 groups of 2 x 8 tiles: a
+// wavefront reads 2 + 8 fragments per 32-deep K chunk for 16 MFMAs (16 rows x all columns read
+// 1 + 16), which keeps the LDS reads under what the MFMAs consume.
+template <int N>
+struct WaveTile {
+    static constexpr int CT = N >= 128 ? 8 : N / 16, CG = N / (16 * CT), RG = 8 / CG, RT = FM / (16 * RG);
+    static_assert(CG * RG == 8 && RT * 16 * RG == FM, "eight wavefronts over FM rows");
+    int wr, wc;   // first row and column of the wavefront's block
+    __device__ __forceinline__ explicit WaveTile(int tid) {
+        const int w = tid >> 6;
+        wr = (w / CG) * RT * 16;
+        wc = (w % CG) * CT * 16;
+    }
+};
+
+// acc = act[the wavefront's rows][0:K] x W^T[its columns][0:K]^T on MFMA, the weights streamed
+// in 32-column chunks through the double-buffered stage ws. With CG > 1 two wavefronts read the
+// same rows of act, so the function ends with a barrier before anyone writes them.
 template <int K, int N, int KSRC = K>
 __device__ __forceinline__ void fused_mfma(const bf16_t (*act)[FPITCH], bf16_t (*ws)[256][WPITCH],
-                                           const bf16_t* __restrict__ wt, int tid, int lane, int wr,
-                                           f32x4_t (&acc)[N / 16]) {
-    constexpr int NT = N / 16, KS = K / 32;
+                                           const bf16_t* __restrict__ wt, int tid, int lane, const WaveTile<N>& wt_,
+                                           f32x4_t (&acc)[WaveTile<N>::RT][WaveTile<N>::CT]) {
+    constexpr int RT = WaveTile<N>::RT, CT = WaveTile<N>::CT, KS = K / 32;
     const int fr = lane & 15, fk = (lane >> 4) * 8;
 #pragma unroll
-    for (int j = 0; j < NT; j++) acc[j] = (f32x4_t){0.f, 0.f, 0.f, 0.f};
+    for (int i = 0; i < RT; i++)
+#pragma unroll
+        for (int j = 0; j < CT; j++) acc[i][j] = (f32x4_t){0.f, 0.f, 0.f, 0.f};
     // chunk c waits in register set c % 2 from its load (two chunks ahead) to its LDS store
     WPiece r0 = load_w<N, K, KSRC>(wt, 0, tid), r1{};
     if (KS > 1) r1 = load_w<N, K, KSRC>(wt, 1, tid);
@@ -387,50 +406,62 @@ __device__ __forceinline__ void fused_mfma(const bf16_t (*act)[FPITCH], bf16_t (
             if ((ks + 1) & 1) r1 = load_w<N, K, KSRC>(wt, ks + 3, tid);
             else r0 = load_w<N, K, KSRC>(wt, ks + 3, tid);
         }
-        const bf16x8_t a = *reinterpret_cast<const bf16x8_t*>(&act[wr + fr][ks * 32 + fk]);
+        bf16x8_t a[RT];
 #pragma unroll
-        for (int j = 0; j < NT; j++) {
-            const bf16x8_t b = *reinterpret_cast<const bf16x8_t*>(&ws[ks & 1][16 * j + fr][fk]);
-            acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[j], 0, 0, 0);
+        for (int i = 0; i < RT; i++) a[i] = *reinterpret_cast<const bf16x8_t*>(&act[wt_.wr + 16 * i + fr][ks * 32 + fk]);
+#pragma unroll
+        for (int j = 0; j < CT; j++) {
+            const bf16x8_t b = *reinterpret_cast<const bf16x8_t*>(&ws[ks & 1][wt_.wc + 16 * j + fr][fk]);
+#pragma unroll
+            for (int i = 0; i < RT; i++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b, acc[i][j], 0, 0, 0);
         }
     }
+    if (WaveTile<N>::CG > 1) __syncthreads();
 }
 
 template <int K, int N, bool LAST, bool TRAIN = false>
 __device__ __forceinline__ void fused_layer(bf16_t (*act)[FPITCH], bf16_t (*ws)[256][WPITCH], const bf16_t* __restrict__ wt,
-                                            const float* __restrict__ bias, int tid, int lane, int wr, long long row0,
+                                            const float* __restrict__ bias, int tid, int lane, long long row0,
                                             int n, float* __restrict__ Y, TrainOut tr = {}) {
-    constexpr int NT = N / 16;
+    using TL = WaveTile<N>;
+    constexpr int RT = TL::RT, CT = TL::CT;
+    const TL tl(tid);
     const int fr = lane & 15;
-    f32x4_t acc[NT];
-    fused_mfma<K, N>(act, ws, wt, tid, lane, wr, acc);
-    // D fragment: lane holds rows 4 * (lane / 16) + r, column lane % 16 of each 16 x 16 tile
+    f32x4_t acc[RT][CT];
+    fused_mfma<K, N>(act, ws, wt, tid, lane, tl, acc);
+    // D fragment: lane holds rows 4 * (lane / 16) + r, column lane % 16 of each 16 x 16 tile; the
+    // wavefront writes only its own block of act
     if constexpr (!LAST && !TRAIN) {
 #pragma unroll
-        for (int j = 0; j < NT; j++) {
-            const int col = 16 * j + fr;
+        for (int j = 0; j < CT; j++) {
+            const int col = tl.wc + 16 * j + fr;
             const float bv = bias[col];
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const bf16_t zb = f2bf(acc[j][r] + bv);
-                act[wr + 4 * (lane >> 4) + r][col] = f2bf(gelu(bf2f(zb)));
-            }
+            for (int i = 0; i < RT; i++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const bf16_t zb = f2bf(acc[i][j][r] + bv);
+                    act[tl.wr + 16 * i + 4 * (lane >> 4) + r][col] = f2bf(gelu(bf2f(zb)));
+                }
         }
     } else if constexpr (!LAST) {
-        // Z into the wavefront's own rows of act; then, in 16-byte pieces (row-contiguous, so the
+        // Z into the wavefront's block of act; then, in 16-byte pieces (row-contiguous, so the
         // global stores coalesce), Z out to HBM, GELU(Z) back into act and out to HBM
 #pragma unroll
-        for (int j = 0; j < NT; j++) {
-            const int col = 16 * j + fr;
+        for (int j = 0; j < CT; j++) {
+            const int col = tl.wc + 16 * j + fr;
             const float bv = bias[col];
 #pragma unroll
-            for (int r = 0; r < 4; r++) act[wr + 4 * (lane >> 4) + r][col] = f2bf(acc[j][r] + bv);
+            for (int i = 0; i < RT; i++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) act[tl.wr + 16 * i + 4 * (lane >> 4) + r][col] = f2bf(acc[i][j][r] + bv);
         }
-        static_assert(N == 256, "16 rows x 256 columns: 8 pieces per lane");
+        constexpr int PR = CT * 2;   // 16-byte pieces per row of the block
+        static_assert(RT * 16 * PR == 8 * 64, "8 pieces per lane");
 #pragma unroll
         for (int t = 0; t < 8; t++) {
-            const int c = lane + 64 * t, r = c >> 5, cc = (c & 31) * 8;
-            uint4 zv = *reinterpret_cast<const uint4*>(&act[wr + r][cc]);
+            const int c = lane + 64 * t, r = c / PR, cc = tl.wc + (c % PR) * 8;
+            uint4 zv = *reinterpret_cast<const uint4*>(&act[tl.wr + r][cc]);
             const unsigned zw[4] = {zv.x, zv.y, zv.z, zv.w};
             unsigned gw[4];
 #pragma unroll
@@ -440,18 +471,19 @@ __device__ __forceinline__ void fused_layer(bf16_t (*act)[FPITCH], bf16_t (*ws)[
                 gw[q] = (unsigned)lo | ((unsigned)hi << 16);
             }
             const uint4 gv = make_uint4(gw[0], gw[1], gw[2], gw[3]);
-            *reinterpret_cast<uint4*>(&act[wr + r][cc]) = gv;
-            const long long row = row0 + wr + r;
+            *reinterpret_cast<uint4*>(&act[tl.wr + r][cc]) = gv;
+            const long long row = row0 + tl.wr + r;
             if (row < n) {
                 *reinterpret_cast<uint4*>(tr.z + (size_t)row * N + cc) = zv;
                 *reinterpret_cast<uint4*>(tr.a + (size_t)row * N + cc) = gv;
             }
         }
     } else {
+        static_assert(RT == 1 && CT == 1, "the output layer: one tile per wavefront");
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-            const long long row = row0 + wr + 4 * (lane >> 4) + r;
-            if (row < n) Y[row * N + fr] = acc[0][r] + bias[fr];
+            const long long row = row0 + tl.wr + 4 * (lane >> 4) + r;
+            if (row < n) Y[row * N + fr] = acc[0][0][r] + bias[fr];
         }
     }
 }
@@ -465,7 +497,7 @@ __global__ void __launch_bounds__(FT) k_mlp_fused(MlpView v, int n_max, const un
                                                   TrainOut tr3 = {}) {
     __shared__ bf16_t act[FM][FPITCH];
     __shared__ bf16_t ws[2][256][WPITCH];
-    const int tid = threadIdx.x, lane = tid & 63, wr = (tid >> 6) * 16;
+    const int tid = threadIdx.x, lane = tid & 63;
     const int n = count ? min(n_max, (int)*count) : n_max;
     // persistent over the row tiles: the grid is sized for n_max, the work by the device count
     for (long long row0 = (long long)blockIdx.x * FM; row0 < n; row0 += (long long)gridDim.x * FM) {
@@ -476,11 +508,11 @@ __global__ void __launch_bounds__(FT) k_mlp_fused(MlpView v, int n_max, const un
             if (row0 + r < n) val = *reinterpret_cast<const uint4*>(X + (size_t)(row0 + r) * 32 + kc);
             *reinterpret_cast<uint4*>(&act[r][kc]) = val;
         }
-        fused_layer<32, 256, false, TRAIN>(act, ws, v.wt[0], v.b[0], tid, lane, wr, row0, n, Y, tr0);
-        fused_layer<256, 256, false, TRAIN>(act, ws, v.wt[1], v.b[1], tid, lane, wr, row0, n, Y, tr1);
-        fused_layer<256, 256, false, TRAIN>(act, ws, v.wt[2], v.b[2], tid, lane, wr, row0, n, Y, tr2);
-        fused_layer<256, 256, false, TRAIN>(act, ws, v.wt[3], v.b[3], tid, lane, wr, row0, n, Y, tr3);
-        fused_layer<256, 16, true>(act, ws, v.wt[4], v.b[4], tid, lane, wr, row0, n, Y);
+        fused_layer<32, 256, false, TRAIN>(act, ws, v.wt[0], v.b[0], tid, lane, row0, n, Y, tr0);
+        fused_layer<256, 256, false, TRAIN>(act, ws, v.wt[1], v.b[1], tid, lane, row0, n, Y, tr1);
+        fused_layer<256, 256, false, TRAIN>(act, ws, v.wt[2], v.b[2], tid, lane, row0, n, Y, tr2);
+        fused_layer<256, 256, false, TRAIN>(act, ws, v.wt[3], v.b[3], tid, lane, row0, n, Y, tr3);
+        fused_layer<256, 16, true>(act, ws, v.wt[4], v.b[4], tid, lane, row0, n, Y);
     }
 }
 
@@ -530,10 +562,14 @@ __global__ void __launch_bounds__(FT) k_mlp_dx(const bf16_t* __restrict__ dz, co
                                                const bf16_t* __restrict__ Z, bf16_t* __restrict__ out, int n) {
     __shared__ bf16_t act[FM][FPITCH];
     __shared__ bf16_t ws[2][256][WPITCH];
+    // the row slices (16 rows per wavefront) move rows between HBM and LDS; the MFMA blocks
+    // (WaveTile) span two slices, so the passes between them are separated by barriers
     const int tid = threadIdx.x, lane = tid & 63, wr = (tid >> 6) * 16, fr = lane & 15;
+    using TL = WaveTile<256>;
+    const TL tl(tid);
     for (long long row0 = (long long)blockIdx.x * FM; row0 < n; row0 += (long long)gridDim.x * FM) {
         rows_in<KIN>(act, dz, row0, wr, lane, n);
-        // the wavefront's Z rows, loaded now and held in registers through the MFMAs
+        // the wavefront's slice of Z, loaded now and held in registers through the MFMAs
         uint4 zr[8];
 #pragma unroll
         for (int t = 0; t < 8; t++) {
@@ -541,23 +577,28 @@ __global__ void __launch_bounds__(FT) k_mlp_dx(const bf16_t* __restrict__ dz, co
             const long long row = row0 + wr + r;
             zr[t] = row < n ? *reinterpret_cast<const uint4*>(Z + (size_t)row * 256 + cc) : make_uint4(0, 0, 0, 0);
         }
-        f32x4_t acc[16];
-        fused_mfma<(KIN < 32 ? 32 : KIN), 256, KIN>(act, ws, wn, tid, lane, wr, acc);
+        f32x4_t acc[TL::RT][TL::CT];
+        fused_mfma<(KIN < 32 ? 32 : KIN), 256, KIN>(act, ws, wn, tid, lane, tl, acc);   // ends with a barrier
 #pragma unroll
-        for (int t = 0; t < 8; t++) {   // the wavefront read only its own rows above
+        for (int t = 0; t < 8; t++) {
             const int c = lane + 64 * t, r = c >> 5, cc = (c & 31) * 8;
             *reinterpret_cast<uint4*>(&act[wr + r][cc]) = zr[t];
         }
+        __syncthreads();   // every slice of Z staged
 #pragma unroll
-        for (int j = 0; j < 16; j++) {
-            const int col = 16 * j + fr;
+        for (int j = 0; j < TL::CT; j++) {
+            const int col = tl.wc + 16 * j + fr;
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
-                bf16_t& e = act[wr + 4 * (lane >> 4) + r][col];
-                e = f2bf(acc[j][r] * dgelu(bf2f(e)));
-            }
+            for (int i = 0; i < TL::RT; i++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    bf16_t& e = act[tl.wr + 16 * i + 4 * (lane >> 4) + r][col];
+                    e = f2bf(acc[i][j][r] * dgelu(bf2f(e)));
+                }
         }
+        __syncthreads();   // every block of the result written
         rows_out(act, out, row0, wr, lane, n);
+        __syncthreads();   // the next tile's rows_in overwrites act
     }
 }
 
