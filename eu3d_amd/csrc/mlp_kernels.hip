@@ -76,20 +76,21 @@ struct GemmArgs {
 // flight while the MFMAs of the current one run: stage_load takes the thread's two 16-byte
 // pieces into registers, stage_store writes them to S[row][k] (the M- or N-contiguous case
 // transposing through 8 two-byte stores).
-constexpr int STAGE_PIECES = 2;   // (ROWS * BK / 8) / 256 for ROWS = 128, BK = 32
-struct StageRegs {
-    uint4 v[STAGE_PIECES];
+template <int P>
+struct StageRegsN {
+    uint4 v[P];
 };
-template <bool KC, int ROWS>
-__device__ __forceinline__ StageRegs stage_load(const bf16_t* G, int ld, int r0, int nrows, int k0, int kend, int tid) {
-    static_assert(ROWS * BK / 8 == STAGE_PIECES * 256, "two pieces per thread");
-    StageRegs R;
+template <bool KC, int ROWS, int NTH = 256, int BKX = BK, int P = ROWS * BKX / 8 / NTH>
+__device__ __forceinline__ StageRegsN<P> stage_load(const bf16_t* G, int ld, int r0, int nrows, int k0, int kend,
+                                                    int tid) {
+    static_assert(ROWS * BKX / 8 == P * NTH, "whole pieces per thread");
+    StageRegsN<P> R;
 #pragma unroll
-    for (int p = 0; p < STAGE_PIECES; p++) {
-        const int c = tid + 256 * p;
+    for (int p = 0; p < P; p++) {
+        const int c = tid + NTH * p;
         uint4 v = make_uint4(0, 0, 0, 0);
         if (KC) {   // G[r * ld + k]; 8-element pieces along k
-            const int r = c / (BK / 8), kc = (c % (BK / 8)) * 8;
+            const int r = c / (BKX / 8), kc = (c % (BKX / 8)) * 8;
             const int gr = r0 + r, gk = k0 + kc;
             if (gr < nrows) {
                 const bf16_t* src = G + (size_t)gr * ld + gk;
@@ -102,7 +103,7 @@ __device__ __forceinline__ StageRegs stage_load(const bf16_t* G, int ld, int r0,
                 }
             }
         } else {    // G[k * ld + r]; 8-element pieces along r, k fastest over the lanes
-            const int k = c % BK, rc = (c / BK) * 8;
+            const int k = c % BKX, rc = (c / BKX) * 8;
             const int gk = k0 + k, gr = r0 + rc;
             if (gk < kend) {
                 const bf16_t* src = G + (size_t)gk * ld + gr;
@@ -119,18 +120,18 @@ __device__ __forceinline__ StageRegs stage_load(const bf16_t* G, int ld, int r0,
     }
     return R;
 }
-template <bool KC, int ROWS>
-__device__ __forceinline__ void stage_store(bf16_t (*S)[LDK], const StageRegs& R, int tid) {
+template <bool KC, int ROWS, int NTH = 256, int BKX = BK, int LD = LDK, int P = ROWS * BKX / 8 / NTH>
+__device__ __forceinline__ void stage_store(bf16_t (*S)[LD], const StageRegsN<P>& R, int tid) {
 #pragma unroll
-    for (int p = 0; p < STAGE_PIECES; p++) {
-        const int c = tid + 256 * p;
+    for (int p = 0; p < P; p++) {
+        const int c = tid + NTH * p;
         if (KC) {
-            const int r = c / (BK / 8), kc = (c % (BK / 8)) * 8;
+            const int r = c / (BKX / 8), kc = (c % (BKX / 8)) * 8;
             *reinterpret_cast<uint4*>(&S[r][kc]) = R.v[p];
         } else {
             // k fastest over the lanes: each two-byte store of a wavefront covers 32 consecutive k
             // of two rows (with r fastest, the rows 8 apart fell in one LDS bank)
-            const int k = c % BK, rc = (c / BK) * 8;
+            const int k = c % BKX, rc = (c / BKX) * 8;
             bf16_t tmp[8];
             memcpy(tmp, &R.v[p], 16);
             for (int e = 0; e < 8; e++) S[rc + e][k] = tmp[e];
@@ -152,8 +153,8 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs a) {
     for (int i = 0; i < 4; i++)
 #pragma unroll
         for (int j = 0; j < 4; j++) acc[i][j] = (f32x4_t){0.f, 0.f, 0.f, 0.f};
-    StageRegs ra = stage_load<AKC, BM>(a.A, a.lda, m0, a.M, kbeg, kend, tid);
-    StageRegs rb = stage_load<BKC, BN>(a.B, a.ldb, n0, a.N, kbeg, kend, tid);
+    auto ra = stage_load<AKC, BM>(a.A, a.lda, m0, a.M, kbeg, kend, tid);
+    auto rb = stage_load<BKC, BN>(a.B, a.ldb, n0, a.N, kbeg, kend, tid);
     for (int k0 = kbeg; k0 < kend; k0 += BK) {
         stage_store<AKC, BM>(As, ra, tid);
         stage_store<BKC, BN>(Bs, rb, tid);
@@ -197,6 +198,61 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs a) {
                 } else {   // EPI_DGELU: gradient wrt the pre-activation of the previous layer
                     a.Cb[o] = f2bf(v * dgelu(bf2f(a.Z[o])));
                 }
+            }
+}
+
+// The weight-gradient GEMM of a 256 x 256 layer, C = A B with A and B stored k-major (the rows
+// of dZ and of the activations): one block computes all 256 x 256 outputs of its K chunk, so each
+// operand is read once (two 128 x 128 tiles read each twice). Eight waves in 4 x 2, each 64 x 128
+// = 4 x 8 MFMA tiles; the next K step is loaded during the current step's MFMAs, as in k_gemm
+// (steps of 64 needed 11 more registers than two waves per SIMD allow and were slower). Split-K
+// partials out, as k_gemm's EPI_F32.
+constexpr int WB = 256, WBK = BK, WLD = WBK + 8;
+__global__ void __launch_bounds__(512) k_gemm_wgrad(GemmArgs a) {
+    __shared__ bf16_t As[WB][WLD];
+    __shared__ bf16_t Bs[WB][WLD];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int kbeg = blockIdx.z * a.kchunk, kend = min(a.K, kbeg + a.kchunk);
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 128;
+    const int fr = lane & 15, fk = (lane >> 4) * 8;
+    f32x4_t acc[4][8];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 8; j++) acc[i][j] = (f32x4_t){0.f, 0.f, 0.f, 0.f};
+    auto ra = stage_load<false, WB, 512, WBK>(a.A, a.lda, 0, a.M, kbeg, kend, tid);
+    auto rb = stage_load<false, WB, 512, WBK>(a.B, a.ldb, 0, a.N, kbeg, kend, tid);
+    for (int k0 = kbeg; k0 < kend; k0 += WBK) {
+        stage_store<false, WB, 512, WBK, WLD>(As, ra, tid);
+        stage_store<false, WB, 512, WBK, WLD>(Bs, rb, tid);
+        __syncthreads();
+        if (k0 + WBK < kend) {   // the next K step's loads, in flight during this step's MFMAs
+            ra = stage_load<false, WB, 512, WBK>(a.A, a.lda, 0, a.M, k0 + WBK, kend, tid);
+            rb = stage_load<false, WB, 512, WBK>(a.B, a.ldb, 0, a.N, k0 + WBK, kend, tid);
+        }
+#pragma unroll
+        for (int kk = 0; kk < WBK; kk += 32) {
+            bf16x8_t af[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) af[i] = *reinterpret_cast<const bf16x8_t*>(&As[wm + 16 * i + fr][kk + fk]);
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                const bf16x8_t b = *reinterpret_cast<const bf16x8_t*>(&Bs[wn + 16 * j + fr][kk + fk]);
+#pragma unroll
+                for (int i = 0; i < 4; i++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], b, acc[i][j], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+    float* C = a.Cf + (size_t)blockIdx.z * a.M * a.ldc;
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int row = wm + 16 * i + 4 * (lane >> 4) + r, col = wn + 16 * j + (lane & 15);
+                if (row < a.M && col < a.N) C[(size_t)row * a.ldc + col] = acc[i][j][r];
             }
 }
 
@@ -1278,9 +1334,18 @@ int eu3d_mlp_train_step(eu3d_mlp* m, int n, const void* feats_bf16, const float*
         g.A = dz; g.lda = m->dout[l];          // element (m = o, k = b) at dz[b * dout + o]
         g.B = m->act[l]; g.ldb = m->din[l];    // element (n = i, k = b) at act[b * din + i]
         g.Cf = m->part; g.ldc = m->din[l];
-        if (gemm<false, false, EPI_F32>(s, g, sp) != hipSuccess) return EU3D_ERR_HIP;
+        const bool wide = g.M == WB && g.N == WB;
+        // K chunk of the split (a multiple of the kernel's K step), and the number of partials
+        const int kstep = wide ? WBK : BK;
+        const int kch = ((n + sp - 1) / sp + kstep - 1) / kstep * kstep, nsp = (n + kch - 1) / kch;
+        if (wide) {   // the 256 x 256 layers: one block per K chunk
+            g.kchunk = kch;
+            k_gemm_wgrad<<<dim3(1, 1, (n + g.kchunk - 1) / g.kchunk), 512, 0, s>>>(g);
+            if (hipGetLastError() != hipSuccess) return EU3D_ERR_HIP;
+        } else if (gemm<false, false, EPI_F32>(s, g, sp) != hipSuccess) {
+            return EU3D_ERR_HIP;
+        }
         const int nw = m->dout[l] * m->din[l];
-        const int kch = ((n + sp - 1) / sp + BK - 1) / BK * BK, nsp = (n + kch - 1) / kch;
         if (nsp > kgrp) {   // two levels: many split-K partials
             float* grp = m->part + (size_t)sp * NH * NH + (size_t)nb * NH;
             const int ngr = (nsp + kgrp - 1) / kgrp;

@@ -71,15 +71,16 @@ def test_forward_matches_fp64():
     assert np.abs(Y[:, 9:]).max() == 0.0   # padding outputs stay zero
 
 
-def test_gradients_match_fp64():
+@pytest.mark.parametrize("n", [4096, 5000])
+def test_gradients_match_fp64(n):
+    """Every layer's gradient; 5000 rows split the weight-gradient sums into ragged K chunks."""
     torch = _t()
     from eu3d_amd.api import MLP
     from oracle import mlp as om
-    say("case mlp grads")
+    say("case mlp grads", n)
     m = MLP(seed=4)
     p = _random_params(m, 5)
     rng = np.random.default_rng(6)
-    n = 4096
     X = np.zeros((n, 32))
     X[:, :12] = rng.normal(size=(n, 12))
     T = np.zeros((n, 16))
