@@ -1014,14 +1014,22 @@ struct GridCells {
         c.T = FLD(F, EU3D_FIELD_T)[cc];
         n = (int)FLD(F, EU3D_FIELD_NCHEM)[cc];
     }
-    __device__ void store(unsigned id, const double (&cs)[NC], const Prim& o, const KinOut& k) const {
+    // the chemical source, stored as soon as the last sub-step's rates are known so that they are
+    // not held through its energy Newton
+    __device__ void store_src(unsigned id, const KinOut& k) const {
+#pragma unroll
+        for (int s = 0; s < NS; s++) FLD(F, EU3D_FIELD_CMS)[s * g.n3 + (size_t)id] = k.w[s] * 1e3;
+    }
+    __device__ void store_state(unsigned id, const double (&cs)[NC], const Prim& o) const {
         const size_t cc = id;
         double* CS = FLD(F, EU3D_FIELD_CS);
 #pragma unroll
         for (int s = 0; s < NC; s++) CS[s * g.n3 + cc] = cs[s];
         store_prims(F, g, cc, cs, o);
-#pragma unroll
-        for (int s = 0; s < NS; s++) FLD(F, EU3D_FIELD_CMS)[s * g.n3 + cc] = k.w[s] * 1e3;
+    }
+    __device__ void store(unsigned id, const double (&cs)[NC], const Prim& o, const KinOut& k) const {
+        store_state(id, cs, o);
+        store_src(id, k);
     }
 };
 
@@ -1043,7 +1051,15 @@ struct GridCellsStep {
         c.T = FLD(F, EU3D_FIELD_T)[cc];
         n = (int)FLD(F, EU3D_FIELD_NCHEM)[cc];
     }
+    __device__ void store_src(unsigned id, const KinOut& k) const {
+#pragma unroll
+        for (int s = 0; s < NS; s++) FLD(F, EU3D_FIELD_CMS)[s * g.n3 + (size_t)id] = k.w[s] * 1e3;
+    }
     __device__ void store(unsigned id, const double (&cs)[NC], const Prim& o, const KinOut& k) const {
+        store_state(id, cs, o);
+        store_src(id, k);
+    }
+    __device__ void store_state(unsigned id, const double (&cs)[NC], const Prim& o) const {
         const size_t cc = id;
         double* CS = FLD(F, EU3D_FIELD_CS);
 #pragma unroll
@@ -1059,10 +1075,7 @@ struct GridCellsStep {
         FLD(F, EU3D_FIELD_GAMMA)[cc] = o.Gamma;
         FLD(F, EU3D_FIELD_C)[cc] = o.C;
 #pragma unroll
-        for (int s = 0; s < NS; s++) {
-            FLD(F, EU3D_FIELD_YI)[s * g.n3 + cc] = o.Y[s];
-            FLD(F, EU3D_FIELD_CMS)[s * g.n3 + cc] = k.w[s] * 1e3;
-        }
+        for (int s = 0; s < NS; s++) FLD(F, EU3D_FIELD_YI)[s * g.n3 + cc] = o.Y[s];
     }
 };
 
@@ -1079,11 +1092,15 @@ struct BatchCells {
         c.T = T[id];
         nn = nchem[id];
     }
-    __device__ void store(unsigned id, const double (&cs)[NC], const Prim& o, const KinOut&) const {
+    __device__ void store_src(unsigned, const KinOut&) const {}
+    __device__ void store_state(unsigned id, const double (&cs)[NC], const Prim& o) const {
 #pragma unroll
         for (int s = 0; s < NS; s++) rho[(size_t)s * n + id] = cs[s];
         T[id] = o.T;
         E[id] = o.E;
+    }
+    __device__ void store(unsigned id, const double (&cs)[NC], const Prim& o, const KinOut& k) const {
+        store_state(id, cs, o);
     }
 };
 
@@ -1215,15 +1232,12 @@ k_trap_first(G3 g, Fields F, uint32_t q, const Scalars* __restrict__ scc, Scalar
         if (nch <= 0) atomicOr(&sc->nonfinite, 2);
         to_net = nthr > 0 && nch > nthr && cell.T >= T_cold;
         if (!predict_only && nch > 0 && !to_net) {
+            using Acc = typename std::conditional<ADV, GridCellsStep, GridCells>::type;
+            const Acc acc{g, F};
+            acc.store_src((unsigned)c, ko);
             double cs[NC];
             const Prim o = chem_apply(cell, ko, Dr, dtm / nch, q, cs, tp);
-            if (ADV) {
-                GridCellsStep acc{g, F};
-                acc.store((unsigned)c, cs, o, ko);
-            } else {
-                GridCells acc{g, F};
-                acc.store((unsigned)c, cs, o, ko);
-            }
+            acc.store_state((unsigned)c, cs, o);
             active = nch > 1;
         }
     }
@@ -1310,9 +1324,11 @@ k_chem_persist(Acc acc, const unsigned* __restrict__ list, Scalars* sc, uint32_t
             KinOut ko;
             const TPow tp = tpow(cell.T);
             kinetics_p<true, false, C3>(tp, Mc, Dr, Dr, (q & Q_C2) ? 1 : 2, q & Q_C4, ko);
+            const bool last = --rem == 0;
+            if (last) acc.store_src(id, ko);
             const Prim o = chem_apply(cell, ko, Dr, dtm / nch, q, cs, tp);
-            if (--rem == 0) {
-                acc.store(id, cs, o, ko);
+            if (last) {
+                acc.store_state(id, cs, o);
                 have = false;
             }
         }
