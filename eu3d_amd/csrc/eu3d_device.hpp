@@ -288,9 +288,15 @@ struct Prim {
 // mixture fit per range on demand (LazyMix): it pays where a wavefront's cells are neighbours in
 // space and mostly on one side of 1000 K (the grid step), not where they are unrelated cells of
 // a work queue (there both ranges are usually needed and the branches cost more than they save).
-template <bool LAZY = false>
+// pre(o) runs before the temperature Newton with the fields that do not depend on T (D, U, V, W,
+// E, Y) set, so that a caller can store them then instead of holding them through the Newton.
+struct NoPre {
+    __device__ void operator()(const Prim&) const {}
+};
+
+template <bool LAZY = false, class Pre = NoPre>
 __device__ __forceinline__ Prim prim_from_cs_p(const double (&cs)[NC], const TPow& p_old, double E_old, uint32_t q,
-                                                int* iters = nullptr) {
+                                                int* iters = nullptr, const Pre& pre = Pre()) {
     Prim o;
     double D = 0;
 #pragma unroll
@@ -303,6 +309,7 @@ __device__ __forceinline__ Prim prim_from_cs_p(const double (&cs)[NC], const TPo
     o.V = cs[NS + 1] * rD;
     o.W = cs[NS + 2] * rD;
     o.E = (cs[NS + 3] - 0.5 * D * (o.U * o.U + o.V * o.V + o.W * o.W)) * rD;
+    pre(o);
     double cp;
     if constexpr (LAZY) {
         LazyMix L;

@@ -29,14 +29,25 @@ __device__ __forceinline__ void load_cs(const Fields& F, const G3& g, size_t c, 
 }
 
 // Write everything Update_after_CS leaves behind for one cell.
-__device__ __forceinline__ void store_prims(const Fields& F, const G3& g, size_t c, const double (&cs)[NC],
-                                            const Prim& o) {
+// The primitives that do not depend on T (D, U, V, W, E, Y) and the densities derived from CS.
+__device__ __forceinline__ void store_prims_pre(const Fields& F, const G3& g, size_t c, const double (&cs)[NC],
+                                                const Prim& o) {
     FLD(F, EU3D_FIELD_D)[c] = o.D;
-    FLD(F, EU3D_FIELD_T)[c] = o.T;
     FLD(F, EU3D_FIELD_U)[c] = o.U;
     FLD(F, EU3D_FIELD_V)[c] = o.V;
     FLD(F, EU3D_FIELD_W)[c] = o.W;
     FLD(F, EU3D_FIELD_E)[c] = o.E;
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        FLD(F, EU3D_FIELD_DI)[s * g.n3 + c] = cs[s];
+        FLD(F, EU3D_FIELD_MC)[s * g.n3 + c] = cs[s] * RMW(s) * 1000;
+        FLD(F, EU3D_FIELD_YI)[s * g.n3 + c] = o.Y[s];
+    }
+}
+
+// The rest of the primitives.
+__device__ __forceinline__ void store_prims_post(const Fields& F, size_t c, const Prim& o) {
+    FLD(F, EU3D_FIELD_T)[c] = o.T;
     FLD(F, EU3D_FIELD_WAV)[c] = o.Wav;
     FLD(F, EU3D_FIELD_RGAS)[c] = o.Rgas;
     FLD(F, EU3D_FIELD_P)[c] = o.P;
@@ -45,12 +56,12 @@ __device__ __forceinline__ void store_prims(const Fields& F, const G3& g, size_t
     FLD(F, EU3D_FIELD_GAMMA)[c] = o.Gamma;
     FLD(F, EU3D_FIELD_C)[c] = o.C;
     FLD(F, EU3D_FIELD_MA)[c] = sqrt(o.U * o.U + o.V * o.V + o.W * o.W) / o.C;
-#pragma unroll
-    for (int s = 0; s < NS; s++) {
-        FLD(F, EU3D_FIELD_DI)[s * g.n3 + c] = cs[s];
-        FLD(F, EU3D_FIELD_MC)[s * g.n3 + c] = cs[s] * RMW(s) * 1000;
-        FLD(F, EU3D_FIELD_YI)[s * g.n3 + c] = o.Y[s];
-    }
+}
+
+__device__ __forceinline__ void store_prims(const Fields& F, const G3& g, size_t c, const double (&cs)[NC],
+                                            const Prim& o) {
+    store_prims_pre(F, g, c, cs, o);
+    store_prims_post(F, c, o);
 }
 
 // ---------------------------------------------------------------- layout conversion
@@ -957,8 +968,10 @@ __device__ __forceinline__ void chem_inputs(const ChemCell& c, double (&Mc)[NS],
 // Sub-step tail: the trapezoid update from the kinetics k, then Flowfield::Explicit(cell) and
 // Update_after_CS(cell) (src/Flowfield.cpp:1189-1203,1287-1319). Leaves the conserved state in cs.
 // tp: powers of c.T (the sub-step's starting temperature, shared with the kinetics).
+// pre: passed to prim_from_cs_p (runs before the temperature Newton).
+template <class Pre = NoPre>
 __device__ __forceinline__ Prim chem_apply(ChemCell& c, const KinOut& k, const double (&Dr)[NS], double dt,
-                                           uint32_t q, double (&cs)[NC], const TPow& tp) {
+                                           uint32_t q, double (&cs)[NC], const TPow& tp, const Pre& pre = Pre()) {
     double Dn[NS];
     trap_update(k, Dr, dt, Dn);
     double D = 0.0;
@@ -971,7 +984,7 @@ __device__ __forceinline__ Prim chem_apply(ChemCell& c, const KinOut& k, const d
     cs[NS + 1] = c.V * D;
     cs[NS + 2] = c.W * D;
     cs[NS + 3] = D * c.E + 0.5 * (c.U * c.U + c.V * c.V + c.W * c.W) * D;
-    const Prim o = prim_from_cs_p(cs, tp, c.E, q);
+    const Prim o = prim_from_cs_p<false>(cs, tp, c.E, q, nullptr, pre);
 #pragma unroll
     for (int s = 0; s < NS; s++) c.Di[s] = cs[s];
     c.U = o.U;
@@ -1020,12 +1033,18 @@ struct GridCells {
 #pragma unroll
         for (int s = 0; s < NS; s++) FLD(F, EU3D_FIELD_CMS)[s * g.n3 + (size_t)id] = k.w[s] * 1e3;
     }
-    __device__ void store_state(unsigned id, const double (&cs)[NC], const Prim& o) const {
+    // store_pre: what does not depend on T (from the hook before the Newton); store_post: the rest
+    __device__ void store_pre(unsigned id, const double (&cs)[NC], const Prim& o) const {
         const size_t cc = id;
         double* CS = FLD(F, EU3D_FIELD_CS);
 #pragma unroll
         for (int s = 0; s < NC; s++) CS[s * g.n3 + cc] = cs[s];
-        store_prims(F, g, cc, cs, o);
+        store_prims_pre(F, g, cc, cs, o);
+    }
+    __device__ void store_post(unsigned id, const Prim& o) const { store_prims_post(F, id, o); }
+    __device__ void store_state(unsigned id, const double (&cs)[NC], const Prim& o) const {
+        store_pre(id, cs, o);
+        store_post(id, o);
     }
     __device__ void store(unsigned id, const double (&cs)[NC], const Prim& o, const KinOut& k) const {
         store_state(id, cs, o);
@@ -1059,23 +1078,30 @@ struct GridCellsStep {
         store_state(id, cs, o);
         store_src(id, k);
     }
-    __device__ void store_state(unsigned id, const double (&cs)[NC], const Prim& o) const {
+    __device__ void store_pre(unsigned id, const double (&cs)[NC], const Prim& o) const {
         const size_t cc = id;
         double* CS = FLD(F, EU3D_FIELD_CS);
 #pragma unroll
         for (int s = 0; s < NC; s++) CS[s * g.n3 + cc] = cs[s];
         FLD(F, EU3D_FIELD_D)[cc] = o.D;
-        FLD(F, EU3D_FIELD_T)[cc] = o.T;
         FLD(F, EU3D_FIELD_U)[cc] = o.U;
         FLD(F, EU3D_FIELD_V)[cc] = o.V;
         FLD(F, EU3D_FIELD_W)[cc] = o.W;
         FLD(F, EU3D_FIELD_E)[cc] = o.E;
+#pragma unroll
+        for (int s = 0; s < NS; s++) FLD(F, EU3D_FIELD_YI)[s * g.n3 + cc] = o.Y[s];
+    }
+    __device__ void store_post(unsigned id, const Prim& o) const {
+        const size_t cc = id;
+        FLD(F, EU3D_FIELD_T)[cc] = o.T;
         FLD(F, EU3D_FIELD_P)[cc] = o.P;
         FLD(F, EU3D_FIELD_H)[cc] = o.H;
         FLD(F, EU3D_FIELD_GAMMA)[cc] = o.Gamma;
         FLD(F, EU3D_FIELD_C)[cc] = o.C;
-#pragma unroll
-        for (int s = 0; s < NS; s++) FLD(F, EU3D_FIELD_YI)[s * g.n3 + cc] = o.Y[s];
+    }
+    __device__ void store_state(unsigned id, const double (&cs)[NC], const Prim& o) const {
+        store_pre(id, cs, o);
+        store_post(id, o);
     }
 };
 
@@ -1093,11 +1119,15 @@ struct BatchCells {
         nn = nchem[id];
     }
     __device__ void store_src(unsigned, const KinOut&) const {}
-    __device__ void store_state(unsigned id, const double (&cs)[NC], const Prim& o) const {
+    __device__ void store_pre(unsigned id, const double (&cs)[NC], const Prim& o) const {
 #pragma unroll
         for (int s = 0; s < NS; s++) rho[(size_t)s * n + id] = cs[s];
-        T[id] = o.T;
         E[id] = o.E;
+    }
+    __device__ void store_post(unsigned id, const Prim& o) const { T[id] = o.T; }
+    __device__ void store_state(unsigned id, const double (&cs)[NC], const Prim& o) const {
+        store_pre(id, cs, o);
+        store_post(id, o);
     }
     __device__ void store(unsigned id, const double (&cs)[NC], const Prim& o, const KinOut& k) const {
         store_state(id, cs, o);
@@ -1236,8 +1266,9 @@ k_trap_first(G3 g, Fields F, uint32_t q, const Scalars* __restrict__ scc, Scalar
             const Acc acc{g, F};
             acc.store_src((unsigned)c, ko);
             double cs[NC];
-            const Prim o = chem_apply(cell, ko, Dr, dtm / nch, q, cs, tp);
-            acc.store_state((unsigned)c, cs, o);
+            const Prim o = chem_apply(cell, ko, Dr, dtm / nch, q, cs, tp,
+                                      [&](const Prim& o1) { acc.store_pre((unsigned)c, cs, o1); });
+            acc.store_post((unsigned)c, o);
             active = nch > 1;
         }
     }
@@ -1326,9 +1357,11 @@ k_chem_persist(Acc acc, const unsigned* __restrict__ list, Scalars* sc, uint32_t
             kinetics_p<true, false, C3>(tp, Mc, Dr, Dr, (q & Q_C2) ? 1 : 2, q & Q_C4, ko);
             const bool last = --rem == 0;
             if (last) acc.store_src(id, ko);
-            const Prim o = chem_apply(cell, ko, Dr, dtm / nch, q, cs, tp);
+            const Prim o = chem_apply(cell, ko, Dr, dtm / nch, q, cs, tp, [&](const Prim& o1) {
+                if (last) acc.store_pre(id, cs, o1);
+            });
             if (last) {
-                acc.store_state(id, cs, o);
+                acc.store_post(id, o);
                 have = false;
             }
         }
