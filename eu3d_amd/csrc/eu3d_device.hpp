@@ -290,13 +290,15 @@ struct Prim {
 // a work queue (there both ranges are usually needed and the branches cost more than they save).
 // pre(o) runs before the temperature Newton with the fields that do not depend on T (D, U, V, W,
 // E, Y) set, so that a caller can store them then instead of holding them through the Newton.
+// p_final, if given, receives the powers of the solved T (by series from the last iterate's).
 struct NoPre {
     __device__ void operator()(const Prim&) const {}
 };
 
 template <bool LAZY = false, class Pre = NoPre>
 __device__ __forceinline__ Prim prim_from_cs_p(const double (&cs)[NC], const TPow& p_old, double E_old, uint32_t q,
-                                                int* iters = nullptr, const Pre& pre = Pre()) {
+                                                int* iters = nullptr, const Pre& pre = Pre(),
+                                                TPow* p_final = nullptr) {
     Prim o;
     double D = 0;
 #pragma unroll
@@ -319,12 +321,16 @@ __device__ __forceinline__ Prim prim_from_cs_p(const double (&cs)[NC], const TPo
         TPow pl;
         o.T = newton_T_lazy(L, o.Y, q & Q_C4, p_old, (q & Q_F3) ? E_old : o.E, iters, &pl);
         L.need(o.Y, q & Q_C4, o.T < 1000);
-        cp = mix_cp_p(L.m, tpow_near(pl, o.T));
+        const TPow pf = tpow_near(pl, o.T);
+        cp = mix_cp_p(L.m, pf);
+        if (p_final) *p_final = pf;
     } else {
         const MixFit m = mix_fit(o.Y, q & Q_C4);
         TPow pl;
         o.T = newton_T_from(m, p_old, (q & Q_F3) ? E_old : o.E, iters, &pl);
-        cp = mix_cp_p(m, tpow_near(pl, o.T));
+        const TPow pf = tpow_near(pl, o.T);
+        cp = mix_cp_p(m, pf);
+        if (p_final) *p_final = pf;
     }
     double sy = 0;
 #pragma unroll

@@ -1008,6 +1008,9 @@ __device__ __forceinline__ int chem_nchem(const KinOut& k, const double (&Dr)[NS
         }
     }
     double nd = ceil(dtm / dt);
+    // dtm = 0 gives 0/0: a count that is NaN or below 1 is the reference's Nchem <= 0, returned
+    // as 0 rather than through an undefined NaN-to-int conversion
+    if (!(nd >= 1.0)) return 0;
     if (nd > (double)nmax) nd = (double)nmax;
     return (int)nd;
 }
@@ -1206,6 +1209,7 @@ k_trap_first(G3 g, Fields F, uint32_t q, const Scalars* __restrict__ scc, Scalar
     if (in) {
         ChemCell cell;
         double Mc[NS];
+        TPow tp;   // powers of cell.T: with ADV from the Newton that solved it
         if (ADV) {
             double cs0[NC];
             load_cs(F, g, c, cs0);
@@ -1214,7 +1218,8 @@ k_trap_first(G3 g, Fields F, uint32_t q, const Scalars* __restrict__ scc, Scalar
 #pragma unroll
             for (int s = 0; s < NC; s++)
                 if (s < nadd) cs0[s] = cs0[s] + R[s * g.n3 + c];
-            const Prim o0 = prim_from_cs(cs0, FLD(F, EU3D_FIELD_T)[c], FLD(F, EU3D_FIELD_E)[c], q);
+            const Prim o0 = prim_from_cs_p(cs0, tpow(FLD(F, EU3D_FIELD_T)[c]), FLD(F, EU3D_FIELD_E)[c], q,
+                                           nullptr, NoPre(), &tp);
 #pragma unroll
             for (int s = 0; s < NS; s++) {
                 cell.Di[s] = cs0[s];
@@ -1238,7 +1243,7 @@ k_trap_first(G3 g, Fields F, uint32_t q, const Scalars* __restrict__ scc, Scalar
             cell.T = FLD(F, EU3D_FIELD_T)[c];
         }
         KinOut ko;
-        const TPow tp = tpow(cell.T);
+        if (!ADV) tp = tpow(cell.T);
         // the trapezoid's rates do not read Dr; it and Y are formed after the kinetics so that they
         // are not held through them (Y as Update_after_CS forms it: Di / sum Di)
         kinetics_p<true, false, C3>(tp, Mc, Mc, Mc, (q & Q_C2) ? 1 : 2, q & Q_C4, ko);
@@ -1291,13 +1296,19 @@ __device__ __forceinline__ void adv_update_cell(const G3& g, const Fields& F, ui
     store_prims(F, g, c, cs, o);
 }
 
-// After the fused prediction pass: the cells whose prediction failed (Nchem <= 0).
-__global__ void __launch_bounds__(256) k_adv_nochem(G3 g, Fields F, uint32_t q) {
-    int i, j, k;
-    if (!interior_ijk(g, i, j, k)) return;
-    const size_t c = IDX(g, i, j, k);
-    if (FLD(F, EU3D_FIELD_NCHEM)[c] > 0) return;
-    adv_update_cell(g, F, q, c);
+// After the fused prediction pass: the cells whose prediction failed (Nchem <= 0). The pass sets
+// bit 2 of sc->nonfinite for any such cell; without it there is nothing to do, so the kernel is
+// launched on a small grid that scans the interior only when the bit is set.
+__global__ void __launch_bounds__(256) k_adv_nochem(G3 g, Fields F, uint32_t q, const Scalars* __restrict__ sc) {
+    if (!(sc->nonfinite & 2)) return;
+    const size_t n = (size_t)g.ni * g.nj * g.nk;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
+        const size_t r = t / g.ni;
+        const int i = (int)(t - r * g.ni) + g.bc, j = (int)(r % g.nj) + g.bc, k = (int)(r / g.nj) + g.bc;
+        const size_t c = IDX(g, i, j, k);
+        if (FLD(F, EU3D_FIELD_NCHEM)[c] > 0) continue;
+        adv_update_cell(g, F, q, c);
+    }
 }
 
 // After the fused prediction pass with surrogate routing: the cells listed for the network
@@ -2078,7 +2089,7 @@ hipError_t launch_trap_adaptive(eu3d_ctx* c, int nchem_max, bool predict_only, i
     else
         k_trap_first<false, false><<<gr, bl, 0, c->stream>>>(G(c), c->fl, c->quirks, c->sc, c->sc, c->active, nmax, po,
                                                              nthr, T_cold, slist, scount, rt);
-    if (adv) k_adv_nochem<<<gr, bl, 0, c->stream>>>(G(c), c->fl, c->quirks);
+    if (adv) k_adv_nochem<<<4 * c->num_cu, 256, 0, c->stream>>>(G(c), c->fl, c->quirks, c->sc);
     if (!predict_only && adv) {   // the step: the derived fields are refreshed after its last step
         GridCellsStep acc{c->g, c->fl};
         launch_persist(c, acc, c->active, slot);

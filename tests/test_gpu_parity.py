@@ -227,3 +227,33 @@ def test_steps_zn1d_2000(chem):
     o.step(**kw)
     assert g.check_finite() == 0
     compare(g, o, ["CS", "T", "D", "P", "E", "YI"])
+
+
+def test_step_failed_prediction_stores_advected_state():
+    """A step clamped to a final time already reached has dt = 0, so every cell's sub-step count
+    is ceil(0/0): the reference's Nchem <= 0 abort (src/Reaction.cpp:664-668). The fused
+    prediction pass leaves such cells to k_adv_nochem, which must store Update_after_Adv and
+    Update_after_CS for every interior cell (src/Flowfield.cpp:1140-1157,1246-1284); the error
+    surfaces on the chemistry statistics. With skip_derive nothing else writes Ma, so a poisoned
+    Ma shows which cells were stored."""
+    say("case failed prediction, dt = 0")
+    from eu3d_amd.api import Context, Eu3dError
+    from tests.common import init_state
+    wl = small_det()
+    g = Context(wl.xnode, wl.ynode, wl.znode, quirks=Q_FIXED)
+    init_state(g, wl)
+    ma = g.get("MA")
+    ma[2:-2, 2:-2, 2:-2] = -7.0
+    g.set("MA", ma)
+    cs0 = g.get("CS")
+    g.step(1, cfl=0.05, final_time=0.0, bc_kinds=wl.bc_kinds, rk=1, chem=3, nchem_max=200, skip_derive=True)
+    with pytest.raises(Eu3dError, match="Nchem"):
+        g.chem_stats()
+    inner = (slice(2, -2),) * 3
+    ma = g.get("MA")[inner]
+    assert (ma != -7.0).all()
+    u, v, w, c = (g.get(f)[inner] for f in ("U", "V", "W", "C"))
+    np.testing.assert_allclose(ma, np.sqrt(u * u + v * v + w * w) / c, rtol=1e-13, atol=0)
+    # RHS = dt L = 0, so the conserved state is the one before the step
+    assert np.array_equal(g.get("CS")[inner], cs0[inner])
+    assert (g.get("NCHEM")[inner] == 0).all()
