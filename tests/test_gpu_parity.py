@@ -257,3 +257,26 @@ def test_step_failed_prediction_stores_advected_state():
     # RHS = dt L = 0, so the conserved state is the one before the step
     assert np.array_equal(g.get("CS")[inner], cs0[inner])
     assert (g.get("NCHEM")[inner] == 0).all()
+
+
+@pytest.mark.parametrize("field,value,ijk,flagged", [
+    ("D", np.nan, (5, 7, 2), True),      # first interior plane
+    ("T", np.inf, (25, 13, 11), True),   # last interior plane: the partial group of z planes
+    ("T", np.nan, (2, 2, 7), True),
+    ("D", np.nan, (0, 7, 5), False),     # a ghost cell is not checked
+])
+def test_check_finite_flags_one_cell(field, value, ijk, flagged):
+    """eu3d_check_finite (src/Euler.cpp:319-331) flags one non-finite D or T anywhere in the
+    interior (24 x 12 x 10 cells, two ghost layers) and ignores the ghost cells."""
+    say("case check_finite", field, value, ijk)
+    from eu3d_amd.api import Context
+    from tests.common import init_state
+    wl = small_bubble()
+    g = Context(wl.xnode, wl.ynode, wl.znode, quirks=Q_FIXED)
+    init_state(g, wl)
+    assert g.check_finite() == 0
+    a = g.get(field)
+    assert a.shape == (28, 16, 14)
+    a[ijk] = value
+    g.set(field, a)
+    assert (g.check_finite() != 0) == flagged
