@@ -1132,7 +1132,7 @@ struct BatchCells {
         store_pre(id, cs, o);
         store_post(id, o);
     }
-    __device__ void store(unsigned id, const double (&cs)[NC], const Prim& o, const KinOut& k) const {
+    __device__ void store(unsigned id, const double (&cs)[NC], const Prim& o, const KinOut&) const {
         store_state(id, cs, o);
     }
 };
