@@ -15,7 +15,9 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle.so")
+# EU3D_ORACLE_LIB points the checker at another build of oracle.c (oracle/check_mutants.py uses it
+# to load deliberately broken copies and confirm the pins fail).
+LIB_PATH = os.environ.get("EU3D_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 
 Q_REF, Q_FIXED = 0x7FF, 0
 FIELD3 = ["U", "V", "W", "P", "D", "T", "C", "Ma", "Wav", "Rgas", "Cp", "H", "E", "Gamma"]
