@@ -32,7 +32,7 @@ FIELDS = {"CS": 0, "T": 1, "E": 2, "U": 3, "V": 4, "W": 5, "P": 6, "D": 7, "H": 
           "PT": 18, "RHS": 19, "F": 20, "G": 21, "Q": 22, "CMS": 23, "MD": 24, "NCHEM": 25}
 
 EXPORTED = [
-    "eu3d_create", "eu3d_destroy", "eu3d_last_error", "eu3d_sync", "eu3d_set_quirks",
+    "eu3d_create", "eu3d_grid_sizes", "eu3d_destroy", "eu3d_last_error", "eu3d_sync", "eu3d_set_quirks",
     "eu3d_field_components", "eu3d_set_field", "eu3d_get_field", "eu3d_set_time", "eu3d_get_time",
     "eu3d_field_initial", "eu3d_cfl_dt", "eu3d_fill_ghosts", "eu3d_advection", "eu3d_advection_fused",
     "eu3d_update_after_adv", "eu3d_update_after_cs", "eu3d_get_partial_t", "eu3d_explicit",
@@ -111,6 +111,8 @@ def load_library(path: str = LIB_PATH):
     vp, dp, ip = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)
     lib.eu3d_create.argtypes = [C.POINTER(vp), C.POINTER(_Grid), C.POINTER(_Mech), C.c_uint32]
     lib.eu3d_destroy.argtypes = [vp]
+    lib.eu3d_grid_sizes.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_longlong),
+                                    C.POINTER(C.c_ulonglong)]
     lib.eu3d_last_error.argtypes = [vp]
     lib.eu3d_last_error.restype = C.c_char_p
     for name in ["eu3d_sync", "eu3d_field_initial", "eu3d_update_after_adv", "eu3d_update_after_cs",
@@ -164,6 +166,16 @@ def load_library(path: str = LIB_PATH):
     lib.eu3d_get_surrogate_stats.argtypes = [vp, C.POINTER(SurrogateStats)]
     _lib = lib
     return lib
+
+
+def grid_sizes(ni: int, nj: int, nk: int, bc: int = 2):
+    """(CFL partial count, device bytes) of a context for this grid; host arithmetic, no GPU."""
+    lib = load_library()
+    parts, nbytes = C.c_longlong(), C.c_ulonglong()
+    rc = lib.eu3d_grid_sizes(ni, nj, nk, bc, C.byref(parts), C.byref(nbytes))
+    if rc != 0:
+        raise Eu3dError(rc, "eu3d_grid_sizes")
+    return parts.value, nbytes.value
 
 
 def _dptr(a: np.ndarray):

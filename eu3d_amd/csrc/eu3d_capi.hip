@@ -109,7 +109,7 @@ int eu3d_create(eu3d_ctx** out, const eu3d_grid* grid, const eu3d_mech* mech, ui
     if (hipMalloc(&c->xnode, nx * sizeof(double)) != hipSuccess || hipMalloc(&c->ynode, ny * sizeof(double)) != hipSuccess ||
         hipMalloc(&c->znode, nz * sizeof(double)) != hipSuccess || hipMalloc(&c->sc, sizeof(Scalars)) != hipSuccess ||
         hipMalloc(&c->active, g.n3 * sizeof(unsigned)) != hipSuccess ||
-        hipMalloc(&c->part, (g.n3 / 64 + 64) * sizeof(double)) != hipSuccess ||
+        hipMalloc(&c->part, (size_t)interior_blocks(g.ni, g.nj, g.nk) * sizeof(double)) != hipSuccess ||
         hipMalloc(&c->rk_save, g.n3 * NC * sizeof(double)) != hipSuccess) {
         eu3d_destroy(c);
         return EU3D_ERR_HIP;
@@ -121,6 +121,21 @@ int eu3d_create(eu3d_ctx** out, const eu3d_grid* grid, const eu3d_mech* mech, ui
     (void)launch_set_scalars(c, 0.0, 0.0);
     if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) { eu3d_destroy(c); return EU3D_ERR_HIP; }
     *out = c;
+    return EU3D_OK;
+}
+
+int eu3d_grid_sizes(int ni, int nj, int nk, int bc, long long* cfl_partials, unsigned long long* device_bytes) {
+    if (bc != 2 || ni < 1 || nj < 1 || nk < 1) return EU3D_ERR_ARG;
+    const unsigned long long NX = ni + 2ull * bc, NY = nj + 2ull * bc, NZ = nk + 2ull * bc, n3 = NX * NY * NZ;
+    const long long parts = interior_blocks(ni, nj, nk);
+    if (cfl_partials) *cfl_partials = parts;
+    if (device_bytes) {
+        unsigned long long b = 0;
+        for (int f = 0; f < EU3D_FIELD_COUNT; f++) b += n3 * ncomp(f) * sizeof(double);
+        b += (NX + NY + NZ) * sizeof(double) + sizeof(Scalars) + n3 * sizeof(unsigned) +
+             (unsigned long long)parts * sizeof(double) + n3 * NC * sizeof(double);
+        *device_bytes = b;
+    }
     return EU3D_OK;
 }
 
@@ -320,12 +335,22 @@ int eu3d_step(eu3d_ctx* c, const eu3d_step_params* p, int nsteps) {
     if (p->rk == 2 && p->chem == EU3D_CHEM_IMEX) return fail(c, EU3D_ERR_UNSUPPORTED, "IMEX couples to the EE stage only");
     if (p->diff < 0 || p->diff > 1 || p->limiter < 0 || p->limiter > 4) return fail(c, EU3D_ERR_ARG, "bad scheme");
     if (p->chem == EU3D_CHEM_SURROGATE && !c->mlp) return fail(c, EU3D_ERR_ARG, "no surrogate attached");
+    const bool adaptive = p->chem == EU3D_CHEM_TRAP_ADAPTIVE || p->chem == EU3D_CHEM_SURROGATE;
+    if (adaptive && nsteps > 0) HIPCHK(c, launch_clear_nchem_flag(c), "step flag");
     for (int n = 0; n < nsteps; n++) HIPCHK(c, launch_step(c, p, n == 0), "step");
     if (p->chem == EU3D_CHEM_TRAP_ADAPTIVE) c->stats.cells = (long long)c->g.ni * c->g.nj * c->g.nk;
     if (p->chem == EU3D_CHEM_SURROGATE && nsteps > 0) HIPCHK(c, surrogate_grid_stats(c), "surrogate stats");
     if (nsteps > 0) {
         if (!p->skip_derive) HIPCHK(c, launch_derive(c), "step derive");
         HIPCHK(c, launch_check_finite(c), "step finite");
+    }
+    if (adaptive && nsteps > 0) {
+        // The reference aborts on Nchem <= 0 (src/Reaction.cpp:664-668). The device keeps stepping
+        // (such cells store the advected state); the error is reported once, after the steps.
+        int flag = 0;
+        HIPCHK(c, hipMemcpyAsync(&flag, &c->sc->nonfinite, sizeof(int), hipMemcpyDeviceToHost, c->stream), "step flag");
+        HIPCHK(c, hipStreamSynchronize(c->stream), "step sync");
+        if (flag & 6) return fail(c, EU3D_ERR_NCHEM, "Nchem <= 0 in a step (src/Reaction.cpp:664-668)");
     }
     return EU3D_OK;
 }

@@ -20,6 +20,14 @@ struct Fields {
     double* f[EU3D_FIELD_COUNT];
 };
 
+// Interior kernels run 64x4 blocks over (i, j) and one grid layer per k. k_cfl and k_imex_fused
+// write one CFL partial minimum per block of that grid, so the partial buffer holds exactly
+// interior_blocks() doubles.
+constexpr int BX = 64, BY = 4;
+inline long long interior_blocks(int ni, int nj, int nk) {
+    return (long long)((ni + BX - 1) / BX) * ((nj + BY - 1) / BY) * nk;
+}
+
 #define IDX(g, i, j, k) ((((size_t)(k)) * (size_t)(g).NY + (size_t)(j)) * (size_t)(g).NX + (size_t)(i))
 #define FLD(F, id) ((F).f[id])
 
@@ -85,6 +93,7 @@ hipError_t launch_flux_z(eu3d_ctx* c, int diff, int limiter);
 hipError_t launch_flux_xy(eu3d_ctx* c, int diff, int limiter);
 hipError_t launch_imex_fused(eu3d_ctx* c);
 hipError_t launch_derive(eu3d_ctx* c);
+hipError_t launch_clear_nchem_flag(eu3d_ctx* c);
 hipError_t launch_update_after_adv(eu3d_ctx* c);
 hipError_t launch_explicit(eu3d_ctx* c);
 hipError_t launch_diagonalized(eu3d_ctx* c);

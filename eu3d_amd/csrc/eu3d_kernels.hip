@@ -11,8 +11,6 @@ using namespace eu3d;
 
 namespace {
 
-constexpr int BX = 64, BY = 4;
-
 inline dim3 interior_grid(const G3& g) { return dim3((g.ni + BX - 1) / BX, (g.nj + BY - 1) / BY, g.nk); }
 
 __device__ __forceinline__ bool interior_ijk(const G3& g, int& i, int& j, int& k) {
@@ -530,6 +528,8 @@ __global__ void k_set_scalars(Scalars* sc, double dt, double time) {
 }
 
 __global__ void k_advance_time(Scalars* sc) { sc->time += sc->dt; }
+
+__global__ void k_clear_nchem_flag(Scalars* sc) { sc->nonfinite &= ~6; }
 
 // ---------------------------------------------------------------- state updates
 // Flowfield::Update_after_Adv, src/Flowfield.cpp:1140-1157 (quirk F1: species only)
@@ -1818,7 +1818,11 @@ __global__ void __launch_bounds__(256) k_batch_first_list(BatchCells acc, int* n
     }
 }
 
+// Bit 2 of sc->nonfinite reports a cell with Nchem <= 0 in the current chemistry call (the passes
+// of that call read it); each call starts clear and keeps an earlier call's report in bit 4,
+// which eu3d_step reads back once after its steps.
 __global__ void k_chem_reset(Scalars* sc, double* dtm_slot, double dtm, int use_dt) {
+    if (sc->nonfinite & 2) sc->nonfinite = (sc->nonfinite & ~2) | 4;
     sc->coop_count = 0;
     sc->coop_head = 0;
     sc->chem_head = 0;
@@ -1920,10 +1924,7 @@ hipError_t launch_advection(eu3d_ctx* c, int diff, int limiter) {
     return hipGetLastError();
 }
 
-static int n_interior_blocks(const G3& g) {
-    const dim3 b = interior_grid(g);
-    return (int)(b.x * b.y * b.z);
-}
+static int n_interior_blocks(const G3& g) { return (int)interior_blocks(g.ni, g.nj, g.nk); }
 
 // The tail kernel after the persistent kernel: one wavefront per queued cell, grid-stride over the
 // device count.
@@ -2007,6 +2008,11 @@ hipError_t launch_flux_fused(eu3d_ctx* c, int diff, int limiter) {
 hipError_t launch_imex_fused(eu3d_ctx* c) {
     k_imex_fused<<<interior_grid(c->g), dim3(BX, BY), 0, c->stream>>>(G(c), c->fl, c->quirks, c->sc, c->dx0, c->dy0,
                                                                      c->dz0, c->part);
+    return hipGetLastError();
+}
+
+hipError_t launch_clear_nchem_flag(eu3d_ctx* c) {
+    k_clear_nchem_flag<<<1, 1, 0, c->stream>>>(c->sc);
     return hipGetLastError();
 }
 

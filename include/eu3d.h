@@ -110,6 +110,12 @@ typedef struct {
  * (src/Flowfield.cpp:72-145, src/Reaction.cpp:49-99). Allocates every field on the current HIP
  * device. Fails with EU3D_ERR_MECH_MISMATCH unless the mechanism equals the compiled-in one. */
 int eu3d_create(eu3d_ctx **out, const eu3d_grid *grid, const eu3d_mech *mech, uint32_t quirks);
+/* Sizes a context for this grid takes, from host arithmetic alone (no GPU call): the number of
+ * per-block CFL partial minima its kernels write (one per 64x4 block of the interior, per k
+ * layer) and the device bytes eu3d_create allocates (fields, nodes, scalars, compaction list,
+ * partials, RK2 copy; the lazily grown chemistry and surrogate lists come on top). Supports
+ * planning the memory of Flowfield::Construction (src/Flowfield.cpp:72-145) on one device. */
+int eu3d_grid_sizes(int ni, int nj, int nk, int bc, long long *cfl_partials, unsigned long long *device_bytes);
 int eu3d_destroy(eu3d_ctx *ctx);
 const char *eu3d_last_error(const eu3d_ctx *ctx);
 int eu3d_sync(eu3d_ctx *ctx);

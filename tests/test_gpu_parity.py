@@ -246,7 +246,8 @@ def test_step_failed_prediction_stores_advected_state():
     ma[2:-2, 2:-2, 2:-2] = -7.0
     g.set("MA", ma)
     cs0 = g.get("CS")
-    g.step(1, cfl=0.05, final_time=0.0, bc_kinds=wl.bc_kinds, rk=1, chem=3, nchem_max=200, skip_derive=True)
+    with pytest.raises(Eu3dError, match="Nchem"):
+        g.step(1, cfl=0.05, final_time=0.0, bc_kinds=wl.bc_kinds, rk=1, chem=3, nchem_max=200, skip_derive=True)
     with pytest.raises(Eu3dError, match="Nchem"):
         g.chem_stats()
     inner = (slice(2, -2),) * 3
@@ -257,6 +258,31 @@ def test_step_failed_prediction_stores_advected_state():
     # RHS = dt L = 0, so the conserved state is the one before the step
     assert np.array_equal(g.get("CS")[inner], cs0[inner])
     assert (g.get("NCHEM")[inner] == 0).all()
+
+
+def test_nchem_error_does_not_stick():
+    """A failed step reports Nchem <= 0 once: the next clean step, the next chemistry call and the
+    statistics after it return OK; a multi-step call whose later steps fail reports it once."""
+    say("case nchem error is per call")
+    from eu3d_amd.api import Context, Eu3dError
+    from tests.common import init_state
+    wl = small_det()
+    g = Context(wl.xnode, wl.ynode, wl.znode, quirks=Q_FIXED)
+    init_state(g, wl)
+    kw = dict(cfl=0.05, bc_kinds=wl.bc_kinds, rk=1, chem=3, nchem_max=200)
+    with pytest.raises(Eu3dError, match="Nchem"):
+        g.step(1, final_time=0.0, **kw)
+    g.step(1, **kw)                      # clean: dt > 0
+    g.chem_stats()
+    g.set_time(1e-8, 0.0)
+    g.trapezoid_adaptive(200)
+    g.chem_stats()
+    # a final time 1e-9 s ahead: the first step of the call reaches it, the later ones have dt = 0
+    dt, t = g.get_time()
+    with pytest.raises(Eu3dError, match="Nchem"):
+        g.step(3, final_time=t + 1e-9, **kw)
+    g.step(1, **kw)
+    g.chem_stats()
 
 
 @pytest.mark.parametrize("field,value,ijk,flagged", [
@@ -280,3 +306,33 @@ def test_check_finite_flags_one_cell(field, value, ijk, flagged):
     a[ijk] = value
     g.set(field, a)
     assert (g.check_finite() != 0) == flagged
+
+
+@pytest.mark.parametrize("shape", [(1, 1, 200), (8, 64, 64)])
+def test_steps_imex_thin_grid(shape):
+    """Grids that write more CFL partials (one per 64x4 block per k layer) than n3/64 + 64: a
+    1x1x200 z tube and an 8x64x64 slab. The partial buffer is sized by the block count, so the
+    step's dt and state match the oracle."""
+    say("case imex thin grid", shape)
+    from eu3d_amd import configs
+    from eu3d_amd.mech import load_default
+    m = load_default()
+    ni, nj, nk = shape
+    dx = 1e-4
+    x, y, z = configs.make_nodes(ni, dx), configs.make_nodes(nj, dx), configs.make_nodes(nk, dx)
+    sh = (ni + 4, nj + 4, nk + 4)
+    X, Y, Z = np.meshgrid(*(np.arange(n, dtype=float) for n in sh), indexing="ij")
+    wave = np.sin(0.21 * X + 0.13 * Y + 0.17 * Z)
+    prims = {"T": 1200.0 + 400.0 * wave, "P": 101325.0 * (2.0 + wave), "U": 80.0 * wave,
+             "V": -40.0 * wave, "W": 60.0 * np.cos(0.17 * Z)}
+    phi = 0.5 * (1 + wave)[..., None]
+    prims["YI"] = phi * configs.premix(m) + (1 - phi) * configs.burnt(m)
+    wl = configs.Workload("thin", x, y, z, prims, (1,) * 6, {}, 2)
+    g, o = make_pair(wl, Q_FIXED)
+    kw = dict(nsteps=2, cfl=0.3, bc_kinds=wl.bc_kinds, rk=1, chem=1)
+    g.step(**kw)
+    o.step(**kw)
+    assert g.check_finite() == 0
+    dg, do = g.get_time(), o.get_time()
+    assert abs(dg[0] - do[0]) <= 1e-12 * do[0] and abs(dg[1] - do[1]) <= 1e-12 * do[1]
+    compare(g, o, ["CS", "T", "D", "P", "E", "YI"])

@@ -52,7 +52,9 @@ def test_gelu_closed_forms():
 def test_bf16_rounding_model():
     x = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -9, -3.14159])
     r = om.bf16_round(x)
-    assert r[0] == 1.0 and r[1] == 1.0 + 2 ** -7 * 0 + 2 ** -8 * 0 or True
+    # spacing 2^-7 on [1, 2) and 2^-6 on [2, 4): 1 + 2^-8 is a tie (to even: 1.0), 1 + 3 * 2^-9 is
+    # above the midpoint, and 3.14159 is 201.06 steps of 2^-6
+    assert list(r) == [1.0, 1.0, 1.0 + 2 ** -7, -201 * 2 ** -6]
     assert np.all(np.abs(r - x) <= np.abs(x) * 2 ** -8)
     assert om.bf16_round(np.array([1.0 + 2 ** -8]))[0] == 1.0        # tie -> even
     assert om.bf16_round(np.array([1.0 + 3 * 2 ** -8]))[0] == 1.0 + 2 ** -6
