@@ -38,7 +38,7 @@ EXPORTED = [
     "eu3d_update_after_adv", "eu3d_update_after_cs", "eu3d_get_partial_t", "eu3d_explicit",
     "eu3d_diagonalized", "eu3d_update_imex", "eu3d_trapezoid", "eu3d_trapezoid_prediction",
     "eu3d_trapezoid_adaptive", "eu3d_check_finite", "eu3d_step", "eu3d_get_chem_stats",
-    "eu3d_chem_batch", "eu3d_chem_set_schedule", "eu3d_chem_get_schedule", "eu3d_device_field",
+    "eu3d_chem_batch", "eu3d_chem_batch_noalloc", "eu3d_chem_set_schedule", "eu3d_chem_get_schedule", "eu3d_device_field",
     "eu3d_cells_with_ghosts", "eu3d_stream", "eu3d_time_kernel",
     "eu3d_mlp_create", "eu3d_mlp_destroy", "eu3d_mlp_param_count", "eu3d_mlp_get_params", "eu3d_mlp_set_params",
     "eu3d_mlp_get_grads", "eu3d_mlp_forward", "eu3d_mlp_forward_layers", "eu3d_mlp_train_step", "eu3d_gemm_bf16", "eu3d_mlp_stream",
@@ -111,6 +111,8 @@ def load_library(path: str = LIB_PATH):
     vp, dp, ip = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)
     lib.eu3d_create.argtypes = [C.POINTER(vp), C.POINTER(_Grid), C.POINTER(_Mech), C.c_uint32]
     lib.eu3d_destroy.argtypes = [vp]
+    lib.eu3d_chem_batch_noalloc.argtypes = [C.c_longlong, vp, vp, vp, C.c_double, C.c_int, C.c_uint32,
+                                            C.POINTER(ChemSchedule), vp, vp, C.POINTER(ChemStats)]
     lib.eu3d_grid_sizes.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_longlong),
                                     C.POINTER(C.c_ulonglong)]
     lib.eu3d_last_error.argtypes = [vp]
@@ -176,6 +178,32 @@ def grid_sizes(ni: int, nj: int, nk: int, bc: int = 2):
     if rc != 0:
         raise Eu3dError(rc, "eu3d_grid_sizes")
     return parts.value, nbytes.value
+
+
+NOALLOC_MAX_CELLS = 1 << 22
+
+
+def chem_batch_noalloc(n: int, rho_ptr: int, T_ptr: int, E_ptr: int, dtm: float, nchem_max: int = 0,
+                       quirks: int = 0, schedule=None, stream: int | None = None, nchem_ptr: int | None = None,
+                       stats: bool = True):
+    """eu3d_chem_batch_noalloc: the batched chemistry with no context and no device allocation,
+    on device pointers (rho_s [ns][n], T [n], E [n]). schedule: (block, blocks_per_cu, waves,
+    tail) or None; stream: a hipStream_t handle (None: the null stream). With stats=False the
+    call only enqueues and returns None."""
+    lib = load_library()
+    sp = None
+    if schedule is not None:
+        sp = C.byref(ChemSchedule(*schedule))
+    st = ChemStats() if stats else None
+    rc = lib.eu3d_chem_batch_noalloc(n, C.c_void_p(rho_ptr), C.c_void_p(T_ptr), C.c_void_p(E_ptr), dtm, nchem_max,
+                                     quirks, sp, C.c_void_p(stream) if stream else None,
+                                     C.c_void_p(nchem_ptr) if nchem_ptr else None, C.byref(st) if stats else None)
+    if rc != 0:
+        raise Eu3dError(rc, "eu3d_chem_batch_noalloc")
+    if not stats:
+        return None
+    return {"cells": st.cells, "active_cells": st.active_cells, "substeps_total": st.substeps_total,
+            "nchem_max_seen": st.nchem_max_seen}
 
 
 def _dptr(a: np.ndarray):

@@ -4,6 +4,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "eu3d_internal.hpp"
@@ -39,6 +40,45 @@ int hip_fail(eu3d_ctx* c, hipError_t e, const char* where) {
         hipError_t e_ = (call);                         \
         if (e_ != hipSuccess) return hip_fail(c, e_, where); \
     } while (0)
+
+// ---- scratch of eu3d_chem_batch_noalloc: counters (one Scalars), the compaction list and the
+// per-cell sub-step counts (launch_chem_batch keeps them side by side), and the tail queue.
+// 12 bytes a cell for EU3D_NOALLOC_MAX_CELLS cells, 48 MiB and 4.3 KB per GPU, held from the
+// library's load on that GPU until the process ends.
+constexpr size_t NOALLOC_SCALARS = (sizeof(Scalars) + 255) & ~(size_t)255;
+constexpr size_t NOALLOC_LISTS = (size_t)EU3D_NOALLOC_MAX_CELLS * (sizeof(unsigned) + sizeof(int));
+constexpr size_t NOALLOC_BYTES = NOALLOC_SCALARS + NOALLOC_LISTS + (size_t)EU3D_NOALLOC_MAX_CELLS * sizeof(unsigned);
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+// The array's device address and the CU count, looked up once per device with that device
+// current. In host code `scratch` names a host-side stand-in, never the device memory.
+constexpr int NOALLOC_MAX_DEVICES = 64;
+struct NoallocDevice {
+    unsigned char* ptr = nullptr;
+    int num_cu = 0;
+};
+NoallocDevice noalloc_devices[NOALLOC_MAX_DEVICES];
+std::mutex noalloc_mutex;
+
+hipError_t noalloc_device(unsigned char*& ptr, int& num_cu) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(noalloc_mutex);
+    NoallocDevice& d = noalloc_devices[dev];
+    if (!d.ptr) {
+        void* p = nullptr;
+        if ((e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch))) != hipSuccess) return e;
+        int cu = 0;
+        if ((e = hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
+        d.num_cu = cu > 0 ? cu : 256;
+        d.ptr = static_cast<unsigned char*>(p);
+    }
+    ptr = d.ptr;
+    num_cu = d.num_cu;
+    return hipSuccess;
+}
 
 bool same(double a, double b) { return std::memcmp(&a, &b, sizeof(double)) == 0; }
 
@@ -447,6 +487,44 @@ int eu3d_time_kernel(eu3d_ctx* c, int kernel, int reps, int diff, int limiter, d
     if (e != hipSuccess) return hip_fail(c, e, "eu3d_time_kernel");
     *avg_ms = total / reps;
     return EU3D_OK;
+}
+
+int eu3d_chem_batch_noalloc(long long n, double* rho_s, double* T, double* E, double dtm, int nchem_max,
+                            uint32_t quirks, const eu3d_chem_schedule* sched, void* stream, int* nchem_out,
+                            eu3d_chem_stats* stats) {
+    if (n <= 0 || n > EU3D_NOALLOC_MAX_CELLS || !rho_s || !T || !E || !(dtm > 0)) return EU3D_ERR_ARG;
+    if (sched && ((sched->block != 64 && sched->block != 128 && sched->block != 256) || sched->blocks_per_cu < 1 ||
+                  sched->blocks_per_cu > 64))
+        return EU3D_ERR_ARG;
+    unsigned char* base = nullptr;
+    int num_cu = 0;
+    if (noalloc_device(base, num_cu) != hipSuccess) return EU3D_ERR_HIP;
+    // A context on the stack that owns nothing: its scalars and lists point into this GPU's
+    // scratch, with capacities that make launch_chem_batch's growth checks never allocate.
+    eu3d_ctx c;
+    c.quirks = quirks;
+    c.num_cu = num_cu;
+    c.stream = (hipStream_t)stream;
+    c.sc = reinterpret_cast<Scalars*>(base);
+    c.batch_active = reinterpret_cast<unsigned*>(base + NOALLOC_SCALARS);   // list, then keep
+    c.batch_cap = EU3D_NOALLOC_MAX_CELLS;
+    c.coop_list = reinterpret_cast<unsigned*>(base + NOALLOC_SCALARS + NOALLOC_LISTS);
+    c.coop_cap = EU3D_NOALLOC_MAX_CELLS;
+    if (sched) {
+        c.sched = *sched;
+        if (c.sched.waves != 1 && c.sched.waves != 2 && c.sched.waves != 4) c.sched.waves = 2;
+        if (c.sched.tail < 0) c.sched.tail = 0;
+    }
+    // the previous call's counters and flags stay in the array: k_chem_reset clears the queue
+    // heads and statistics, this clears the Nchem report
+    if (launch_clear_nchem_flag(&c) != hipSuccess) return EU3D_ERR_HIP;
+    if (launch_chem_batch(&c, n, rho_s, T, E, dtm, nchem_max, nchem_out) != hipSuccess) return EU3D_ERR_HIP;
+    if (!stats) return EU3D_OK;
+    c.stats = eu3d_chem_stats{};
+    const int rc = chem_status(&c);
+    c.stats.cells = n;
+    *stats = c.stats;
+    return rc;
 }
 
 size_t eu3d_cells_with_ghosts(const eu3d_ctx* c) { return c ? c->g.n3 : 0; }

@@ -207,6 +207,24 @@ typedef struct {
 int eu3d_chem_batch(eu3d_ctx *ctx, long long n, double *rho_s, double *T, double *E, double dtm,
                     int nchem_max, int *nchem_out, eu3d_chem_stats *stats);
 int eu3d_chem_set_schedule(eu3d_ctx *ctx, const eu3d_chem_schedule *s);
+
+/* eu3d_chem_batch with no context and no allocation. No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). Same cells, arguments and results as eu3d_chem_batch; its
+ * scratch (counters, the compaction list, per-cell sub-step counts and the tail queue: 12 bytes a
+ * cell) lives in one static device array per GPU sized for n <= EU3D_NOALLOC_MAX_CELLS (2^22,
+ * 48 MiB); larger n returns EU3D_ERR_ARG, as do bad pointers, dtm <= 0 or a schedule that
+ * eu3d_chem_set_schedule would refuse. quirks as eu3d_create (the compiled mechanism); sched NULL
+ * takes the default schedule; stream is a hipStream_t (NULL: the null stream).
+ * With stats NULL the call only enqueues kernels on the stream (no copy, no synchronisation);
+ * with stats it waits, reads the counters back and reports Nchem <= 0 as EU3D_ERR_NCHEM.
+ * The first call on a device looks up the array's address (hipGetSymbolAddress). Every call on
+ * one GPU shares that GPU's array: calls that overlap in time on one GPU race, so serialise them
+ * (one stream, or events between streams). */
+#define EU3D_NOALLOC_MAX_CELLS (1ll << 22)
+int eu3d_chem_batch_noalloc(long long n, double *rho_s, double *T, double *E, double dtm, int nchem_max,
+                            uint32_t quirks, const eu3d_chem_schedule *sched, void *stream, int *nchem_out,
+                            eu3d_chem_stats *stats);
 int eu3d_chem_get_schedule(eu3d_ctx *ctx, eu3d_chem_schedule *s);
 
 /* ---- MLP chemistry surrogate (no reference counterpart: the reference's "DNN" scheme only
