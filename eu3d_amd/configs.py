@@ -91,6 +91,13 @@ def zn1d(ni: int = 2000, dx: float = 5e-5, n_driver: int = 40, nsteps: int = 200
                     dict(cfl=0.5, rk=2, chem=3, diff=1, limiter=0, nchem_max=10000), nsteps)
 
 
+def cfl_ramp(step: int, cfl: float = 0.5, start: float = 0.05, nsteps: int = 20) -> float:
+    """CFL number of step `step` (from 0) of a detonation start: linear from `start` to `cfl` over
+    the first `nsteps` steps. The first AUSM step across the 30:1 driver jump at CFL 0.5 leaves
+    the finite range in the reference arithmetic too; the ramp is build-defined."""
+    return start + (cfl - start) * min(1.0, step / nsteps)
+
+
 def shock_post(M: float, T_pre: float, P_pre: float):
     """Post-shock state of the moving normal shock, as src/Flowfield.cpp:172-174 forms it."""
     U_post = M * (1 - (2 + 0.40 * M ** 2) / (2.40 * M ** 2)) * np.sqrt(1.40 * 287 * T_pre)

@@ -100,3 +100,28 @@ def small_det(seed=9):
 
 def small_zn(ni=120):
     return configs.zn1d(ni=ni, n_driver=10)
+
+
+def oracle_batch(rho, T, E, dtm, quirks=0):
+    """The oracle's Euler::Trapezoid(type 1) cell loop on states at rest, laid out along x."""
+    from oracle import OracleFlow
+    n = T.shape[0]
+    x = configs.make_nodes(n, 1e-4)
+    y = configs.make_nodes(1, 1e-4)
+    o = OracleFlow(x, y, y, quirks=quirks)
+    sh = o.shape3
+    cs = np.zeros(sh + (13,))
+    cs[..., 0:9] = rho[0]            # ghosts: any valid state
+    cs[..., 12] = rho[0].sum() * E[0]
+    cs[2:-2, 2, 2, :9] = rho
+    cs[2:-2, 2, 2, 12] = rho.sum(1) * E
+    To = np.full(sh, T[0])
+    To[2:-2, 2, 2] = T
+    o.set("CS", cs)
+    o.set("T", To)
+    o.update_after_cs()
+    T_in = o.get("T")[2:-2, 2, 2].copy()
+    E_in = o.get("E")[2:-2, 2, 2].copy()
+    o.set_time(dtm)
+    o.trapezoid_adaptive(10000)
+    return T_in, E_in, o.get("CS")[2:-2, 2, 2, :9], o.get("T")[2:-2, 2, 2], o.get("NCHEM")[2:-2, 2, 2]

@@ -1,0 +1,180 @@
+"""The benchmark configurations as stated, against the oracle through the C ABI.
+
+- 1-D detonation tube: 2000 cells, the 40-cell driver zone, SSP-RK2 with the adaptive trapezoid,
+  the CFL ramp to 0.5, 100 steps (src/Euler.cpp:266-333 per step).
+- Batched chemistry on equilibrium-line states: the cells with the longest sub-step chains (up to
+  the 10000 cap, through the tail kernel) and a random sample of the rest, one dt = 1e-8 s step
+  (src/Euler.cpp:526-556, src/Reaction.cpp:489-670).
+- The surrogate chemistry on the grid with cells the network accepts: element conservation, the
+  |dT| bound, and every rejected or non-network cell equal to the FP64 step.
+"""
+import numpy as np
+import pytest
+
+from tests.common import compare, make_pair, oracle_batch, say, small_det
+
+pytestmark = pytest.mark.gpu
+Q_FIXED = 0
+
+
+def _t():
+    import torch
+    assert torch.cuda.is_available()
+    return torch
+
+
+def test_zn1d_2000_as_stated_100_steps():
+    """The 1-D detonation configuration exactly as stated (2000 cells of 5e-5 m, driver zone of 40
+    cells at 30 atm and 3000 K, RK2, adaptive trapezoid, CFL ramp to 0.5) for 100 steps."""
+    from eu3d_amd import configs
+    say("case zn1d 2000 as stated, 100 steps")
+    wl = configs.zn1d()
+    assert wl.ncells == 2000 and (wl.prims["T"][2:-2, 2, 2] == 3000.0).sum() == 40
+    g, o = make_pair(wl, Q_FIXED)
+    for s in range(100):
+        kw = dict(cfl=configs.cfl_ramp(s), bc_kinds=wl.bc_kinds, rk=2, chem=3, nchem_max=10000)
+        g.step(1, skip_derive=s < 99, **kw)
+        o.step(1, **kw)
+    assert g.check_finite() == 0
+    dg, do = g.get_time(), o.get_time()
+    assert abs(dg[1] - do[1]) <= 1e-10 * do[1]
+    worst = compare(g, o, ["CS", "T", "D", "P", "E", "YI"])
+    say("  t", do[1], "worst rel", worst)
+
+
+def test_batch_equilibrium_states_vs_oracle():
+    """2^18 equilibrium-line states; the 48 cells with the most sub-steps at dt = 1e-8 s (the
+    10000-sub-step cap among them) and 2000 others, against the oracle's adaptive trapezoid:
+    sub-step counts bit for bit; states within 1e-10, or four times the oracle's own response to a
+    1-ulp change of its inputs where that is larger (a chain of 10000 dependent sub-steps)."""
+    torch = _t()
+    from eu3d_amd import chemistry, configs
+    from eu3d_amd.api import Context
+    say("case batch equilibrium states")
+    x = configs.make_nodes(8, 1e-4)
+    g = Context(x, x, x, quirks=Q_FIXED)
+    n = 1 << 18
+    (r, t, e), (rho, T, E) = chemistry.states(g, n, seed=2024, burnt="equilibrium")
+    nch = torch.empty(n, dtype=torch.int32, device="cuda")
+    g.chem_batch(n, r.data_ptr(), t.data_ptr(), e.data_ptr(), 1e-8, 10000, nch.data_ptr())
+    nch = nch.cpu().numpy()
+    order = np.argsort(-nch, kind="stable")
+    heavy = order[:48]
+    rng = np.random.default_rng(5)
+    rest = rng.choice(order[48:], 2000, replace=False)
+    sel = np.concatenate([heavy, rest])
+    say("  sub-steps of the heavy cells", nch[heavy].min(), "..", nch[heavy].max(),
+        "cells at the cap in 2^18:", int((nch == 10000).sum()))
+    assert nch[heavy].max() == 10000
+    rs, Ts, Es = rho[sel], T[sel], E[sel]
+    T_in, E_in, ro, To, nch_o = oracle_batch(rs, Ts, Es, 1e-8)
+    ro_p = oracle_batch(rs * (1 + 2.0 ** -52), Ts, Es, 1e-8)[2]
+    fin = np.isfinite(ro).all(1) & np.isfinite(ro_p).all(1) & np.isfinite(To)
+    assert fin.all()
+    scale = ro.sum(1).max()
+    cond = np.abs(ro_p - ro).max() / scale
+    tol = max(1e-10, 4 * cond)
+    m = len(sel)
+    rd = torch.tensor(rs.T.copy(), device="cuda")
+    td = torch.tensor(T_in, device="cuda")
+    ed = torch.tensor(E_in, device="cuda")
+    nd = torch.empty(m, dtype=torch.int32, device="cuda")
+    st = g.chem_batch(m, rd.data_ptr(), td.data_ptr(), ed.data_ptr(), 1e-8, 10000, nd.data_ptr())
+    torch.cuda.synchronize()
+    assert (nd.cpu().numpy() == nch_o).all()
+    assert st["nchem_max_seen"] == 10000 and st["substeps_total"] == int(nch_o.sum())
+    err = np.abs(rd.cpu().numpy().T - ro).max() / scale
+    terr = np.abs(td.cpu().numpy() - To).max()
+    say("  substeps", st["substeps_total"], "rel err", err, "T err", terr, "conditioning", cond, "tol", tol)
+    assert err < tol
+    assert terr < max(1e-10, 4 * cond) * 3000
+
+
+# H, O, N atoms of H, O, H2O, OH, O2, H2, H2O2, HO2, N2
+ATOMS = np.array([[1, 0, 2, 1, 0, 2, 2, 1, 0], [0, 1, 1, 1, 2, 0, 2, 2, 0], [0, 0, 0, 0, 0, 0, 0, 0, 2]], dtype=float)
+
+
+def test_surrogate_grid_step_accepts_cells():
+    """A network trained on the detonation's own FP64 chemistry takes the cells needing more than
+    one sub-step (hybrid mode) in one step of a small detonation. Three contexts start from the
+    same state: surrogate, FP64 adaptive trapezoid, and no chemistry. Every cell the surrogate
+    step leaves equal to the FP64 step is a non-network or rejected cell; every other cell was
+    accepted and keeps its density and element content (H, O, N) of the no-chemistry step within
+    the element tolerance, with |T - T_nochem| within the |dT| bound."""
+    _t()
+    from eu3d_amd import configs, surrogate
+    from eu3d_amd.api import MLP, Context
+    from eu3d_amd.mech import load_default
+    say("case surrogate grid accepts cells")
+    mech = load_default()
+    wl = small_det()
+    H = surrogate.Harvest()
+    surrogate.harvest_run(H, wl, nsteps=8, every=1, min_T=600.0)
+    x = configs.make_nodes(8, 1e-4)
+    gb = Context(x, x, x, quirks=Q_FIXED)
+    surrogate.harvest_trajectories(gb, H, 1 << 12, [1e-9, 2e-9], nsteps=12, seed=9)
+    r0, r1, t0, dt = H.cat()
+    m = MLP(seed=3)
+    p = surrogate.fit_params(r0, r1, t0, dt, nchem_threshold=1, T_cold=600.0)
+    m.set_surrogate(p)
+    X, Y = surrogate.make_dataset(m, r0, r1, t0, dt)
+    hist = surrogate.train(m, X, Y, epochs=12, batch=4096, lr=2e-3)
+    say("  pairs", H.n, "loss", hist[0], "->", hist[-1])
+    # first with the fitted bounds, then with the |dT| bound at the median of the accepted cells'
+    # temperature change, so that about half of them are rejected and must equal FP64
+    dT = _grid_step_check(wl, m, p, mech, expect_rejects=False)
+    p.dT_max = float(np.median(dT))
+    m.set_surrogate(p)
+    _grid_step_check(wl, m, p, mech, expect_rejects=True)
+
+
+def _grid_step_check(wl, m, p, mech, expect_rejects):
+    from eu3d_amd import configs
+    from eu3d_amd.api import Context
+    from tests.common import init_state
+    ctx = {}
+    for name in ("sur", "fp64", "none"):
+        g = Context(wl.xnode, wl.ynode, wl.znode, quirks=Q_FIXED)
+        init_state(g, wl)
+        g.step(1, cfl=configs.cfl_ramp(0), bc_kinds=wl.bc_kinds, rk=1, chem=3, nchem_max=10000)
+        ctx[name] = g
+    ctx["sur"].set_surrogate(m)
+    kw = dict(cfl=configs.cfl_ramp(1), bc_kinds=wl.bc_kinds, rk=1, nchem_max=10000)
+    ctx["sur"].step(1, chem=4, **kw)
+    ctx["fp64"].step(1, chem=3, **kw)
+    ctx["none"].step(1, chem=0, **kw)
+    st = ctx["sur"].surrogate_stats()
+    accepted = st["network"] - st["fallback"]
+    say("  dT_max", p.dT_max, "stats", st)
+    assert st["network"] > 0 and accepted > 0
+    if expect_rejects:
+        assert st["fallback"] > 0 and st["fallback"] == st["dT"]
+    I = (slice(2, -2),) * 3
+    cs = {k: g.get("CS")[I].reshape(-1, 13) for k, g in ctx.items()}
+    T = {k: g.get("T")[I].reshape(-1) for k, g in ctx.items()}
+    # non-network and rejected cells take the FP64 sub-steps (the rejected ones re-run from their
+    # advected state through the list kernels, equal to the fused pass up to the last bits)
+    csc = np.abs(cs["fp64"]).max(0)
+    same = (np.abs(T["sur"] - T["fp64"]) <= 1e-12 * T["fp64"]) & \
+        (np.abs(cs["sur"] - cs["fp64"]) <= 1e-12 * csc).all(1)
+    diff = ~same
+    say("  cells changed by the network", int(diff.sum()), "accepted", accepted)
+    assert 0 < diff.sum() <= accepted
+    rho_s, rho_n = cs["sur"][diff, :9], cs["none"][diff, :9]
+    assert np.allclose(rho_s.sum(1), rho_n.sum(1), rtol=1e-13, atol=0)
+    el_s, el_n = (rho_s / mech.mw) @ ATOMS.T, (rho_n / mech.mw) @ ATOMS.T
+    # the element check bounds the projected increment by elem_tol of the content; clipping the
+    # mass fractions that end in [-y_neg_tol, 0) moves at most y_neg_tol of the density per
+    # species, and the renormalisation that follows scales by at most 9 y_neg_tol
+    D = rho_n.sum(1, keepdims=True)
+    clip = D * p.y_neg_tol * (ATOMS / mech.mw).sum(1)[None, :]
+    bound = (p.elem_tol + 9 * p.y_neg_tol) * el_n + clip
+    assert (np.abs(el_s - el_n) <= bound).all()
+    say("  worst element change / content", float((np.abs(el_s - el_n) / np.maximum(el_n, 1e-300)).max()))
+    dT = np.abs(T["sur"][diff] - T["none"][diff])
+    assert (dT <= p.dT_max).all()
+    # momentum is untouched by the chemistry
+    assert np.allclose(cs["sur"][diff, 9:12], cs["none"][diff, 9:12], rtol=1e-12, atol=1e-12)
+    for g in ctx.values():
+        g.close()
+    return dT

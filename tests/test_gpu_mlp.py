@@ -127,30 +127,7 @@ def _states(n, seed=11):  # device copies of configs.chem_states
             torch.tensor(E, device="cuda"), rho, T, E)
 
 
-def _oracle_batch(rho, T, E, dtm, quirks=0):
-    """The oracle's Euler::Trapezoid(type 1) cell loop on states at rest, laid out along x."""
-    from eu3d_amd import configs
-    from oracle import OracleFlow
-    n = T.shape[0]
-    x = configs.make_nodes(n, 1e-4)
-    y = configs.make_nodes(1, 1e-4)
-    o = OracleFlow(x, y, y, quirks=quirks)
-    sh = o.shape3
-    cs = np.zeros(sh + (13,))
-    cs[..., 0:9] = rho[0]            # ghosts: any valid state
-    cs[..., 12] = rho[0].sum() * E[0]
-    cs[2:-2, 2, 2, :9] = rho
-    cs[2:-2, 2, 2, 12] = rho.sum(1) * E
-    To = np.full(sh, T[0])
-    To[2:-2, 2, 2] = T
-    o.set("CS", cs)
-    o.set("T", To)
-    o.update_after_cs()
-    T_in = o.get("T")[2:-2, 2, 2].copy()
-    E_in = o.get("E")[2:-2, 2, 2].copy()
-    o.set_time(dtm)
-    o.trapezoid_adaptive(10000)
-    return T_in, E_in, o.get("CS")[2:-2, 2, 2, :9], o.get("T")[2:-2, 2, 2], o.get("NCHEM")[2:-2, 2, 2]
+from tests.common import oracle_batch as _oracle_batch  # noqa: E402
 
 
 Q_C3 = 1 << 7   # the reference's trapezoid production/destruction grouping (include/eu3d.h)
