@@ -88,7 +88,8 @@ class SurrogateParams(C.Structure):
     _fields_ = [("feat_mean", C.c_float * 12), ("feat_std", C.c_float * 12), ("out_mean", C.c_float * 9),
                 ("out_std", C.c_float * 9), ("hull_lo", C.c_float * 12), ("hull_hi", C.c_float * 12),
                 ("dT_max", C.c_double), ("elem_tol", C.c_double), ("y_neg_tol", C.c_double),
-                ("T_cold", C.c_double), ("nchem_threshold", C.c_int)]
+                ("T_cold", C.c_double), ("nchem_threshold", C.c_int), ("out_eps", C.c_double),
+                ("nchem_threshold_hi", C.c_int)]
 
 
 class SurrogateStats(C.Structure):
@@ -399,10 +400,10 @@ class Context:
         return self.lib.eu3d_stream(self.h)
 
     def time_kernel(self, kernel: str, reps: int = 10, diff: int = 1, limiter: int = 0) -> float:
-        """Mean launch duration [ms] of one step kernel ("flux_z", "flux_xy", "imex_fused"),
-        HIP events on the context stream."""
+        """Mean launch duration [ms] of one step kernel ("flux_z", "flux_xy", "imex_fused",
+        "trap_first"), HIP events on the context stream."""
         ms = C.c_double()
-        k = {"flux_z": 0, "flux_xy": 1, "imex_fused": 2}[kernel]
+        k = {"flux_z": 0, "flux_xy": 1, "imex_fused": 2, "trap_first": 3}[kernel]
         self._chk(self.lib.eu3d_time_kernel(self.h, k, reps, diff, limiter, C.byref(ms)), "time_kernel")
         return ms.value
 

@@ -438,7 +438,7 @@ double* eu3d_device_field(eu3d_ctx* c, int field) {
 // each of `reps` launches on identical inputs (the state the kernel writes is restored between
 // launches, outside the timed region). A profiling hook with no reference counterpart.
 int eu3d_time_kernel(eu3d_ctx* c, int kernel, int reps, int diff, int limiter, double* avg_ms) {
-    if (!c || reps < 1 || !avg_ms || kernel < 0 || kernel > EU3D_KERNEL_IMEX_FUSED || diff < 0 || diff > 1 ||
+    if (!c || reps < 1 || !avg_ms || kernel < 0 || kernel > EU3D_KERNEL_TRAP_FIRST || diff < 0 || diff > 1 ||
         limiter < 0 || limiter > 4)
         return fail(c, EU3D_ERR_ARG, "eu3d_time_kernel: bad arguments");
     const size_t n3 = c->g.n3;
@@ -449,7 +449,8 @@ int eu3d_time_kernel(eu3d_ctx* c, int kernel, int reps, int diff, int limiter, d
         if (e0) (void)hipEventDestroy(e0);
         if (e1) (void)hipEventDestroy(e1);
     };
-    const bool imex = kernel == EU3D_KERNEL_IMEX_FUSED;
+    // k_imex_fused and k_trap_first overwrite their inputs CS, T, E: restored between launches
+    const bool imex = kernel == EU3D_KERNEL_IMEX_FUSED || kernel == EU3D_KERNEL_TRAP_FIRST;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess ||
         (imex && hipMalloc(&save, n3 * (NC + 2) * sizeof(double)) != hipSuccess)) {
         cleanup();
@@ -469,10 +470,12 @@ int eu3d_time_kernel(eu3d_ctx* c, int kernel, int reps, int diff, int limiter, d
     double total = 0;
     hipError_t e = imex ? copy3(true) : hipSuccess;
     for (int r = 0; r < reps && e == hipSuccess; r++) {
+        if (kernel == EU3D_KERNEL_TRAP_FIRST && (e = launch_chem_reset_step(c)) != hipSuccess) break;
         if ((e = hipEventRecord(e0, c->stream)) != hipSuccess) break;
         switch (kernel) {
         case EU3D_KERNEL_FLUX_Z: e = launch_flux_z(c, diff, limiter); break;
         case EU3D_KERNEL_FLUX_XY: e = launch_flux_xy(c, diff, limiter); break;
+        case EU3D_KERNEL_TRAP_FIRST: e = launch_trap_first_step(c, 10000); break;
         default: e = launch_imex_fused(c); break;
         }
         if (e != hipSuccess || (e = hipEventRecord(e1, c->stream)) != hipSuccess || (e = hipEventSynchronize(e1)) != hipSuccess)

@@ -103,8 +103,11 @@ hipError_t launch_trapezoid(eu3d_ctx* c);
 // in slist / scount for the surrogate network instead of taking the FP64 sub-steps. adv: run the
 // step's Update_after_Adv (CS += RHS) and Update_after_CS inside the prediction pass.
 hipError_t launch_trap_adaptive(eu3d_ctx* c, int nchem_max, bool predict_only, int nthr = 0, double T_cold = 0.0,
-                                unsigned* slist = nullptr, unsigned* scount = nullptr, bool adv = false);
+                                unsigned* slist = nullptr, unsigned* scount = nullptr, bool adv = false,
+                                int nthr_hi = 0);
 hipError_t launch_check_finite(eu3d_ctx* c);
+hipError_t launch_chem_reset_step(eu3d_ctx* c);
+hipError_t launch_trap_first_step(eu3d_ctx* c, int nchem_max);
 hipError_t ensure_coop(eu3d_ctx* c, long long n);
 hipError_t launch_set_scalars(eu3d_ctx* c, double dt, double time);
 hipError_t launch_step(eu3d_ctx* c, const eu3d_step_params* p, bool first);
@@ -120,5 +123,5 @@ hipError_t launch_batch_list(eu3d_ctx* c, long long n, double* rho, double* T, d
                              int* nchem_keep, int nchem_max);
 hipError_t launch_chem_batch(eu3d_ctx* c, long long n, double* rho, double* T, double* E, double dtm, int nchem_max,
                              int* nchem_out, int nthr = 0, double T_cold = 0.0, unsigned* slist = nullptr,
-                             unsigned* scount = nullptr);
+                             unsigned* scount = nullptr, int nthr_hi = 0);
 }  // namespace eu3d

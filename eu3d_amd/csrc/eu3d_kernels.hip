@@ -457,13 +457,17 @@ __global__ void __launch_bounds__(256) k_ee(G3 g, Fields F, const double* __rest
 // ---------------------------------------------------------------- CFL
 // Flowfield::CFLcondition, src/Flowfield.cpp:242-272: each block reduces its cells' bound and
 // writes one partial; k_cfl_finish reduces the partials in one block (no contended atomics).
+// The per-cell time-step bound of Flowfield::CFLcondition (src/Flowfield.cpp:242-272). REF (quirk
+// F6): min(dx/(|u|+c), dy/(|v|+c)), z ignored. FIXED: the bound of the unsplit three-direction
+// update, 1 / sum_d (|u_d|+c)/dx_d, where the launcher passes dx_d = inf for a one-cell direction.
 __device__ __forceinline__ double cfl_bound(double dx, double dy, double dz, double U, double V, double W, double C,
                                             uint32_t q) {
-    const double a = dx / (fabs(U) + C), b = dy / (fabs(V) + C);
-    double v = (b < a) ? b : a;
-    if (!(q & Q_F6)) {
-        const double z = dz / (fabs(W) + C);
-        v = (z < v) ? z : v;
+    double v;
+    if (q & Q_F6) {
+        const double a = dx / (fabs(U) + C), b = dy / (fabs(V) + C);
+        v = (b < a) ? b : a;
+    } else {
+        v = 1.0 / ((fabs(U) + C) / dx + (fabs(V) + C) / dy + (fabs(W) + C) / dz);
     }
     return (v < 1e10) ? v : 1e10;
 }
@@ -1191,7 +1195,8 @@ __device__ __forceinline__ void chem_count_tail(Scalars* sc, int nch, bool activ
 
 // Prediction + first sub-step for every interior cell; cells needing more sub-steps are
 // compacted into the work list. With nthr > 0 (surrogate chemistry), a cell at or above T_cold
-// needing more than nthr sub-steps keeps its state and goes to the network list instead.
+// needing more than nthr sub-steps (and at most nthr_hi, when nthr_hi > 0) keeps its state and
+// goes to the network list instead.
 // ADV: the step's Update_after_Adv (CS += RHS, src/Flowfield.cpp:1140-1157) and Update_after_CS
 // run first in registers, and the inputs of the prediction come from that state instead of the
 // fields it would have written. A cell whose prediction fails (Nchem <= 0, the reference's abort)
@@ -1200,7 +1205,8 @@ __device__ __forceinline__ void chem_count_tail(Scalars* sc, int nch, bool activ
 template <bool C3, bool ADV>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ADV ? 2 : 1, ADV ? 2 : 1)))
 k_trap_first(G3 g, Fields F, uint32_t q, const Scalars* __restrict__ scc, Scalars* sc, unsigned* list, int nmax,
-             int predict_only, int nthr, double T_cold, unsigned* slist, unsigned* scount, TailRoute rt) {
+             int predict_only, int nthr, int nthr_hi, double T_cold, unsigned* slist, unsigned* scount,
+             TailRoute rt) {
     int i, j, k;
     const bool in = interior_ijk(g, i, j, k);
     const size_t c = in ? IDX(g, i, j, k) : 0;
@@ -1265,7 +1271,7 @@ k_trap_first(G3 g, Fields F, uint32_t q, const Scalars* __restrict__ scc, Scalar
         nch = chem_nchem(ko, Dr, Y, dtm, nmax);
         FLD(F, EU3D_FIELD_NCHEM)[c] = (double)nch;
         if (nch <= 0) atomicOr(&sc->nonfinite, 2);
-        to_net = nthr > 0 && nch > nthr && cell.T >= T_cold;
+        to_net = nthr > 0 && nch > nthr && (nthr_hi <= 0 || nch <= nthr_hi) && cell.T >= T_cold;
         if (!predict_only && nch > 0 && !to_net) {
             using Acc = typename std::conditional<ADV, GridCellsStep, GridCells>::type;
             const Acc acc{g, F};
@@ -1691,7 +1697,7 @@ __global__ void __launch_bounds__(64) k_chem_tail(Acc acc, const unsigned* __res
 // Batch: prediction + first sub-step on flat cells at rest.
 template <bool C3>
 __global__ void __launch_bounds__(256) k_batch_first(BatchCells acc, int* nchem_out, int* nchem_keep, uint32_t q,
-                                                     double dtm, int nmax, Scalars* sc, unsigned* list, int nthr,
+                                                     double dtm, int nmax, Scalars* sc, unsigned* list, int nthr, int nthr_hi,
                                                      double T_cold, unsigned* slist, unsigned* scount, TailRoute rt) {
     const long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     const bool in = id < acc.n;
@@ -1717,7 +1723,7 @@ __global__ void __launch_bounds__(256) k_batch_first(BatchCells acc, int* nchem_
         nchem_keep[id] = nch;
         if (nchem_out) nchem_out[id] = nch;
         if (nch <= 0) atomicOr(&sc->nonfinite, 2);
-        to_net = nthr > 0 && nch > nthr && cell.T >= T_cold;
+        to_net = nthr > 0 && nch > nthr && (nthr_hi <= 0 || nch <= nthr_hi) && cell.T >= T_cold;
         if (nch > 0 && !to_net) {
             double cs[NC];
             const Prim o = chem_apply(cell, ko, Dr, dtm / nch, q, cs, tp);
@@ -1946,9 +1952,22 @@ hipError_t ensure_coop(eu3d_ctx* c, long long n) {
     return e;
 }
 
+// Node spacings for cfl_bound: FIXED takes a one-cell direction out of the sum (spacing inf).
+static void cfl_spacing(const eu3d_ctx* c, double& dx, double& dy, double& dz) {
+    dx = c->dx0;
+    dy = c->dy0;
+    dz = c->dz0;
+    if (!(c->quirks & Q_F6)) {
+        if (c->g.ni == 1) dx = INFINITY;
+        if (c->g.nj == 1) dy = INFINITY;
+        if (c->g.nk == 1) dz = INFINITY;
+    }
+}
+
 hipError_t launch_cfl(eu3d_ctx* c, double cfl, double final_time) {
-    k_cfl<<<interior_grid(c->g), dim3(BX, BY), 0, c->stream>>>(G(c), c->fl, c->dx0, c->dy0, c->dz0, c->quirks,
-                                                              c->part);
+    double dx, dy, dz;
+    cfl_spacing(c, dx, dy, dz);
+    k_cfl<<<interior_grid(c->g), dim3(BX, BY), 0, c->stream>>>(G(c), c->fl, dx, dy, dz, c->quirks, c->part);
     k_cfl_finish<<<1, 1024, 0, c->stream>>>(c->sc, c->part, n_interior_blocks(c->g), cfl, final_time);
     return hipGetLastError();
 }
@@ -2006,8 +2025,10 @@ hipError_t launch_flux_fused(eu3d_ctx* c, int diff, int limiter) {
 }
 
 hipError_t launch_imex_fused(eu3d_ctx* c) {
-    k_imex_fused<<<interior_grid(c->g), dim3(BX, BY), 0, c->stream>>>(G(c), c->fl, c->quirks, c->sc, c->dx0, c->dy0,
-                                                                     c->dz0, c->part);
+    double dx, dy, dz;
+    cfl_spacing(c, dx, dy, dz);
+    k_imex_fused<<<interior_grid(c->g), dim3(BX, BY), 0, c->stream>>>(G(c), c->fl, c->quirks, c->sc, dx, dy, dz,
+                                                                     c->part);
     return hipGetLastError();
 }
 
@@ -2073,7 +2094,7 @@ hipError_t launch_check_finite(eu3d_ctx* c) {
 }
 
 hipError_t launch_trap_adaptive(eu3d_ctx* c, int nchem_max, bool predict_only, int nthr, double T_cold,
-                                unsigned* slist, unsigned* scount, bool adv) {
+                                unsigned* slist, unsigned* scount, bool adv, int nthr_hi) {
     const int nmax = nchem_max > 0 ? nchem_max : 10000;
     double* slot = &c->sc->dtm;
     hipError_t e = ensure_coop(c, (long long)c->g.ni * c->g.nj * c->g.nk);
@@ -2085,16 +2106,16 @@ hipError_t launch_trap_adaptive(eu3d_ctx* c, int nchem_max, bool predict_only, i
     const int po = predict_only ? 1 : 0;
     if (c3 && adv)
         k_trap_first<true, true><<<gr, bl, 0, c->stream>>>(G(c), c->fl, c->quirks, c->sc, c->sc, c->active, nmax, po,
-                                                           nthr, T_cold, slist, scount, rt);
+                                                           nthr, nthr_hi, T_cold, slist, scount, rt);
     else if (c3)
         k_trap_first<true, false><<<gr, bl, 0, c->stream>>>(G(c), c->fl, c->quirks, c->sc, c->sc, c->active, nmax, po,
-                                                            nthr, T_cold, slist, scount, rt);
+                                                            nthr, nthr_hi, T_cold, slist, scount, rt);
     else if (adv)
         k_trap_first<false, true><<<gr, bl, 0, c->stream>>>(G(c), c->fl, c->quirks, c->sc, c->sc, c->active, nmax, po,
-                                                            nthr, T_cold, slist, scount, rt);
+                                                            nthr, nthr_hi, T_cold, slist, scount, rt);
     else
         k_trap_first<false, false><<<gr, bl, 0, c->stream>>>(G(c), c->fl, c->quirks, c->sc, c->sc, c->active, nmax, po,
-                                                             nthr, T_cold, slist, scount, rt);
+                                                             nthr, nthr_hi, T_cold, slist, scount, rt);
     if (adv) k_adv_nochem<<<4 * c->num_cu, 256, 0, c->stream>>>(G(c), c->fl, c->quirks, c->sc);
     if (!predict_only && adv) {   // the step: the derived fields are refreshed after its last step
         GridCellsStep acc{c->g, c->fl};
@@ -2108,8 +2129,31 @@ hipError_t launch_trap_adaptive(eu3d_ctx* c, int nchem_max, bool predict_only, i
     return hipGetLastError();
 }
 
+// The step's prediction pass alone (k_trap_first with the fused Update_after_Adv and
+// Update_after_CS, the prediction and the first sub-step), for timing it: launch_chem_reset_step
+// is the per-call reset that precedes it in the step.
+hipError_t launch_chem_reset_step(eu3d_ctx* c) {
+    k_chem_reset<<<1, 1, 0, c->stream>>>(c->sc, &c->sc->dtm, 0.0, 1);
+    return hipGetLastError();
+}
+
+hipError_t launch_trap_first_step(eu3d_ctx* c, int nchem_max) {
+    hipError_t e = ensure_coop(c, (long long)c->g.ni * c->g.nj * c->g.nk);
+    if (e != hipSuccess) return e;
+    const TailRoute rt{c->coop_list, c->sched.tail};
+    const int nmax = nchem_max > 0 ? nchem_max : 10000;
+    const dim3 gr = interior_grid(c->g), bl(BX, BY);
+    if (c->quirks & Q_C3)
+        k_trap_first<true, true><<<gr, bl, 0, c->stream>>>(G(c), c->fl, c->quirks, c->sc, c->sc, c->active, nmax, 0, 0,
+                                                           0, 0.0, nullptr, nullptr, rt);
+    else
+        k_trap_first<false, true><<<gr, bl, 0, c->stream>>>(G(c), c->fl, c->quirks, c->sc, c->sc, c->active, nmax, 0, 0,
+                                                            0, 0.0, nullptr, nullptr, rt);
+    return hipGetLastError();
+}
+
 hipError_t launch_chem_batch(eu3d_ctx* c, long long n, double* rho, double* T, double* E, double dtm, int nchem_max,
-                             int* nchem_out, int nthr, double T_cold, unsigned* slist, unsigned* scount) {
+                             int* nchem_out, int nthr, double T_cold, unsigned* slist, unsigned* scount, int nthr_hi) {
     const int nmax = nchem_max > 0 ? nchem_max : 10000;
     if (c->batch_cap < n) {
         if (c->batch_active) (void)hipFree(c->batch_active);
@@ -2129,10 +2173,10 @@ hipError_t launch_chem_batch(eu3d_ctx* c, long long n, double* rho, double* T, d
     BatchCells acc{n, rho, T, E, keep};
     const unsigned gb = (unsigned)((n + 255) / 256);
     if (c->quirks & Q_C3) {
-        k_batch_first<true><<<gb, 256, 0, c->stream>>>(acc, nchem_out, keep, c->quirks, dtm, nmax, c->sc, list, nthr,
+        k_batch_first<true><<<gb, 256, 0, c->stream>>>(acc, nchem_out, keep, c->quirks, dtm, nmax, c->sc, list, nthr, nthr_hi,
                                                        T_cold, slist, scount, rt);
     } else {
-        k_batch_first<false><<<gb, 256, 0, c->stream>>>(acc, nchem_out, keep, c->quirks, dtm, nmax, c->sc, list, nthr,
+        k_batch_first<false><<<gb, 256, 0, c->stream>>>(acc, nchem_out, keep, c->quirks, dtm, nmax, c->sc, list, nthr, nthr_hi,
                                                         T_cold, slist, scount, rt);
     }
     launch_persist(c, acc, list, slot);

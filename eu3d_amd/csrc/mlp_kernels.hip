@@ -813,11 +813,13 @@ __device__ __forceinline__ int atoms(int e, int s) {
     return e == 0 ? H[s] : (e == 1 ? O[s] : N[s]);
 }
 
-// Orthogonal projection of a mass-fraction increment onto the increments that conserve the H, O
-// and N atoms: dY -= A^T (A A^T)^-1 A dY with A[e][s] = atoms(e, s) / Mw_s. The network's raw
-// increment conserves the elements only to its training error; the projected one conserves them
-// to rounding.
-__device__ __forceinline__ void project_elements(double (&dY)[NS]) {
+// Projection of a mass-fraction increment onto the increments that conserve the H, O and N
+// atoms, in the metric weighted by w: dY -= W A^T (A W A^T)^-1 A dY with A[e][s] = atoms(e, s) /
+// Mw_s and W = diag(w) (w = 1: the orthogonal projection). The network's raw increment conserves
+// the elements only to its training error; the projected one conserves them to rounding. With
+// w_s the species' abundance, each species is corrected in proportion to its own amount, so a
+// trace species is not driven negative by the correction of a major one.
+__device__ __forceinline__ void project_elements(double (&dY)[NS], const double (&w)[NS]) {
     double M[NEL][NEL], b[NEL];
 #pragma unroll
     for (int e = 0; e < NEL; e++) {
@@ -828,7 +830,7 @@ __device__ __forceinline__ void project_elements(double (&dY)[NS]) {
         for (int f = 0; f < NEL; f++) {
             double m = 0.0;
 #pragma unroll
-            for (int s = 0; s < NS; s++) m += (atoms(e, s) * RMW(s)) * (atoms(f, s) * RMW(s));
+            for (int s = 0; s < NS; s++) m += (atoms(e, s) * RMW(s)) * w[s] * (atoms(f, s) * RMW(s));
             M[e][f] = m;
         }
     }
@@ -848,7 +850,7 @@ __device__ __forceinline__ void project_elements(double (&dY)[NS]) {
         double corr = 0.0;
 #pragma unroll
         for (int e = 0; e < NEL; e++) corr += atoms(e, s) * RMW(s) * lam[e];
-        dY[s] -= corr;
+        dY[s] -= w[s] * corr;
     }
 }
 
@@ -911,7 +913,10 @@ __global__ void k_sur_targets(eu3d_surrogate_params sp, long long n, const doubl
     for (int s = 0; s < NOUT; s++) {
         float v = 0.f;
         if (s < NS) {
-            const double dy = rho1[(size_t)s * n + id] / D1 - rho0[(size_t)s * n + id] / D0;
+            const double y0 = rho0[(size_t)s * n + id] / D0, y1 = rho1[(size_t)s * n + id] / D1;
+            // (the FP64 chemistry leaves some mass fractions a rounding error below zero)
+            const double dy = sp.out_eps > 0 ? log(fmax(y1, 0.0) + sp.out_eps) - log(fmax(y0, 0.0) + sp.out_eps)
+                                             : y1 - y0;
             v = (float)((dy - sp.out_mean[s]) / sp.out_std[s]);
         }
         tgt[(size_t)id * NOUT + s] = v;
@@ -936,10 +941,23 @@ __device__ __forceinline__ int sur_apply(const eu3d_surrogate_params& sp, const 
     for (int s = 0; s < NS; s++) Y[s] = c.Di[s] / D;
     float f[NIN_REAL];
     if (!sur_features(sp, Y, c.T, D, dt, f)) return 1;
-    double dY[NS], Yn[NS], ymin = 1.0;
+    double dY[NS], Yn[NS], w[NS], ymin = 1.0;
+    if (sp.out_eps > 0) {
+        // log increments: the unprojected new mass fractions are above -eps by construction
 #pragma unroll
-    for (int s = 0; s < NS; s++) dY[s] = (double)out[s] * sp.out_std[s] + sp.out_mean[s];
-    project_elements(dY);
+        for (int s = 0; s < NS; s++) {
+            const double yn = (fmax(Y[s], 0.0) + sp.out_eps) * exp((double)out[s] * sp.out_std[s] + sp.out_mean[s]);
+            dY[s] = (yn - sp.out_eps) - Y[s];
+            w[s] = yn;
+        }
+    } else {
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+            dY[s] = (double)out[s] * sp.out_std[s] + sp.out_mean[s];
+            w[s] = 1.0;
+        }
+    }
+    project_elements(dY, w);
 #pragma unroll
     for (int s = 0; s < NS; s++) {
         Yn[s] = Y[s] + dY[s];
@@ -1423,6 +1441,7 @@ int eu3d_mlp_set_surrogate(eu3d_mlp* m, const eu3d_surrogate_params* p) {
         if (!(p->feat_std[i] > 0)) return EU3D_ERR_ARG;
     for (int i = 0; i < NOUT_REAL; i++)
         if (!(p->out_std[i] > 0)) return EU3D_ERR_ARG;
+    if (!(p->out_eps >= 0)) return EU3D_ERR_ARG;
     m->sp = *p;
     m->sp_set = true;
     return EU3D_OK;
@@ -1500,7 +1519,7 @@ int eu3d_surrogate_batch(eu3d_ctx* c, eu3d_mlp* m, long long n, double* rho, dou
         k_sur_apply_batch<<<nb, 256, 0, c->stream>>>(sp, n, rho, T, E, m->iout, dtm, c->quirks, b.cnt, b.rej);
     } else {   // stiff cells through the network, the rest through the FP64 sub-steps
         if (eu3d::launch_chem_batch(c, n, rho, T, E, dtm, nchem_max, nullptr, sp.nchem_threshold, sp.T_cold, b.net,
-                                    b.cnt + CNT_NET) != hipSuccess)
+                                    b.cnt + CNT_NET, sp.nchem_threshold_hi) != hipSuccess)
             return EU3D_ERR_HIP;
         const unsigned nl = std::min<unsigned>(nb, (unsigned)c->num_cu * 8);
         k_sur_feat_batch_list<<<nl, 256, 0, c->stream>>>(sp, n, rho, T, dtm, b.net, b.cnt + CNT_NET, m->ifeat);
@@ -1561,7 +1580,7 @@ hipError_t launch_surrogate_grid(eu3d_ctx* c, int nchem_max, bool adv) {
         k_sur_apply_grid<<<grid, blk, 0, c->stream>>>(sp, c->g, c->fl, c->sc, m->iout, c->quirks, b.cnt, b.rej);
     } else {   // stiff cells through the network, the rest through the FP64 sub-steps
         hipError_t e = launch_trap_adaptive(c, nchem_max, false, sp.nchem_threshold, sp.T_cold, b.net, b.cnt + CNT_NET,
-                                            adv);
+                                            adv, sp.nchem_threshold_hi);
         if (e != hipSuccess) return e;
         // the network cells' state after Update_after_CS, which the fused pass does not store
         if (adv && (e = launch_adv_list(c, b.net, b.cnt + CNT_NET, (unsigned)n)) != hipSuccess) return e;

@@ -148,13 +148,25 @@ def harvest_run(H: Harvest, wl, nsteps: int, every: int, cfl=0.5, ramp=(0.05, 20
     g.close()
 
 
+def increments(rho0, rho1, out_eps=0.0):
+    """Training outputs of harvested pairs [n x 9]: dY_k, or d ln(Y_k + eps) when eps > 0 (the
+    quantity the kernels' targets and apply step use)."""
+    torch = _torch()
+    Y0, Y1 = (rho0 / rho0.sum(0)).T, (rho1 / rho1.sum(0)).T
+    if out_eps > 0:   # the FP64 chemistry leaves some mass fractions a rounding error below zero
+        return torch.log(Y1.clamp(min=0) + out_eps) - torch.log(Y0.clamp(min=0) + out_eps)
+    return Y1 - Y0
+
+
 def fit_params(rho0, rho1, T0, dt, hull_margin=0.25, dT_max=40.0, elem_tol=2e-3, y_neg_tol=2e-4, T_cold=500.0,
-               nchem_threshold=0):
+               nchem_threshold=0, out_eps=0.0, nchem_threshold_hi=0):
     """Normalisation and training hull from harvested pairs. nchem_threshold > 0 sends only the
-    cells needing more FP64 sub-steps than that through the network."""
+    cells needing more FP64 sub-steps than that through the network (and, with
+    nchem_threshold_hi > 0, no more than that many). out_eps > 0 makes the
+    network's outputs log increments d ln(Y_k + out_eps), positive mass fractions by construction."""
     torch = _torch()
     f = raw_features(rho0, T0, dt)
-    dY = (rho1 / rho1.sum(0) - rho0 / rho0.sum(0)).T
+    dY = increments(rho0, rho1, out_eps)
     p = SurrogateParams()
     # the kernels normalise with the FP32 copies of these statistics: round first, so the hull
     # below is taken on the features exactly as the kernels compute them
@@ -171,6 +183,8 @@ def fit_params(rho0, rho1, T0, dt, hull_margin=0.25, dT_max=40.0, elem_tol=2e-3,
         p.out_mean[i], p.out_std[i] = float(om[i]), float(osd[i])
     p.dT_max, p.elem_tol, p.y_neg_tol, p.T_cold = dT_max, elem_tol, y_neg_tol, T_cold
     p.nchem_threshold = nchem_threshold
+    p.nchem_threshold_hi = nchem_threshold_hi
+    p.out_eps = out_eps
     return p
 
 
@@ -218,7 +232,7 @@ def train(mlp: MLP, feats, tg, epochs=20, batch=16384, lr=2e-3, lr_end=1e-4, wd=
 
 
 def predict_dY(mlp: MLP, p: SurrogateParams, feats):
-    """Network increments dY [n x 9] (FP64, de-normalised)."""
+    """Network outputs [n x 9] (FP64, de-normalised): dY_k, or d ln(Y_k + eps) in the log mode."""
     torch = _torch()
     out = torch.empty((feats.shape[0], MLP.NOUT), dtype=torch.float32, device="cuda")
     mlp.forward(feats, out)

@@ -229,7 +229,7 @@ int eu3d_chem_get_schedule(eu3d_ctx *ctx, eu3d_chem_schedule *s);
 
 /* ---- MLP chemistry surrogate (no reference counterpart: the reference's "DNN" scheme only
  * advects, src/Euler.cpp:773-780). 12 features (T, ln p/atm, 9 ln max(Y_k, 1e-20), ln dt),
- * 4 hidden layers of 256 with GELU, 9 outputs (normalised dY_k). BF16 weights and activations
+ * 4 hidden layers of 256 with GELU, 9 outputs (normalised dY_k or d ln(Y_k + eps)). BF16 weights and activations
  * on MFMA with FP32 accumulation; FP32 master weights and Adam. Device pointers throughout. */
 typedef struct eu3d_mlp eu3d_mlp;
 typedef struct {
@@ -243,6 +243,13 @@ typedef struct {
     int nchem_threshold;                 /* 0: every cell goes through the network; n > 0: only
                                             cells whose predicted sub-step count exceeds n, the
                                             others take the FP64 sub-steps */
+    double out_eps;                      /* 0: the outputs are normalised dY_k; eps > 0: they are
+                                            normalised d ln(max(Y_k, 0) + eps), so
+                                            Y_k' = (max(Y_k, 0) + eps) exp(.) - eps
+                                            stays above -eps, and the element projection is weighted
+                                            by each species' abundance Y_k' + eps */
+    int nchem_threshold_hi;              /* with nchem_threshold > 0: cells needing more than this
+                                            many sub-steps stay on the FP64 path too (0: no bound) */
 } eu3d_surrogate_params;
 typedef struct {
     long long cells, fallback, hull, negative, element, dT;
@@ -284,11 +291,15 @@ void *eu3d_stream(eu3d_ctx *ctx);
 /* Profiling hook (no reference counterpart): mean duration in ms of one kernel of the plain step,
  * timed with HIP events on the context stream around each of `reps` launches on identical
  * inputs. FLUX_Z / FLUX_XY are the two passes of eu3d_advection_fused; IMEX_FUSED is the fused
- * Diagonalized + Update_IMEX + Update_after_CS cell kernel of eu3d_step (its CS, T, E inputs
- * are restored between launches, outside the timed region). */
+ * Diagonalized + Update_IMEX + Update_after_CS cell kernel of the IMEX step and TRAP_FIRST the
+ * prediction pass of the adaptive-trapezoid step (for both, the CS, T, E inputs are restored
+ * between launches, outside the timed region). The timed kernel's other outputs stay in the
+ * context, so time a context only after its state is no longer needed. */
 #define EU3D_KERNEL_FLUX_Z 0
 #define EU3D_KERNEL_FLUX_XY 1
 #define EU3D_KERNEL_IMEX_FUSED 2
+#define EU3D_KERNEL_TRAP_FIRST 3    /* the adaptive-trapezoid step's prediction pass (Update_after_Adv,
+                                       Update_after_CS, TrapezoidPrediction, first sub-step) */
 int eu3d_time_kernel(eu3d_ctx *ctx, int kernel, int reps, int diff, int limiter, double *avg_ms);
 
 #ifdef __cplusplus

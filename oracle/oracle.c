@@ -375,17 +375,34 @@ void orc_advection(orc_flow *f, int diff_order, int limit) {
 }
 
 /* Flowfield::CFLcondition, src/Flowfield.cpp:242-272 (single rank: the MPI min is the identity).
- * Quirk F6: z is ignored; FIXED includes dz/(|w|+c). */
+ * Quirk F6 (REF): dt = cfl min(dx/(|u|+c), dy/(|v|+c)), z ignored. FIXED: the bound of the unsplit
+ * update in all three directions, dt = cfl / max sum_d (|u_d|+c)/dx_d, over the directions with
+ * more than one cell (a one-cell direction has no flux difference). The per-direction minimum lets
+ * the three-direction update run at an effective Courant number of up to 3 cfl, and the 3-D
+ * detonation then drives a cell's internal energy below the mixture's lowest tabulated energy. */
 void orc_cfl(orc_flow *f, double cfl, double final_time) {
     const int bc = f->bc;
     double dx = f->xnode[1] - f->xnode[0], dy = f->ynode[1] - f->ynode[0], dz = f->znode[1] - f->znode[0];
+    const int fixed = !(f->quirks & ORC_Q_F6);
+    if (fixed) {   /* a one-cell direction contributes (|u|+c)/inf = 0 */
+        if (f->ni == 1) dx = INFINITY;
+        if (f->nj == 1) dy = INFINITY;
+        if (f->nk == 1) dz = INFINITY;
+    }
     double dt_conv = 1e10;
     for (int i = bc; i < f->ni + bc; i++)
         for (int j = bc; j < f->nj + bc; j++)
             for (int k = bc; k < f->nk + bc; k++) {
                 const size_t c = I3(f, i, j, k);
-                double temp = dmin(dx / (fabs(f->U[c]) + f->C[c]), dy / (fabs(f->V[c]) + f->C[c]));
-                if (!(f->quirks & ORC_Q_F6)) temp = dmin(temp, dz / (fabs(f->W[c]) + f->C[c]));
+                double temp;
+                if (fixed) {
+                    const double rate = (fabs(f->U[c]) + f->C[c]) / dx + (fabs(f->V[c]) + f->C[c]) / dy +
+                                        (fabs(f->W[c]) + f->C[c]) / dz;
+                    temp = 1.0 / rate;
+                    if (!(temp < 1e10)) temp = 1e10;
+                } else {
+                    temp = dmin(dx / (fabs(f->U[c]) + f->C[c]), dy / (fabs(f->V[c]) + f->C[c]));
+                }
                 if (temp < dt_conv) dt_conv = temp;
             }
     f->dt = cfl * dt_conv;

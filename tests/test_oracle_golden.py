@@ -288,17 +288,26 @@ def test_periodic_domain_conserves_mass_and_energy():
         assert abs(R[..., c].sum()) < 1e-11 * np.abs(cs[..., c]).sum() + 1e-14
 
 
-def test_cfl_brute_force():
+@pytest.mark.parametrize("shape", [(6, 5, 4), (6, 1, 4), (1, 1, 7)])
+def test_cfl_brute_force(shape):
+    """REF: cfl min(dx/(|u|+c), dy/(|v|+c)) (z ignored, src/Flowfield.cpp:254). FIXED: the unsplit
+    three-direction bound cfl / max sum_d (|u_d|+c)/dx_d over the directions with more than one
+    cell, never above cfl times the per-direction minimum."""
     for q in (Q_FIXED, Q_REF):
-        o = flow(6, 5, 4, quirks=q)
+        o = flow(*shape, quirks=q)
         random_prims(o, 6)
         U, V, W, Cs = (o.get(k)[2:-2, 2:-2, 2:-2] for k in ("U", "V", "W", "C"))
         d = 1e-4
-        b = np.minimum(d / (np.abs(U) + Cs), d / (np.abs(V) + Cs))
         if q == Q_FIXED:
-            b = np.minimum(b, d / (np.abs(W) + Cs))
+            rate = sum((np.abs(X) + Cs) / d for X, n in zip((U, V, W), shape) if n > 1)
+            b = 1.0 / rate
+        else:
+            b = np.minimum(d / (np.abs(U) + Cs), d / (np.abs(V) + Cs))
         o.cfl(0.5, 1.0)
         assert o.get_time()[0] == pytest.approx(0.5 * b.min(), rel=1e-12)
+        if q == Q_FIXED:
+            per_dir = min((d / (np.abs(X) + Cs)).min() for X, n in zip((U, V, W), shape) if n > 1)
+            assert o.get_time()[0] <= 0.5 * per_dir * (1 + 1e-12)
 
 
 # ------------------------------------------------------------------ kinetics
