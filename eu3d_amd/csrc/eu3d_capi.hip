@@ -377,7 +377,7 @@ int eu3d_step(eu3d_ctx* c, const eu3d_step_params* p, int nsteps) {
     if (p->chem == EU3D_CHEM_SURROGATE && !c->mlp) return fail(c, EU3D_ERR_ARG, "no surrogate attached");
     const bool adaptive = p->chem == EU3D_CHEM_TRAP_ADAPTIVE || p->chem == EU3D_CHEM_SURROGATE;
     if (adaptive && nsteps > 0) HIPCHK(c, launch_clear_nchem_flag(c), "step flag");
-    for (int n = 0; n < nsteps; n++) HIPCHK(c, launch_step(c, p, n == 0), "step");
+    for (int n = 0; n < nsteps; n++) HIPCHK(c, launch_step(c, p, n == 0, n == nsteps - 1), "step");
     if (p->chem == EU3D_CHEM_TRAP_ADAPTIVE) c->stats.cells = (long long)c->g.ni * c->g.nj * c->g.nk;
     if (p->chem == EU3D_CHEM_SURROGATE && nsteps > 0) HIPCHK(c, surrogate_grid_stats(c), "surrogate stats");
     if (nsteps > 0) {

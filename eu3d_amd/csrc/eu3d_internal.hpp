@@ -102,15 +102,18 @@ hipError_t launch_trapezoid(eu3d_ctx* c);
 // nthr > 0: cells above T_cold needing more than nthr sub-steps keep their state and are listed
 // in slist / scount for the surrogate network instead of taking the FP64 sub-steps. adv: run the
 // step's Update_after_Adv (CS += RHS) and Update_after_CS inside the prediction pass.
+// step_src: store the chemical source and every cell's sub-step count (the last step of an
+// eu3d_step call); step_cfl: the passes leave the next step's CFL partial minima in ctx->part.
 hipError_t launch_trap_adaptive(eu3d_ctx* c, int nchem_max, bool predict_only, int nthr = 0, double T_cold = 0.0,
                                 unsigned* slist = nullptr, unsigned* scount = nullptr, bool adv = false,
-                                int nthr_hi = 0);
+                                int nthr_hi = 0, bool step_src = true, bool step_cfl = false);
 hipError_t launch_check_finite(eu3d_ctx* c);
 hipError_t launch_chem_reset_step(eu3d_ctx* c);
 hipError_t launch_trap_first_step(eu3d_ctx* c, int nchem_max);
 hipError_t ensure_coop(eu3d_ctx* c, long long n);
 hipError_t launch_set_scalars(eu3d_ctx* c, double dt, double time);
-hipError_t launch_step(eu3d_ctx* c, const eu3d_step_params* p, bool first);
+// first / last: the step is the first / the last of its eu3d_step call
+hipError_t launch_step(eu3d_ctx* c, const eu3d_step_params* p, bool first, bool last);
 // adv: the step's Update_after_Adv + Update_after_CS have not run yet (fused into the passes)
 hipError_t launch_surrogate_grid(eu3d_ctx* c, int nchem_max, bool adv = false);
 // Update_after_Adv + Update_after_CS of the listed cells (count on the device)

@@ -1066,6 +1066,10 @@ struct GridCells {
 struct GridCellsStep {
     G3 g;
     Fields F;
+    bool src = true;           // store the chemical source (CMS): on the last step of an eu3d_step call
+    double* cfl = nullptr;     // fold each stored cell's CFL bound into the next step's block partials
+    double cdx = 0, cdy = 0, cdz = 0;
+    uint32_t q = 0;
     __device__ void load(unsigned id, ChemCell& c, int& n) const {
         const size_t cc = id;
 #pragma unroll
@@ -1078,12 +1082,24 @@ struct GridCellsStep {
         n = (int)FLD(F, EU3D_FIELD_NCHEM)[cc];
     }
     __device__ void store_src(unsigned id, const KinOut& k) const {
+        if (!src) return;
 #pragma unroll
         for (int s = 0; s < NS; s++) FLD(F, EU3D_FIELD_CMS)[s * g.n3 + (size_t)id] = k.w[s] * 1e3;
     }
     __device__ void store(unsigned id, const double (&cs)[NC], const Prim& o, const KinOut& k) const {
         store_state(id, cs, o);
         store_src(id, k);
+    }
+    // the cell's CFL bound into the partial of the interior block that owns it (the layout
+    // block_min_store writes: 64 x 4 blocks over (i, j), one grid layer per k); the bounds are
+    // positive, so their bit patterns order as unsigned integers
+    __device__ void cfl_min(unsigned id, const Prim& o) const {
+        const size_t cc = id, plane = (size_t)g.NX * g.NY;
+        const int k = (int)(cc / plane), j = (int)((cc / g.NX) % g.NY), i = (int)(cc % g.NX);
+        const int gx = (g.ni + BX - 1) / BX, gy = (g.nj + BY - 1) / BY;
+        const size_t b = ((size_t)(k - g.bc) * gy + (j - g.bc) / BY) * gx + (i - g.bc) / BX;
+        const double v = cfl_bound(cdx, cdy, cdz, o.U, o.V, o.W, o.C, q);
+        atomicMin(reinterpret_cast<unsigned long long*>(cfl + b), (unsigned long long)__double_as_longlong(v));
     }
     __device__ void store_pre(unsigned id, const double (&cs)[NC], const Prim& o) const {
         const size_t cc = id;
@@ -1105,6 +1121,7 @@ struct GridCellsStep {
         FLD(F, EU3D_FIELD_H)[cc] = o.H;
         FLD(F, EU3D_FIELD_GAMMA)[cc] = o.Gamma;
         FLD(F, EU3D_FIELD_C)[cc] = o.C;
+        if (cfl) cfl_min(id, o);
     }
     __device__ void store_state(unsigned id, const double (&cs)[NC], const Prim& o) const {
         store_pre(id, cs, o);
@@ -1166,6 +1183,16 @@ __device__ __forceinline__ void chem_count(Scalars* sc, int nch, bool active, un
     if (active) list[base + __popcll(am & ((1ull << lane) - 1ull))] = id;
 }
 
+// What the step's chemistry passes store beyond the state the next step reads. src: the chemical
+// source and the sub-step count of every cell (the last step of an eu3d_step call; on the other
+// steps only the cells that continue, or whose prediction failed, keep their count). part: the
+// next step's per-block CFL partial minima, from each cell's final state (null: none).
+struct StepOut {
+    int src = 1;
+    double* part = nullptr;
+    double dx = 0, dy = 0, dz = 0;
+};
+
 // Where a cell with more than one sub-step continues: the persistent kernel's list, or, above the
 // tail threshold, the list of the one-cell-per-wavefront kernel.
 struct TailRoute {
@@ -1206,12 +1233,13 @@ template <bool C3, bool ADV>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ADV ? 2 : 1, ADV ? 2 : 1)))
 k_trap_first(G3 g, Fields F, uint32_t q, const Scalars* __restrict__ scc, Scalars* sc, unsigned* list, int nmax,
              int predict_only, int nthr, int nthr_hi, double T_cold, unsigned* slist, unsigned* scount,
-             TailRoute rt) {
+             TailRoute rt, StepOut so) {
     int i, j, k;
     const bool in = interior_ijk(g, i, j, k);
     const size_t c = in ? IDX(g, i, j, k) : 0;
     int nch = 0;
     bool active = false, to_net = false;
+    double cflv = 1e10;   // this cell's CFL bound when its state is final here
     if (in) {
         ChemCell cell;
         double Mc[NS];
@@ -1269,22 +1297,27 @@ k_trap_first(G3 g, Fields F, uint32_t q, const Scalars* __restrict__ scc, Scalar
         for (int s = 0; s < NS; s++) Dr[s] = cell.Di[s] * 1e-3;
         const double dtm = scc->dt;
         nch = chem_nchem(ko, Dr, Y, dtm, nmax);
-        FLD(F, EU3D_FIELD_NCHEM)[c] = (double)nch;
+        if (so.src || nch != 1) FLD(F, EU3D_FIELD_NCHEM)[c] = (double)nch;
         if (nch <= 0) atomicOr(&sc->nonfinite, 2);
         to_net = nthr > 0 && nch > nthr && (nthr_hi <= 0 || nch <= nthr_hi) && cell.T >= T_cold;
         if (!predict_only && nch > 0 && !to_net) {
             using Acc = typename std::conditional<ADV, GridCellsStep, GridCells>::type;
-            const Acc acc{g, F};
+            Acc acc{g, F};
+            if constexpr (ADV) acc.src = so.src != 0;
             acc.store_src((unsigned)c, ko);
             double cs[NC];
             const Prim o = chem_apply(cell, ko, Dr, dtm / nch, q, cs, tp,
                                       [&](const Prim& o1) { acc.store_pre((unsigned)c, cs, o1); });
             acc.store_post((unsigned)c, o);
             active = nch > 1;
+            if (!active) cflv = cfl_bound(so.dx, so.dy, so.dz, o.U, o.V, o.W, o.C, q);
         }
     }
     chem_count_tail(sc, in && !to_net ? nch : 0, active, list, (unsigned)c, rt);
     if (nthr > 0) wave_append(slist, scount, to_net, (unsigned)c);
+    // the next step's CFL partial of this block from its finished cells; the cells that continue
+    // (and those whose prediction failed) fold theirs in where their final state is stored
+    if (so.part) block_min_store(cflv, so.part);
 }
 
 // Update_after_Adv (CS += RHS; REF quirk F1 adds the species only) and Update_after_CS of one
@@ -1305,7 +1338,8 @@ __device__ __forceinline__ void adv_update_cell(const G3& g, const Fields& F, ui
 // After the fused prediction pass: the cells whose prediction failed (Nchem <= 0). The pass sets
 // bit 2 of sc->nonfinite for any such cell; without it there is nothing to do, so the kernel is
 // launched on a small grid that scans the interior only when the bit is set.
-__global__ void __launch_bounds__(256) k_adv_nochem(G3 g, Fields F, uint32_t q, const Scalars* __restrict__ sc) {
+__global__ void __launch_bounds__(256) k_adv_nochem(G3 g, Fields F, uint32_t q, const Scalars* __restrict__ sc,
+                                                    StepOut so) {
     if (!(sc->nonfinite & 2)) return;
     const size_t n = (size_t)g.ni * g.nj * g.nk;
     for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
@@ -1314,6 +1348,20 @@ __global__ void __launch_bounds__(256) k_adv_nochem(G3 g, Fields F, uint32_t q, 
         const size_t c = IDX(g, i, j, k);
         if (FLD(F, EU3D_FIELD_NCHEM)[c] > 0) continue;
         adv_update_cell(g, F, q, c);
+        if (so.part) {
+            GridCellsStep acc{g, F};
+            acc.cfl = so.part;
+            acc.cdx = so.dx;
+            acc.cdy = so.dy;
+            acc.cdz = so.dz;
+            acc.q = q;
+            Prim o;
+            o.U = FLD(F, EU3D_FIELD_U)[c];
+            o.V = FLD(F, EU3D_FIELD_V)[c];
+            o.W = FLD(F, EU3D_FIELD_W)[c];
+            o.C = FLD(F, EU3D_FIELD_C)[c];
+            acc.cfl_min((unsigned)c, o);
+        }
     }
 }
 
@@ -2094,8 +2142,15 @@ hipError_t launch_check_finite(eu3d_ctx* c) {
 }
 
 hipError_t launch_trap_adaptive(eu3d_ctx* c, int nchem_max, bool predict_only, int nthr, double T_cold,
-                                unsigned* slist, unsigned* scount, bool adv, int nthr_hi) {
+                                unsigned* slist, unsigned* scount, bool adv, int nthr_hi, bool step_src,
+                                bool step_cfl) {
     const int nmax = nchem_max > 0 ? nchem_max : 10000;
+    StepOut so;
+    so.src = step_src ? 1 : 0;
+    if (step_cfl) {
+        so.part = c->part;
+        cfl_spacing(c, so.dx, so.dy, so.dz);
+    }
     double* slot = &c->sc->dtm;
     hipError_t e = ensure_coop(c, (long long)c->g.ni * c->g.nj * c->g.nk);
     if (e != hipSuccess) return e;
@@ -2106,19 +2161,25 @@ hipError_t launch_trap_adaptive(eu3d_ctx* c, int nchem_max, bool predict_only, i
     const int po = predict_only ? 1 : 0;
     if (c3 && adv)
         k_trap_first<true, true><<<gr, bl, 0, c->stream>>>(G(c), c->fl, c->quirks, c->sc, c->sc, c->active, nmax, po,
-                                                           nthr, nthr_hi, T_cold, slist, scount, rt);
+                                                           nthr, nthr_hi, T_cold, slist, scount, rt, so);
     else if (c3)
         k_trap_first<true, false><<<gr, bl, 0, c->stream>>>(G(c), c->fl, c->quirks, c->sc, c->sc, c->active, nmax, po,
-                                                            nthr, nthr_hi, T_cold, slist, scount, rt);
+                                                            nthr, nthr_hi, T_cold, slist, scount, rt, so);
     else if (adv)
         k_trap_first<false, true><<<gr, bl, 0, c->stream>>>(G(c), c->fl, c->quirks, c->sc, c->sc, c->active, nmax, po,
-                                                            nthr, nthr_hi, T_cold, slist, scount, rt);
+                                                            nthr, nthr_hi, T_cold, slist, scount, rt, so);
     else
         k_trap_first<false, false><<<gr, bl, 0, c->stream>>>(G(c), c->fl, c->quirks, c->sc, c->sc, c->active, nmax, po,
-                                                             nthr, nthr_hi, T_cold, slist, scount, rt);
-    if (adv) k_adv_nochem<<<4 * c->num_cu, 256, 0, c->stream>>>(G(c), c->fl, c->quirks, c->sc);
+                                                             nthr, nthr_hi, T_cold, slist, scount, rt, so);
+    if (adv) k_adv_nochem<<<4 * c->num_cu, 256, 0, c->stream>>>(G(c), c->fl, c->quirks, c->sc, so);
     if (!predict_only && adv) {   // the step: the derived fields are refreshed after its last step
         GridCellsStep acc{c->g, c->fl};
+        acc.src = step_src;
+        acc.cfl = so.part;
+        acc.cdx = so.dx;
+        acc.cdy = so.dy;
+        acc.cdz = so.dz;
+        acc.q = c->quirks;
         launch_persist(c, acc, c->active, slot);
         launch_coop(c, acc, slot);
     } else if (!predict_only) {
@@ -2143,12 +2204,16 @@ hipError_t launch_trap_first_step(eu3d_ctx* c, int nchem_max) {
     const TailRoute rt{c->coop_list, c->sched.tail};
     const int nmax = nchem_max > 0 ? nchem_max : 10000;
     const dim3 gr = interior_grid(c->g), bl(BX, BY);
+    StepOut so;   // as on a step that is not the last of its call
+    so.src = 0;
+    so.part = c->part;
+    cfl_spacing(c, so.dx, so.dy, so.dz);
     if (c->quirks & Q_C3)
         k_trap_first<true, true><<<gr, bl, 0, c->stream>>>(G(c), c->fl, c->quirks, c->sc, c->sc, c->active, nmax, 0, 0,
-                                                           0, 0.0, nullptr, nullptr, rt);
+                                                           0, 0.0, nullptr, nullptr, rt, so);
     else
         k_trap_first<false, true><<<gr, bl, 0, c->stream>>>(G(c), c->fl, c->quirks, c->sc, c->sc, c->active, nmax, 0, 0,
-                                                            0, 0.0, nullptr, nullptr, rt);
+                                                            0, 0.0, nullptr, nullptr, rt, so);
     return hipGetLastError();
 }
 
@@ -2232,12 +2297,15 @@ hipError_t launch_batch_list(eu3d_ctx* c, long long n, double* rho, double* T, d
     return hipGetLastError();
 }
 
-hipError_t launch_step(eu3d_ctx* c, const eu3d_step_params* p, bool first) {
+hipError_t launch_step(eu3d_ctx* c, const eu3d_step_params* p, bool first, bool last) {
     hipError_t e;
     const bool imex = p->rk == 1 && p->chem == EU3D_CHEM_IMEX;
+    const bool trap = p->rk == 1 && p->chem == EU3D_CHEM_TRAP_ADAPTIVE;
     if (p->cfl > 0) {
-        // the fused IMEX epilogue leaves the CFL partials of the new state behind
-        e = (imex && !first) ? launch_cfl_finish(c, p->cfl, p->final_time) : launch_cfl(c, p->cfl, p->final_time);
+        // the fused IMEX epilogue and the adaptive-trapezoid passes leave the CFL partials of the
+        // new state behind for the next step of the same call
+        e = ((imex || trap) && !first) ? launch_cfl_finish(c, p->cfl, p->final_time)
+                                       : launch_cfl(c, p->cfl, p->final_time);
         if (e != hipSuccess) return e;
     } else {
         k_advance_time<<<1, 1, 0, c->stream>>>(c->sc);
@@ -2272,7 +2340,7 @@ hipError_t launch_step(eu3d_ctx* c, const eu3d_step_params* p, bool first) {
         e = launch_explicit(c);
         break;
     case EU3D_CHEM_TRAP_ADAPTIVE:   // Update_after_Adv fused into the prediction pass
-        e = launch_trap_adaptive(c, p->nchem_max, false, 0, 0.0, nullptr, nullptr, true);
+        e = launch_trap_adaptive(c, p->nchem_max, false, 0, 0.0, nullptr, nullptr, true, 0, last, !last);
         break;
     case EU3D_CHEM_SURROGATE: e = launch_surrogate_grid(c, p->nchem_max, true); break;
     default: e = launch_update_after_adv(c);
