@@ -5,6 +5,12 @@ import pytest
 from tests.common import say
 
 pytestmark = pytest.mark.gpu
+# tolerances of the network against the FP64 restatement (oracle/mlp.py). Measured on one MI355X:
+# gradients 2.2e-3 .. 4.9e-3 relative L2 per tensor; forward 1.7e-3 of scale against the
+# BF16-rounded FP64 network and 5.4e-3 against plain FP64. A 1 % scale error in one layer's
+# gradient exceeds GRAD_REL_TOL.
+GRAD_REL_TOL = 8e-3
+FWD_TOL_BF16, FWD_TOL_FP64 = 5e-3, 1.5e-2
 
 
 def _t():
@@ -66,8 +72,9 @@ def test_forward_matches_fp64():
     Yb, _ = om.forward(p, X, bf16=True)
     Yf, _ = om.forward(p, X, bf16=False)
     sc = np.abs(Yf[:, :9]).max()
-    assert np.abs(Y[:, :9] - Yb[:, :9]).max() <= 2e-2 * sc
-    assert np.abs(Y[:, :9] - Yf[:, :9]).max() <= 6e-2 * sc
+    eb, ef = np.abs(Y[:, :9] - Yb[:, :9]).max() / sc, np.abs(Y[:, :9] - Yf[:, :9]).max() / sc
+    say(f"  forward max error / scale: vs BF16-rounded FP64 {eb:.2e}, vs FP64 {ef:.2e}")
+    assert eb <= FWD_TOL_BF16 and ef <= FWD_TOL_FP64
     assert np.abs(Y[:, 9:]).max() == 0.0   # padding outputs stay zero
 
 
@@ -90,13 +97,18 @@ def test_gradients_match_fp64(n):
     lref, gref = om.loss_and_grads(p, bf16_vals(X), T)
     assert abs(loss - lref) <= 2e-2 * lref
     off = 0
+    worst = []
     for din, dout in zip(om.DIN, om.DOUT):
         for sz in (din * dout, dout):
             a, b = g[off:off + sz], gref[off:off + sz]
             off += sz
             cos = a @ b / (np.linalg.norm(a) * np.linalg.norm(b) + 1e-30)
             rel = np.linalg.norm(a - b) / (np.linalg.norm(b) + 1e-30)
-            assert cos > 0.995 and rel < 0.1, (din, dout, sz, cos, rel)
+            worst.append((rel, cos, din, dout, sz))
+    say("  gradient relative L2 error per tensor:", [f"{w[0]:.2e}" for w in worst])
+    rel, cos, din, dout, sz = max(worst)
+    # the BF16 rounding of the activations and of dZ between layers bounds the agreement
+    assert rel < GRAD_REL_TOL and cos > 1 - GRAD_REL_TOL ** 2, (din, dout, sz, cos, rel)
 
 
 def test_training_reduces_loss():
