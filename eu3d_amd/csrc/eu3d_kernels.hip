@@ -505,8 +505,22 @@ __global__ void __launch_bounds__(256) k_cfl(G3 g, Fields F, double dx, double d
 
 __global__ void __launch_bounds__(1024) k_cfl_finish(Scalars* sc, const double* __restrict__ part, int n, double cfl,
                                                      double final_time) {
-    double v = 1e10;
-    for (int t = threadIdx.x; t < n; t += blockDim.x) v = (part[t] < v) ? part[t] : v;
+    // eight independent loads in flight per thread (one block reads every partial)
+    double v = 1e10, u[8];
+#pragma unroll
+    for (int m = 0; m < 8; m++) u[m] = 1e10;
+    const int bd = blockDim.x;
+    int t = threadIdx.x;
+    for (; t + 7 * bd < n; t += 8 * bd) {
+#pragma unroll
+        for (int m = 0; m < 8; m++) {
+            const double a = part[t + m * bd];
+            u[m] = (a < u[m]) ? a : u[m];
+        }
+    }
+    for (; t < n; t += bd) v = (part[t] < v) ? part[t] : v;
+#pragma unroll
+    for (int m = 0; m < 8; m++) v = (u[m] < v) ? u[m] : v;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         const double o = __shfl_xor(v, off);
@@ -1384,6 +1398,9 @@ template <class Acc, bool C3, int W>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W)))
 k_chem_persist(Acc acc, const unsigned* __restrict__ list, Scalars* sc, uint32_t q, const double* __restrict__ dtm_p) {
     const unsigned long long n_active = sc->chem_active;
+    // nothing queued (the usual case of a grid step): leave before every wave's atomic on the
+    // queue head, which cost about 50 us over a whole-GPU grid
+    if (n_active == 0) return;
     const double dtm = *dtm_p;
     const int lane = __lane_id();
     ChemCell cell;
@@ -1522,6 +1539,7 @@ __device__ __forceinline__ double tail_newton(const TailLds& S, const TPow& p0, 
 template <class Acc, bool C3>
 __global__ void __launch_bounds__(64) k_chem_tail(Acc acc, const unsigned* __restrict__ list, Scalars* sc, uint32_t q,
                                                   const double* __restrict__ dtm_p) {
+    if (sc->coop_count == 0) return;   // nothing queued: no set-up, no queue atomics
     __shared__ TailLds L[2];
     const int lane = threadIdx.x, hl = lane & 31;
     TailLds& S = L[lane >> 5];
