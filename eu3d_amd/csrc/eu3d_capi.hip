@@ -140,6 +140,13 @@ int eu3d_create(eu3d_ctx** out, const eu3d_grid* grid, const eu3d_mech* mech, ui
     // a blocking stream: work the caller issued to the null stream (where it produced the device
     // buffers it hands over) completes before the context's kernels read them
     if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamDefault)) != hipSuccess) { delete c; return EU3D_ERR_HIP; }
+    // the side stream of the chemistry tail kernel (ordered by events from the context stream)
+    if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
+        eu3d_destroy(c);
+        return EU3D_ERR_HIP;
+    }
     for (int f = 0; f < EU3D_FIELD_COUNT; f++) {
         const size_t bytes = g.n3 * ncomp(f) * sizeof(double);
         if ((e = hipMalloc(&c->fl.f[f], bytes)) != hipSuccess) { eu3d_destroy(c); return EU3D_ERR_HIP; }
@@ -194,6 +201,10 @@ int eu3d_destroy(eu3d_ctx* c) {
     if (c->coop_list) (void)hipFree(c->coop_list);
     if (c->batch_active) (void)hipFree(c->batch_active);
     if (c->sur_list) (void)hipFree(c->sur_list);
+    if (c->side) (void)hipStreamSynchronize(c->side);
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    if (c->side) (void)hipStreamDestroy(c->side);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return EU3D_OK;

@@ -54,6 +54,8 @@ struct eu3d_ctx {
     int device = 0;
     int num_cu = 256;
     hipStream_t stream = nullptr;
+    hipStream_t side = nullptr;                                    // the tail kernel's stream (default priority)
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;               // context stream -> side -> context stream
     eu3d::Fields fl{};
     double *xnode = nullptr, *ynode = nullptr, *znode = nullptr;   // device, ghost-inclusive
     double dx0 = 0, dy0 = 0, dz0 = 0;                              // node(1) - node(0)

@@ -2008,6 +2008,25 @@ void launch_coop(eu3d_ctx* c, const Acc& acc, const double* dtm_slot) {
     else k_chem_tail<Acc, false><<<nb, 64, 0, c->stream>>>(acc, c->coop_list, c->sc, c->quirks, dtm_slot);
 }
 
+// The rest of a chemistry call after its first pass: the persistent kernel on the context stream
+// and, when cells were routed to it, the tail kernel. The two take disjoint cells, so with a side
+// stream the tail kernel (the longest sub-step chains, a few dozen wavefronts) runs beside the
+// persistent kernel instead of after it, and the context stream waits for both.
+template <class Acc>
+void launch_rest(eu3d_ctx* c, const Acc& acc, const unsigned* list, const double* dtm_slot) {
+    if (c->side && c->sched.tail > 0 && hipEventRecord(c->ev_fork, c->stream) == hipSuccess &&
+        hipStreamWaitEvent(c->side, c->ev_fork, 0) == hipSuccess) {
+        const unsigned nb = (unsigned)c->num_cu * 16;
+        if (c->quirks & Q_C3) k_chem_tail<Acc, true><<<nb, 64, 0, c->side>>>(acc, c->coop_list, c->sc, c->quirks, dtm_slot);
+        else k_chem_tail<Acc, false><<<nb, 64, 0, c->side>>>(acc, c->coop_list, c->sc, c->quirks, dtm_slot);
+        launch_persist(c, acc, list, dtm_slot);
+        if (hipEventRecord(c->ev_join, c->side) == hipSuccess) (void)hipStreamWaitEvent(c->stream, c->ev_join, 0);
+        return;
+    }
+    launch_persist(c, acc, list, dtm_slot);
+    launch_coop(c, acc, dtm_slot);
+}
+
 hipError_t ensure_coop(eu3d_ctx* c, long long n) {
     if (c->coop_cap >= n) return hipSuccess;
     if (c->coop_list) (void)hipFree(c->coop_list);
@@ -2198,12 +2217,10 @@ hipError_t launch_trap_adaptive(eu3d_ctx* c, int nchem_max, bool predict_only, i
         acc.cdy = so.dy;
         acc.cdz = so.dz;
         acc.q = c->quirks;
-        launch_persist(c, acc, c->active, slot);
-        launch_coop(c, acc, slot);
+        launch_rest(c, acc, c->active, slot);
     } else if (!predict_only) {
         GridCells acc{c->g, c->fl};
-        launch_persist(c, acc, c->active, slot);
-        launch_coop(c, acc, slot);
+        launch_rest(c, acc, c->active, slot);
     }
     return hipGetLastError();
 }
@@ -2262,8 +2279,7 @@ hipError_t launch_chem_batch(eu3d_ctx* c, long long n, double* rho, double* T, d
         k_batch_first<false><<<gb, 256, 0, c->stream>>>(acc, nchem_out, keep, c->quirks, dtm, nmax, c->sc, list, nthr, nthr_hi,
                                                         T_cold, slist, scount, rt);
     }
-    launch_persist(c, acc, list, slot);
-    launch_coop(c, acc, slot);
+    launch_rest(c, acc, list, slot);
     return hipGetLastError();
 }
 
@@ -2285,8 +2301,7 @@ hipError_t launch_trap_list(eu3d_ctx* c, const unsigned* list, const unsigned* c
             k_trap_first_list<false><<<nb, 256, 0, c->stream>>>(G(c), c->fl, c->quirks, list, count, c->sc, work, nmax, rt);
     }
     GridCells acc{c->g, c->fl};
-    launch_persist(c, acc, work, &c->sc->dtm);
-    launch_coop(c, acc, &c->sc->dtm);
+    launch_rest(c, acc, work, &c->sc->dtm);
     return hipGetLastError();
 }
 
@@ -2310,8 +2325,7 @@ hipError_t launch_batch_list(eu3d_ctx* c, long long n, double* rho, double* T, d
             k_batch_first_list<false><<<nb, 256, 0, c->stream>>>(acc, nchem_keep, c->quirks, dtm, nmax, list, count, c->sc,
                                                                   work, rt);
     }
-    launch_persist(c, acc, work, &c->sc->dtm);
-    launch_coop(c, acc, &c->sc->dtm);
+    launch_rest(c, acc, work, &c->sc->dtm);
     return hipGetLastError();
 }
 
