@@ -62,6 +62,7 @@ struct eu3d_ctx {
     eu3d::Scalars* sc = nullptr;                                  // device
     unsigned int* active = nullptr;                                // compaction list (interior cells)
     double* part = nullptr;                                        // per-block CFL partial minima
+    bool part_ready = false;   // the last step's passes left the CFL partial minima of its state in part
     double* rk_save = nullptr;                                     // RK2 copy of CS, T, E
     // batched chemistry scratch
     unsigned int* batch_active = nullptr;
@@ -116,8 +117,13 @@ hipError_t ensure_coop(eu3d_ctx* c, long long n);
 hipError_t launch_set_scalars(eu3d_ctx* c, double dt, double time);
 // first / last: the step is the first / the last of its eu3d_step call
 hipError_t launch_step(eu3d_ctx* c, const eu3d_step_params* p, bool first, bool last);
-// adv: the step's Update_after_Adv + Update_after_CS have not run yet (fused into the passes)
-hipError_t launch_surrogate_grid(eu3d_ctx* c, int nchem_max, bool adv = false);
+// adv: the step's Update_after_Adv + Update_after_CS have not run yet (fused into the passes);
+// last: the step is the last of its eu3d_step call (with adv: the chemical source is stored on
+// that step only, and the steps before it leave the next step's CFL partial minima)
+hipError_t launch_surrogate_grid(eu3d_ctx* c, int nchem_max, bool adv = false, bool last = true);
+// The CFL bounds of the listed cells' final state (U, V, W, C in the fields; count on the device)
+// folded into the block partial minima of ctx->part.
+hipError_t launch_cfl_list(eu3d_ctx* c, const unsigned* list, const unsigned* count, unsigned max_count);
 // Update_after_Adv + Update_after_CS of the listed cells (count on the device)
 hipError_t launch_adv_list(eu3d_ctx* c, const unsigned* list, const unsigned* count, unsigned max_count);
 hipError_t surrogate_grid_stats(eu3d_ctx* c);

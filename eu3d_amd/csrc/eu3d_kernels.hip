@@ -1379,6 +1379,29 @@ __global__ void __launch_bounds__(256) k_adv_nochem(G3 g, Fields F, uint32_t q, 
     }
 }
 
+// After a surrogate step: the CFL bounds of the cells that went to the network (accepted, or
+// redone on FP64 after a rejection), whose final state neither the prediction pass nor the FP64
+// queue kernels saw, into the next step's block partials. Grid-stride over the list.
+__global__ void __launch_bounds__(256) k_cfl_list(G3 g, Fields F, uint32_t q, StepOut so,
+                                                  const unsigned* __restrict__ list, const unsigned* __restrict__ count) {
+    const unsigned n = *count;
+    GridCellsStep acc{g, F};
+    acc.cfl = so.part;
+    acc.cdx = so.dx;
+    acc.cdy = so.dy;
+    acc.cdz = so.dz;
+    acc.q = q;
+    for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+        const size_t c = list[t];
+        Prim o;
+        o.U = FLD(F, EU3D_FIELD_U)[c];
+        o.V = FLD(F, EU3D_FIELD_V)[c];
+        o.W = FLD(F, EU3D_FIELD_W)[c];
+        o.C = FLD(F, EU3D_FIELD_C)[c];
+        acc.cfl_min((unsigned)c, o);
+    }
+}
+
 // After the fused prediction pass with surrogate routing: the cells listed for the network
 // (count on the device), grid-stride.
 __global__ void __launch_bounds__(256) k_adv_list(G3 g, Fields F, uint32_t q, const unsigned* __restrict__ list,
@@ -2143,6 +2166,15 @@ hipError_t launch_adv_list(eu3d_ctx* c, const unsigned* list, const unsigned* co
     return hipGetLastError();
 }
 
+hipError_t launch_cfl_list(eu3d_ctx* c, const unsigned* list, const unsigned* count, unsigned max_count) {
+    StepOut so;
+    so.part = c->part;
+    cfl_spacing(c, so.dx, so.dy, so.dz);
+    const unsigned nb = std::min<unsigned>((max_count + 255) / 256, (unsigned)c->num_cu * 8);
+    if (nb) k_cfl_list<<<nb, 256, 0, c->stream>>>(G(c), c->fl, c->quirks, so, list, count);
+    return hipGetLastError();
+}
+
 hipError_t launch_update_after_adv(eu3d_ctx* c) {
     k_add_rhs<<<interior_grid(c->g), dim3(BX, BY), 0, c->stream>>>(G(c), c->fl, (c->quirks & Q_F1) ? NS : NC);
     return launch_update_after_cs(c);
@@ -2331,17 +2363,16 @@ hipError_t launch_batch_list(eu3d_ctx* c, long long n, double* rho, double* T, d
 
 hipError_t launch_step(eu3d_ctx* c, const eu3d_step_params* p, bool first, bool last) {
     hipError_t e;
-    const bool imex = p->rk == 1 && p->chem == EU3D_CHEM_IMEX;
-    const bool trap = p->rk == 1 && p->chem == EU3D_CHEM_TRAP_ADAPTIVE;
     if (p->cfl > 0) {
-        // the fused IMEX epilogue and the adaptive-trapezoid passes leave the CFL partials of the
-        // new state behind for the next step of the same call
-        e = ((imex || trap) && !first) ? launch_cfl_finish(c, p->cfl, p->final_time)
-                                       : launch_cfl(c, p->cfl, p->final_time);
+        // the fused IMEX epilogue and the step passes of the adaptive trapezoid and of the
+        // surrogate leave the CFL partials of the new state behind for the next step of the call
+        e = (!first && c->part_ready) ? launch_cfl_finish(c, p->cfl, p->final_time)
+                                      : launch_cfl(c, p->cfl, p->final_time);
         if (e != hipSuccess) return e;
     } else {
         k_advance_time<<<1, 1, 0, c->stream>>>(c->sc);
     }
+    c->part_ready = false;
     if ((e = launch_fill_ghosts(c, p->bc_kinds)) != hipSuccess) return e;
     if (p->rk == 2) {
         const size_t n = c->g.n3 * NC;
@@ -2365,7 +2396,10 @@ hipError_t launch_step(eu3d_ctx* c, const eu3d_step_params* p, bool first, bool 
     }
     if ((e = launch_flux_fused(c, p->diff, p->limiter)) != hipSuccess) return e;
     switch (p->chem) {
-    case EU3D_CHEM_IMEX: e = launch_imex_fused(c); break;
+    case EU3D_CHEM_IMEX:
+        e = launch_imex_fused(c);
+        c->part_ready = true;
+        break;
     case EU3D_CHEM_TRAPEZOID:
         if ((e = launch_update_after_adv(c)) != hipSuccess) return e;
         if ((e = launch_trapezoid(c)) != hipSuccess) return e;
@@ -2373,8 +2407,9 @@ hipError_t launch_step(eu3d_ctx* c, const eu3d_step_params* p, bool first, bool 
         break;
     case EU3D_CHEM_TRAP_ADAPTIVE:   // Update_after_Adv fused into the prediction pass
         e = launch_trap_adaptive(c, p->nchem_max, false, 0, 0.0, nullptr, nullptr, true, 0, last, !last);
+        c->part_ready = !last;
         break;
-    case EU3D_CHEM_SURROGATE: e = launch_surrogate_grid(c, p->nchem_max, true); break;
+    case EU3D_CHEM_SURROGATE: e = launch_surrogate_grid(c, p->nchem_max, true, last); break;
     default: e = launch_update_after_adv(c);
     }
     return e;

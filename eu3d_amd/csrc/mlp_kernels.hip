@@ -1561,7 +1561,7 @@ int eu3d_get_surrogate_stats(eu3d_ctx* c, eu3d_surrogate_stats* st) {
 namespace eu3d {
 // Surrogate chemistry over the interior of a grid context (the chemistry half of a step with
 // EU3D_CHEM_SURROGATE): features, network, apply with checks, FP64 fallback on the rejects.
-hipError_t launch_surrogate_grid(eu3d_ctx* c, int nchem_max, bool adv) {
+hipError_t launch_surrogate_grid(eu3d_ctx* c, int nchem_max, bool adv, bool last) {
     eu3d_mlp* m = c->mlp;
     const long long n = (long long)c->g.ni * c->g.nj * c->g.nk;
     if (!m || ensure_infer(m, (int)n) != hipSuccess || sur_ensure(c, n) != EU3D_OK) return hipErrorInvalidValue;
@@ -1579,8 +1579,12 @@ hipError_t launch_surrogate_grid(eu3d_ctx* c, int nchem_max, bool adv) {
         if (e != hipSuccess) return e;
         k_sur_apply_grid<<<grid, blk, 0, c->stream>>>(sp, c->g, c->fl, c->sc, m->iout, c->quirks, b.cnt, b.rej);
     } else {   // stiff cells through the network, the rest through the FP64 sub-steps
+        // within an eu3d_step call (adv) the source is stored on the call's last step, and the
+        // steps before it leave the next step's CFL partials: the prediction pass and the FP64
+        // queue kernels for the cells they finish, k_cfl_list for the network's cells
+        const bool part = adv && !last;
         hipError_t e = launch_trap_adaptive(c, nchem_max, false, sp.nchem_threshold, sp.T_cold, b.net, b.cnt + CNT_NET,
-                                            adv, sp.nchem_threshold_hi);
+                                            adv, sp.nchem_threshold_hi, !part, part);
         if (e != hipSuccess) return e;
         // the network cells' state after Update_after_CS, which the fused pass does not store
         if (adv && (e = launch_adv_list(c, b.net, b.cnt + CNT_NET, (unsigned)n)) != hipSuccess) return e;
@@ -1589,6 +1593,12 @@ hipError_t launch_surrogate_grid(eu3d_ctx* c, int nchem_max, bool adv) {
         if ((e = forward_fused(m, (int)n, m->ifeat, m->iout, c->stream, b.cnt + CNT_NET)) != hipSuccess) return e;
         k_sur_apply_grid_list<<<nb, 256, 0, c->stream>>>(sp, c->g, c->fl, c->sc, m->iout, c->quirks, b.net,
                                                          b.cnt + CNT_NET, b.cnt, b.rej);
+        if ((e = launch_trap_list(c, b.rej, b.cnt, (unsigned)n, b.work, nchem_max)) != hipSuccess) return e;
+        if (part) {
+            if ((e = launch_cfl_list(c, b.net, b.cnt + CNT_NET, (unsigned)n)) != hipSuccess) return e;
+            c->part_ready = true;
+        }
+        return hipSuccess;
     }
     return launch_trap_list(c, b.rej, b.cnt, (unsigned)n, b.work, nchem_max);
 }

@@ -126,6 +126,40 @@ def test_surrogate_grid_step_accepts_cells():
     p.dT_max = float(np.median(dT))
     m.set_surrogate(p)
     _grid_step_check(wl, m, p, mech, expect_rejects=True)
+    _one_call_equals_stepwise(wl, m)
+
+
+def _one_call_equals_stepwise(wl, m):
+    """Three surrogate steps in one call, where each step after the first takes its CFL dt from
+    the partial minima the previous step's passes left (the prediction pass, the FP64 queue
+    kernels and, for the network's accepted and rejected cells, the list pass), equal bit for bit
+    the same steps called one at a time, each of which runs the full CFL pass."""
+    from eu3d_amd import configs
+    from eu3d_amd.api import Context
+    from tests.common import init_state
+    say("case surrogate steps: one call vs one call per step")
+    ctx = []
+    for _ in range(2):
+        g = Context(wl.xnode, wl.ynode, wl.znode, quirks=Q_FIXED)
+        init_state(g, wl)
+        g.step(1, cfl=configs.cfl_ramp(0), bc_kinds=wl.bc_kinds, rk=1, chem=3, nchem_max=10000)
+        g.set_surrogate(m)
+        ctx.append(g)
+    a, b = ctx
+    kw = dict(cfl=configs.cfl_ramp(1), bc_kinds=wl.bc_kinds, rk=1, chem=4, nchem_max=10000)
+    net = 0
+    a.step(3, **kw)
+    for n in range(3):
+        b.step(1, skip_derive=n < 2, **kw)
+        net += b.surrogate_stats()["network"]
+    say("  network cells over the three steps", net)
+    assert net > 0
+    assert a.get_time() == b.get_time()
+    for f in ("CS", "T", "E", "D", "P", "YI", "U", "C", "CMS", "NCHEM"):
+        x, y = a.get(f), b.get(f)
+        assert np.array_equal(x, y, equal_nan=True), f
+    for g in ctx:
+        g.close()
 
 
 def _grid_step_check(wl, m, p, mech, expect_rejects):
