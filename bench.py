@@ -74,7 +74,7 @@ def parse():
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the bubble3d, batched-chemistry and surrogate-network sections")
     ap.add_argument("--cpu-grid", type=str, default="512x64x64")
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--network-only", action="store_true",
                     help="only the surrogate-network section (inference and training at batch 2^20), for profiling")
     ap.add_argument("--copy-calibration", action="store_true",
