@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "exp_fin.hpp"
 #include "mech_gen.h"
 
 namespace eu3d {
@@ -551,7 +552,7 @@ __device__ __forceinline__ void ausm_stencil(const Ld& ld, int order, int limit,
 // k_b is formed without the reference's two-step K_p, K_c division:
 //   k_b = k_f / K_c = k_f exp(sum_s nu_s G_s / RT) (RT / (P0 1e-6))^dnu,
 // with G_s / RT = h_s/(RT) - s_s/R from the NASA fits; the exponents of k_f (b ln T - E/RuT)
-// are folded into one exp. Zero stoichiometry is skipped at compile time.
+// are folded into one exp (exp_fin). Zero stoichiometry is skipped at compile time.
 struct KinOut {
     double w[NS], P[NS], Q[NS], md[NS];
 };
@@ -588,7 +589,7 @@ __device__ __forceinline__ void kinetics_g(const TPow& p, const double (&gx)[NS]
         else if (BF[r] == -2.0) kf *= p.Tm2;
         else if (BF[r] != 0.0) { ex = BF[r] * p.lnT; use_exp = true; }
         if (EAF[r] != 0.0) { ex -= EAF[r] * rRuT; use_exp = true; }
-        if (use_exp) kf *= exp(ex);
+        if (use_exp) kf *= exp_fin(ex);
         double xk = 0.0, rf = 1.0, rb = 1.0, tb = 0.0;
         int xr = 0;
 #pragma unroll
@@ -602,7 +603,7 @@ __device__ __forceinline__ void kinetics_g(const TPow& p, const double (&gx)[NS]
             else if (nb == 2) rb *= Mc[s] * Mc[s];
             if (TB[s][r] != 0.0) tb += Mc[s] * TB[s][r];
         }
-        double kb = kf * exp(xk);
+        double kb = kf * exp_fin(xk);
         if (xr == 1) kb *= rt_p0;
         else if (xr == -1) kb *= p0_rt;
         if (rtb == 1) tb = 0.0;

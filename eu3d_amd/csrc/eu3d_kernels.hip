@@ -1669,7 +1669,7 @@ __global__ void __launch_bounds__(64) k_chem_tail(Acc acc, const unsigned* __res
             const double rRuT = 1.0 / (RU_CAL * T);
             const double tfac = tmode == 1 ? T : tmode == 2 ? p.Tm1 : tmode == 3 ? p.Tm2 : 1.0;
             const double ex = (b_exp ? bf * p.lnT : 0.0) - eaf * rRuT;
-            const double ekf = exp(ex);
+            const double ekf = exp_fin(ex);
             kf = af * tfac;
             if (use_exp) kf *= ekf;
         }
@@ -1695,7 +1695,7 @@ __global__ void __launch_bounds__(64) k_chem_tail(Acc acc, const unsigned* __res
                 rf *= nfj[j] == 0 ? 1.0 : nfj[j] == 1 ? m : m * m;
                 rb *= nbj[j] == 0 ? 1.0 : nbj[j] == 1 ? m : m * m;
             }
-            double kb = kf * exp(xk);
+            double kb = kf * exp_fin(xk);
             kb *= xr == 1 ? RT / P0R : xr == -1 ? P0R / RT : 1.0;
             if (rtb == 1) tb = 0.0;
             else if (rtb == 2 && tb == 0.0) tb = 1.0;
