@@ -1211,7 +1211,17 @@ struct StepOut {
 // tail threshold, the list of the one-cell-per-wavefront kernel.
 struct TailRoute {
     unsigned* list;
-    int tail;   // 0: no routing
+    int tail;         // 0: no routing
+    unsigned few = 0; // list passes: at most this many listed cells send every multi-sub-step cell
+                      // to the tail kernel (as many cells as its wavefronts hold at once)
+    // the threshold for a list pass of cnt cells: a short list (the rejects of the surrogate
+    // network) would leave its long chains to single lanes of a nearly idle persistent kernel,
+    // several times slower per sub-step than the tail kernel's 32 lanes a cell
+    __device__ __forceinline__ TailRoute for_list(unsigned cnt) const {
+        TailRoute r = *this;
+        if (tail > 0 && cnt <= few) r.tail = 1;
+        return r;
+    }
 };
 
 // Wave-aggregated append of the flagged lanes' ids to a device list (one atomic per wave).
@@ -1832,6 +1842,7 @@ __global__ void __launch_bounds__(256) k_trap_first_list(G3 g, Fields F, uint32_
                                                          unsigned* out_list, int nmax, TailRoute rt) {
     // grid-stride over the listed cells (the count is known only on the device)
     const unsigned cnt = *in_count;
+    const TailRoute rl = rt.for_list(cnt);
     for (unsigned b0 = blockIdx.x * blockDim.x; b0 < cnt; b0 += gridDim.x * blockDim.x) {
         const unsigned t = b0 + threadIdx.x;
         const bool in = t < cnt;
@@ -1866,7 +1877,7 @@ __global__ void __launch_bounds__(256) k_trap_first_list(G3 g, Fields F, uint32_
                 active = nch > 1;
             }
         }
-        chem_count_tail(sc, in ? nch : 0, active, out_list, (unsigned)c, rt);
+        chem_count_tail(sc, in ? nch : 0, active, out_list, (unsigned)c, rl);
     }
 }
 
@@ -1877,6 +1888,7 @@ __global__ void __launch_bounds__(256) k_batch_first_list(BatchCells acc, int* n
                                                           unsigned* out_list, TailRoute rt) {
     // grid-stride over the listed cells (the count is known only on the device)
     const unsigned cnt = *in_count;
+    const TailRoute rl = rt.for_list(cnt);
     for (unsigned b0 = blockIdx.x * blockDim.x; b0 < cnt; b0 += gridDim.x * blockDim.x) {
         const unsigned t = b0 + threadIdx.x;
         const bool in = t < cnt;
@@ -1909,7 +1921,7 @@ __global__ void __launch_bounds__(256) k_batch_first_list(BatchCells acc, int* n
                 active = nch > 1;
             }
         }
-        chem_count_tail(sc, in ? nch : 0, active, out_list, (unsigned)id, rt);
+        chem_count_tail(sc, in ? nch : 0, active, out_list, (unsigned)id, rl);
     }
 }
 
@@ -2322,7 +2334,9 @@ hipError_t launch_trap_list(eu3d_ctx* c, const unsigned* list, const unsigned* c
     const int nmax = nchem_max > 0 ? nchem_max : 10000;
     hipError_t e = ensure_coop(c, max_count);
     if (e != hipSuccess) return e;
-    const TailRoute rt{c->coop_list, c->sched.tail};
+    // a short list sends its multi-sub-step cells to the tail kernel, which holds two cells per
+    // wavefront at one wavefront per SIMD
+    const TailRoute rt{c->coop_list, c->sched.tail, 8u * (unsigned)c->num_cu};
     k_chem_reset<<<1, 1, 0, c->stream>>>(c->sc, &c->sc->dtm, 0.0, 1);
     const unsigned nb = std::min<unsigned>((max_count + 255) / 256, (unsigned)c->num_cu * 8);
     const bool c3 = c->quirks & Q_C3;
@@ -2344,7 +2358,9 @@ hipError_t launch_batch_list(eu3d_ctx* c, long long n, double* rho, double* T, d
     const int nmax = nchem_max > 0 ? nchem_max : 10000;
     hipError_t e = ensure_coop(c, max_count);
     if (e != hipSuccess) return e;
-    const TailRoute rt{c->coop_list, c->sched.tail};
+    // a short list sends its multi-sub-step cells to the tail kernel, which holds two cells per
+    // wavefront at one wavefront per SIMD
+    const TailRoute rt{c->coop_list, c->sched.tail, 8u * (unsigned)c->num_cu};
     k_chem_reset<<<1, 1, 0, c->stream>>>(c->sc, &c->sc->dtm, dtm, 0);
     BatchCells acc{n, rho, T, E, nchem_keep};
     const unsigned nb = std::min<unsigned>((max_count + 255) / 256, (unsigned)c->num_cu * 8);

@@ -6,6 +6,10 @@ abundance-weighted element projection) is trained on the solver's own FP64 incre
 seeds, on the cells that need more than 4 FP64 sub-steps. In the hybrid, the cells needing 9..512
 sub-steps go through the network; the rest, and the rejects of the checks, take the FP64 path.
 
+Each run takes the best of four tail thresholds of the chemistry schedule (512, 256, 128, 64: the
+sub-step count above which a cell goes to the 32-lanes-a-cell tail kernel), chosen per leg and per
+case, so that neither side is timed at a schedule tuned for the other.
+
 Two measurements:
 - the whole batch: its time is set by the few cells that ignite within the step (up to the
   10000-sub-step cap), which stay on FP64 in both runs, so the network removes FP64 work
@@ -29,7 +33,10 @@ def _t():
     return torch
 
 
-def _time(ctx, setup, fn, reps=3):
+TAILS = (512, 256, 128, 64)
+
+
+def _time(ctx, setup, fn, reps=2):
     torch = _t()
     s = torch.cuda.ExternalStream(ctx.stream())
     best = None
@@ -51,7 +58,7 @@ def test_network_gain_on_stiff_chemistry():
     from eu3d_amd.api import MLP, Context
     x = configs.make_nodes(8, 1e-4)
     ctx = Context(x, x, x, quirks=0)
-    ctx.set_schedule(256, 4, 2, 512)
+    ctx.set_schedule(256, 4, 2, 512)   # harvest; the timed runs sweep the tail threshold
     say("case surrogate gain: harvest")
     H = surrogate.Harvest()
     for sd in (301, 302, 303):
@@ -83,15 +90,24 @@ def test_network_gain_on_stiff_chemistry():
         rf, tf, ef = ra.clone(), ta.clone(), ea.clone()
 
         rs, ts, es = ra.clone(), ta.clone(), ea.clone()
-        stf, ms_f = _time(ctx, lambda: (rf.copy_(ra), tf.copy_(ta), ef.copy_(ea)),
-                          lambda: ctx.chem_batch(k, rf.data_ptr(), tf.data_ptr(), ef.data_ptr(), DT, 10000))
-        sts, ms_s = _time(ctx, lambda: (rs.copy_(ra), ts.copy_(ta), es.copy_(ea)),
-                          lambda: ctx.surrogate_batch(m, k, rs.data_ptr(), ts.data_ptr(), es.data_ptr(), DT, 10000))
+        ms_f = ms_s = None
+        for tail in TAILS:
+            ctx.set_schedule(256, 4, 2, tail)
+            stf, mf = _time(ctx, lambda: (rf.copy_(ra), tf.copy_(ta), ef.copy_(ea)),
+                            lambda: ctx.chem_batch(k, rf.data_ptr(), tf.data_ptr(), ef.data_ptr(), DT, 10000))
+            sts, mh = _time(ctx, lambda: (rs.copy_(ra), ts.copy_(ta), es.copy_(ea)),
+                            lambda: ctx.surrogate_batch(m, k, rs.data_ptr(), ts.data_ptr(), es.data_ptr(), DT, 10000))
+            say(f"  {name}, tail threshold {tail}: FP64 {mf:.3f} ms, hybrid {mh:.3f} ms")
+            if ms_f is None or mf < ms_f:
+                ms_f, tail_f = mf, tail
+            if ms_s is None or mh < ms_s:
+                ms_s, tail_s = mh, tail
         T_f, T_s = tf.cpu().numpy(), ts.cpu().numpy()
         fin = np.isfinite(T_f)
         dT = np.abs(T_s - T_f)[fin]
         changed = dT > 1e-9 * T_f[fin]
-        res[name] = dict(cells=k, fp64_ms=ms_f, hybrid_ms=ms_s, speedup=ms_f / ms_s, fp64_substeps=stf["substeps_total"],
+        res[name] = dict(cells=k, fp64_ms=ms_f, fp64_tail=tail_f, hybrid_ms=ms_s, hybrid_tail=tail_s,
+                         speedup=ms_f / ms_s, fp64_substeps=stf["substeps_total"],
                          network=sts["network"], rejected=sts["fallback"], T_err_p50=float(np.median(dT[changed])),
                          T_err_p99=float(np.percentile(dT[changed], 99)), T_err_max=float(dT.max()),
                          T_err_max_rel=float((dT / T_f[fin]).max()), fp64_nonfinite=int((~fin).sum()))
@@ -99,12 +115,12 @@ def test_network_gain_on_stiff_chemistry():
         # every cell the network did not answer is the FP64 result
         assert int(changed.sum()) <= sts["network"] - sts["fallback"]
     whole, band = res["whole batch"], res[f"at most {HI} sub-steps"]
-    # measured on one MI355X: band 7.7 -> 5.8 ms (1.33x); the hybrid's own critical path is then the
-    # longest FP64 chain among the cells the |dT| check rejects. Whole batch 31.8 -> 33.6 ms
-    # (0.95x): the FP64 path runs its longest chains (up to the 10000-sub-step cap) beside the rest
-    # of the batch, so the time is that chain's in both runs, and the network's stage adds to it.
+    # measured on one MI355X: band 4.49 ms (FP64, tail 256) -> 2.72 ms (hybrid, tail 128), 1.65x;
+    # the hybrid's own critical path is then the chain of the cells the |dT| check rejects, redone
+    # on FP64 in the tail kernel. Whole batch 30.8 -> 29.6 ms (1.04x): the longest FP64 chains (up
+    # to the 10000-sub-step cap) stay on FP64 in both runs and set the time.
     # T error of the network's cells: median 2.7 K, p99 24 K, largest 456 K (16 %)
-    assert band["speedup"] > 1.15
-    assert whole["speedup"] > 0.85
+    assert band["speedup"] > 1.3
+    assert whole["speedup"] > 0.9
     assert band["network"] > 0.03 * band["cells"]
     assert band["T_err_p50"] < 10.0 and band["T_err_p99"] < 60.0 and band["T_err_max_rel"] < 0.35
