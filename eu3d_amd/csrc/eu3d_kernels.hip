@@ -765,7 +765,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
         const double T0 = FLD(F, EU3D_FIELD_T)[c], E0 = FLD(F, EU3D_FIELD_E)[c];
         const TPow p = tpow(T0);
         KinOut ko;
-        // the conserved state stays in registers through the kinetics (226 VGPRs, still two waves
+        // the conserved state stays in registers through the kinetics (224 VGPRs, still two waves
         // per SIMD): a second read after the kinetics missed L2 often enough to cost 2 %
         double cs[NC];
         load_cs(F, g, c, cs);
