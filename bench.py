@@ -155,6 +155,54 @@ def cpu_baseline(grid, steps):
     return res
 
 
+def cpu_chem_batch(rho, T, E, dtm, grid=(256, 128, 128)):
+    """The oracle's adaptive trapezoid (Euler::Trapezoid(1) cell loop) on flat cells at rest, all
+    of them: rho [n x 9], T, E laid out in the interior of a grid (the chemistry is cell-local;
+    ghost cells hold a copy of the first cell and are not integrated), OpenMP over cells on the
+    host cores. Returns the seconds of the chemistry call and its T."""
+    import oracle
+    from eu3d_amd import configs
+    ni, nj, nk = grid
+    assert ni * nj * nk == T.shape[0]
+    o = oracle.OracleFlow(configs.make_nodes(ni, 1e-4), configs.make_nodes(nj, 1e-4), configs.make_nodes(nk, 1e-4),
+                          quirks=0)
+    sh = o.shape3
+    cs = np.zeros(sh + (13,))
+    cs[..., 0:9] = rho[0]
+    cs[..., 12] = rho[0].sum() * E[0]
+    cs[2:-2, 2:-2, 2:-2, :9] = rho.reshape(ni, nj, nk, 9)
+    cs[2:-2, 2:-2, 2:-2, 12] = (rho.sum(1) * E).reshape(ni, nj, nk)
+    Tg = np.full(sh, T[0])
+    Tg[2:-2, 2:-2, 2:-2] = T.reshape(ni, nj, nk)
+    o.set("CS", cs)
+    o.set("T", Tg)
+    o.update_after_cs()
+    o.set_time(dtm)
+    t0 = time.perf_counter()
+    o.trapezoid_adaptive(10000)
+    el = time.perf_counter() - t0
+    return el, o.get("T")[2:-2, 2:-2, 2:-2].reshape(-1)
+
+
+def cpu_batched_chemistry(n=1 << 22, dtm=1e-8, seed=2024):
+    """The batched-chemistry workload in full on the CPU restatement: the same 2^22
+    equilibrium-line states as the GPU section, one adaptive-trapezoid step of dt = 1e-8 s."""
+    from eu3d_amd import chemistry, configs
+    from eu3d_amd.api import Context
+    import oracle
+    x = configs.make_nodes(8, 1e-4)
+    ctx = Context(x, x, x, quirks=0)
+    (r, t, e), _ = chemistry.states(ctx, n, seed=seed, burnt="equilibrium")
+    rho, T, E = r.cpu().numpy().T.copy(), t.cpu().numpy(), e.cpu().numpy()
+    del r, t, e
+    ctx.close()
+    cores = host_cores()
+    oracle.set_threads(cores)
+    el, _ = cpu_chem_batch(rho, T, E, dtm)
+    return {"value": n / el, "unit": "cells/s", "cores": cores, "kind": "port", "cells": n, "s": el,
+            "sample": f"all {n} equilibrium-line cells (seed {seed}), one adaptive-trapezoid step of dt = {dtm:g} s"}
+
+
 def read_profile(kernel):
     """HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration) and
     FP64 FLOPs per cell of a kernel from its committed rocprofv3 --pmc summary, or {}."""
@@ -485,6 +533,9 @@ def main():
     if not a.no_cpu:
         log("cpu baseline")
         res["cpu_baseline"] = cpu_baseline(a.cpu_grid, a.cpu_steps)
+        if not a.no_extra:
+            log("cpu baseline: batched chemistry")
+            res["cpu_baseline"]["batched_chemistry_equilibrium"] = cpu_batched_chemistry()
     print(json.dumps(res))
 
 
