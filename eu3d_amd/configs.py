@@ -135,6 +135,69 @@ def bubble3d(ni=384, nj=128, nk=128, dx=1e-4, radius_cells=25.0, mach=2.0, nstep
                     dict(cfl=0.5, rk=1, chem=1, diff=1, limiter=0, nchem_max=0), nsteps)
 
 
+def block_nodes(total: int, a: float, b: float, nblocks: int = 1, myid: int = 0, bc: int = BC) -> np.ndarray:
+    """Ghost-inclusive nodes of block `myid` of an axis of `total` nodes over [a, b] split into
+    `nblocks` blocks that share their end nodes (Mesh::Initial, src/Mesh.cpp:75-96: n = (total-1)
+    / nblocks + 1 nodes, node i at a + (myid (n-1) + i - bc) d); ghosts as make_nodes."""
+    d = (b - a) / (total - 1)
+    n = (total - 1) // nblocks + 1
+    x = make_nodes(n, d, a, bc)
+    for i in range(bc, n + bc):
+        x[i] = a + (myid * (n - 1) + i - bc) * d
+    for i in range(bc):
+        x[i] = 2 * x[bc] - x[2 * bc - i] if 2 * bc - i < n + bc else x[bc] - (bc - i) * d
+        x[n + bc + i] = 2 * x[n + bc - 1] - x[n + bc - 1 - i - 1] if n + bc - 2 - i >= bc else x[n + bc - 1] + (i + 1) * d
+    return x
+
+
+def rsbi_reference(decks: dict, block=(0, 0, 0), mech=None) -> Workload:
+    """The reference's own initial state of the 3dRSBI deck, for one block of its MPI
+    decomposition (block = (myid_x, myid_y, myid_z); (0, 0, 0) with m_block 1 x 1 x 1 is the
+    whole domain):
+    - ReactionInitial (src/Reaction.cpp:340-397): a cylinder of pure H2 (axis along z through
+      x = 0.038, y = 0) of radius 0.022 m in air (O2 + 3.76 N2 by moles), with a Gaussian
+      transition out to 1.2 times that radius, Y_H2 = exp(-5 ((r - r_i) / (r_o - r_i))^2) and O2 and
+      N2 sharing the rest in the air's mass ratio;
+    - FieldInitial's first loop (src/Flowfield.cpp:166-190): at rest at P_pre, T_pre where
+      x >= 0.01, the moving shock's post-shock state (shock_post) behind it; V = W = 0.
+    `decks` holds the grid, initialization and ctrl deck values (text, as the decks give them)."""
+    mech = mech or load_default()
+    g, ini, c = decks["grid"], decks["initialization"], decks["ctrl"]
+    nb = [int(c.get(f"m_block_{a}", 1)) for a in "xyz"]
+    bc = int(g.get("bc", BC))
+    nodes = [block_nodes(int(g[f"total_n{a}"]), float(g[f"{b}a"]), float(g[f"{b}b"]), nb[d], block[d], bc)
+             for d, (a, b) in enumerate((("i", "x"), ("j", "y"), ("k", "z")))]
+    x, y, z = nodes
+    sh = (len(x), len(y), len(z))
+    p = _blank(sh, mech.ns)
+    P_pre, T_pre, M_wave = float(ini["P_pre"]), float(ini["T_pre"]), float(ini["M_wave"])
+    U_post, P_post, T_post = shock_post(M_wave, T_pre, P_pre)
+    pre = np.broadcast_to((x >= 0.01)[:, None, None], sh)
+    p["P"][...] = np.where(pre, P_pre, P_post)
+    p["T"][...] = np.where(pre, T_pre, T_post)
+    p["U"][...] = np.where(pre, 0.0, U_post)
+    cx, cy, r_in = 0.038, 0.0, 0.022
+    r_out = 1.2 * r_in
+    iO2, iH2, iN2 = mech.index("O2"), mech.index("H2"), mech.index("N2")
+    r2 = (x[:, None] - cx) ** 2 + (y[None, :] - cy) ** 2
+    mw = mech.mw
+    Y = np.zeros((sh[0], sh[1], mech.ns))
+    tot = 1.0 * mw[iO2] + 3.76 * mw[iN2]
+    outer, inner = r2 > r_out ** 2, r2 < r_in ** 2
+    Y[outer, iO2] = 1.0 * mw[iO2] / tot
+    Y[outer, iN2] = 3.76 * mw[iN2] / tot
+    Y[inner, iH2] = 1.0 * mw[iH2] / (1.0 * mw[iH2])
+    mid = ~(outer | inner)
+    r = np.sqrt(r2[mid])
+    yh2 = np.exp(-5.0 * ((r - r_in) / (r_out - r_in)) ** 2)
+    Y[mid, iH2] = yh2
+    Y[mid, iO2] = (1 - yh2) * mw[iO2] / (mw[iO2] + 3.76 * mw[iN2])
+    Y[mid, iN2] = (1 - yh2) * 3.76 * mw[iN2] / (mw[iO2] + 3.76 * mw[iN2])
+    p["YI"][...] = Y[:, :, None, :]
+    return Workload("rsbi_reference", x, y, z, p, (4, 4, 4, 4, 4, 4),
+                    dict(cfl=float(c.get("cfl", 0.1)), rk=1, chem=1, diff=1, limiter=0, nchem_max=0), 0)
+
+
 def gaussian_field(shape, corr_cells: float, rms: float, seed: int) -> np.ndarray:
     """Periodic Gaussian random field with a Gaussian spectrum of correlation length corr_cells,
     normalised to the given RMS."""

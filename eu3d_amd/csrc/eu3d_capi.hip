@@ -166,6 +166,11 @@ int eu3d_create(eu3d_ctx** out, const eu3d_grid* grid, const eu3d_mech* mech, ui
     (void)hipMemcpyAsync(c->znode, grid->znode, nz * sizeof(double), hipMemcpyHostToDevice, c->stream);
     (void)hipMemsetAsync(c->sc, 0, sizeof(Scalars), c->stream);
     (void)launch_set_scalars(c, 0.0, 0.0);
+    // Nchem starts at 1 in every cell, as Reaction::ReactionConstruction fills it
+    // (src/Reaction.cpp:95-96). The fused steps store a cell's count only where it differs from 1
+    // (or on a call's last step), and the failed-prediction pass takes the cells whose stored
+    // count is <= 0, so a count <= 0 is only ever one that a step wrote and reported.
+    (void)launch_fill(c, c->fl.f[EU3D_FIELD_NCHEM], 1.0, g.n3);
     if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) { eu3d_destroy(c); return EU3D_ERR_HIP; }
     *out = c;
     return EU3D_OK;

@@ -115,6 +115,7 @@ hipError_t launch_chem_reset_step(eu3d_ctx* c);
 hipError_t launch_trap_first_step(eu3d_ctx* c, int nchem_max);
 hipError_t ensure_coop(eu3d_ctx* c, long long n);
 hipError_t launch_set_scalars(eu3d_ctx* c, double dt, double time);
+hipError_t launch_fill(eu3d_ctx* c, double* a, double v, size_t n);   // a[0..n) = v on the context stream
 // first / last: the step is the first / the last of its eu3d_step call
 hipError_t launch_step(eu3d_ctx* c, const eu3d_step_params* p, bool first, bool last);
 // adv: the step's Update_after_Adv + Update_after_CS have not run yet (fused into the passes);

@@ -965,6 +965,11 @@ __global__ void k_copy(const double* __restrict__ a, double* __restrict__ b, siz
         b[t] = a[t];
 }
 
+__global__ void k_fill(double* __restrict__ a, double v, size_t n) {
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
+        a[t] = v;
+}
+
 // ---------------------------------------------------------------- adaptive trapezoid
 // Cell state carried through the sub-steps of Euler::Trapezoid(type 1): partial densities
 // [kg/m^3], velocity, internal energy, temperature.
@@ -2169,6 +2174,11 @@ hipError_t launch_derive(eu3d_ctx* c) {
 
 hipError_t launch_set_scalars(eu3d_ctx* c, double dt, double time) {
     k_set_scalars<<<1, 1, 0, c->stream>>>(c->sc, dt, time);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill(eu3d_ctx* c, double* a, double v, size_t n) {
+    if (n) k_fill<<<grid1d(n, 256), 256, 0, c->stream>>>(a, v, n);
     return hipGetLastError();
 }
 

@@ -2,6 +2,7 @@
 // operation goes through the C ABI of libeu3d.so.
 #include "eu3d_host.hpp"
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -281,6 +282,27 @@ void Flowfield::SetPrimitives(const std::vector<double>& T, const std::vector<do
 
 void Flowfield::FieldInitial() { ctx.check(eu3d_field_initial(ctx.get()), "FieldInitial"); }
 
+void Flowfield::FieldInitial(const InitialDeck& in, const Mesh& m, const std::vector<double>& Yi) {
+    const double M = in.M_wave, T_pre = in.T_pre, P_pre = in.P_pre;
+    // the moving shock's post-shock state as src/Flowfield.cpp:172-174 forms it (ideal gas,
+    // gamma 1.4, R 287 ahead of the shock)
+    const double U_post = M * (1 - (2 + 0.40 * std::pow(M, 2)) / (2.40 * std::pow(M, 2))) * std::sqrt(1.40 * 287 * T_pre);
+    const double P_post = P_pre * (1 + 2 * 1.4 / 2.4 * (std::pow(M, 2) - 1.0));
+    const double T_post = T_pre * (2 + 0.40 * std::pow(M, 2)) / (2.40 * std::pow(M, 2)) * P_post / P_pre;
+    const size_t NX = m.ni + 2 * m.bc, NYZ = (size_t)(m.nj + 2 * m.bc) * (m.nk + 2 * m.bc), n3 = NX * NYZ;
+    std::vector<double> T(n3), P(n3), U(n3), zero(n3, 0.0);
+    for (size_t i = 0; i < NX; i++) {
+        const bool pre = m.xnode[i] >= 0.01;
+        for (size_t c = i * NYZ; c < (i + 1) * NYZ; c++) {
+            P[c] = pre ? P_pre : P_post;
+            T[c] = pre ? T_pre : T_post;
+            U[c] = pre ? 0.0 : U_post;
+        }
+    }
+    SetPrimitives(T, P, U, zero, zero, Yi);
+    FieldInitial();
+}
+
 void Flowfield::CFLcondition(double cfl, double Final_Time) {
     ctx.check(eu3d_cfl_dt(ctx.get(), cfl, Final_Time), "CFLcondition");
 }
@@ -344,7 +366,64 @@ bool Flowfield::IsNan() const {
     return false;
 }
 
+// ------------------------------------------------------------------ initial state decks
+void InitialDeck::InputRead(const std::string& initialization) {
+    FileReader in;
+    if (!in.readFile(initialization)) throw Error(EU3D_ERR_ARG, "cannot read initialization deck " + initialization);
+    P_pre = in.getdoubleParameter("P_pre");
+    T_pre = in.getdoubleParameter("T_pre");
+    M_wave = in.getdoubleParameter("M_wave");
+}
+
 // ------------------------------------------------------------------ Reaction
+namespace {
+int species_index(const Mechanism& me, const char* name) {
+    for (int s = 0; s < me.NS; s++)
+        if (me.species[s] == name) return s;
+    throw Error(EU3D_ERR_ARG, std::string("ReactionInitial: the mechanism has no species ") + name);
+}
+}  // namespace
+
+std::vector<double> Reaction::ReactionInitial(const Mesh& m, const Mechanism& me) {
+    const int NS = me.NS, iO2 = species_index(me, "O2"), iH2 = species_index(me, "H2"),
+              iN2 = species_index(me, "N2");
+    const size_t NX = m.ni + 2 * m.bc, NY = m.nj + 2 * m.bc, NZ = m.nk + 2 * m.bc;
+    const double cx = 0.038, cy = 0.0, r_in = 0.022, r_out = 1.2 * r_in, alpha = 5;
+    // mass fractions of a mixture given by moles, the total summed in species order
+    auto by_moles = [&](const std::vector<double>& mr) {
+        double tot = 0.0;
+        for (int s = 0; s < NS; s++) tot += mr[s] * me.mw[s];
+        std::vector<double> y(NS);
+        for (int s = 0; s < NS; s++) y[s] = mr[s] * me.mw[s] / tot;
+        return y;
+    };
+    std::vector<double> mr_air(NS, 0.0), mr_fuel(NS, 0.0);
+    mr_air[iO2] = 1.0;
+    mr_air[iN2] = 3.76;
+    mr_fuel[iH2] = 1.0;
+    const std::vector<double> y_air = by_moles(mr_air), y_fuel = by_moles(mr_fuel);
+    const double air_mw = me.mw[iO2] + 3.76 * me.mw[iN2];
+    std::vector<double> Y(NX * NY * NZ * NS, 0.0);
+    for (size_t i = 0; i < NX; i++)
+        for (size_t j = 0; j < NY; j++) {
+            const double r2 = std::pow(m.xnode[i] - cx, 2) + std::pow(m.ynode[j] - cy, 2);
+            std::vector<double> y(NS, 0.0);
+            if (r2 > std::pow(r_out, 2)) {
+                y = y_air;
+            } else if (r2 < std::pow(r_in, 2)) {
+                y = y_fuel;
+            } else {
+                const double r = std::sqrt(r2);
+                y[iH2] = std::exp(-alpha * std::pow((r - r_in) / (r_out - r_in), 2));
+                y[iO2] = (1 - y[iH2]) * me.mw[iO2] / air_mw;
+                y[iN2] = (1 - y[iH2]) * 3.76 * me.mw[iN2] / air_mw;
+            }
+            for (size_t k = 0; k < NZ; k++)
+                for (int s = 0; s < NS; s++) Y[((i * NY + j) * NZ + k) * NS + s] = y[s];
+        }
+    return Y;
+}
+
 void Reaction::Trapezoid(Flowfield&) { ctx.check(eu3d_trapezoid(ctx.get()), "Trapezoid"); }
 
 void Reaction::TrapezoidPrediction(Flowfield&, int nchem_max) {
@@ -391,6 +470,9 @@ void Control::FileRead(const std::string& ctrlfile) {
     if (diff == "MUSCL_1") Diff_Sch = 0;
     else if (diff == "MUSCL_2") Diff_Sch = 1;
     else throw Error(EU3D_ERR_UNSUPPORTED, "difference scheme " + diff);
+    if (c.findparameters("m_block_x")) m_block_x = std::max(1, c.getIntParameter("m_block_x"));
+    if (c.findparameters("m_block_y")) m_block_y = std::max(1, c.getIntParameter("m_block_y"));
+    if (c.findparameters("m_block_z")) m_block_z = std::max(1, c.getIntParameter("m_block_z"));
     const std::string ta = c.getStringParameter("time-adv");
     if (ta == "EE") TimeAdv_Sch = TimeAdv::EE;
     else if (ta == "TVD_RK3") TimeAdv_Sch = TimeAdv::TVD_RK3;
@@ -444,6 +526,13 @@ void Euler::IMEX() {
         phase.tr1 += t1 - t0;
         phase.tr2 += now() - t1;
     }
+}
+
+void Euler::Initialize() {
+    if (ctrl.initialization.empty()) throw Error(EU3D_ERR_ARG, "the control deck names no initialization deck");
+    InitialDeck in;
+    in.InputRead(ctrl.initialization);
+    TwoDim.FieldInitial(in, mesh, Reaction::ReactionInitial(mesh, mech));
 }
 
 void Euler::DNN(int type) {
@@ -575,29 +664,54 @@ void Euler::Output_Compute_Time() const {
 }
 
 void Euler::Output(int num) {
-    const std::string name = outdir + "/Series" + ctrl.react + std::to_string(num) + "_0.dat";
     const std::vector<double> nch = React.Nchem();
-    WriteTecplot(name, mesh, TwoDim.field(EU3D_FIELD_T), TwoDim.field(EU3D_FIELD_D), TwoDim.field(EU3D_FIELD_P),
-                 TwoDim.field(EU3D_FIELD_U), TwoDim.field(EU3D_FIELD_V), TwoDim.field(EU3D_FIELD_MA),
-                 TwoDim.field(EU3D_FIELD_YI), mech.NS, nch, 0.0);
+    const std::vector<double> T = TwoDim.field(EU3D_FIELD_T), D = TwoDim.field(EU3D_FIELD_D),
+                              P = TwoDim.field(EU3D_FIELD_P), U = TwoDim.field(EU3D_FIELD_U),
+                              V = TwoDim.field(EU3D_FIELD_V), Ma = TwoDim.field(EU3D_FIELD_MA),
+                              Yi = TwoDim.field(EU3D_FIELD_YI);
+    const int bx = ctrl.m_block_x, by = ctrl.m_block_y, bz = ctrl.m_block_z, nb = bx * by * bz;
+    if (nb <= 1) {
+        WriteTecplot(outdir + "/Series" + ctrl.react + std::to_string(num) + "_0.dat", mesh, T, D, P, U, V, Ma, Yi,
+                     mech.NS, nch, 0.0);
+        return;
+    }
+    const std::string dir = outdir + "/output_" + std::to_string(nb);
+    std::filesystem::create_directories(dir);
+    OutputBlock b;
+    b.ni = (mesh.ni - 1) / bx + 1;
+    b.nj = (mesh.nj - 1) / by + 1;
+    b.nk = (mesh.nk - 1) / bz + 1;
+    for (int id = 0; id < nb; id++) {
+        b.i0 = (id % bx) * (b.ni - 1);
+        b.j0 = (id % (bx * by)) / bx * (b.nj - 1);
+        b.k0 = id / (bx * by) * (b.nk - 1);
+        WriteTecplot(dir + "/results" + ctrl.react + std::to_string(num) + "_" + std::to_string(id) + ".dat", mesh, T,
+                     D, P, U, V, Ma, Yi, mech.NS, nch, 0.0, &b);
+    }
 }
 
 void WriteTecplot(const std::string& filename, const Mesh& m, const std::vector<double>& T, const std::vector<double>& D,
                   const std::vector<double>& P, const std::vector<double>& U, const std::vector<double>& V,
                   const std::vector<double>& Ma, const std::vector<double>& Yi, int ns, const std::vector<double>& Nchem,
-                  double wait_time) {
+                  double wait_time, const OutputBlock* blk) {
+    OutputBlock b;
+    if (blk) b = *blk;
+    else b.ni = m.ni, b.nj = m.nj, b.nk = m.nk;
+    if (b.i0 < 0 || b.j0 < 0 || b.k0 < 0 || b.ni < 1 || b.nj < 1 || b.nk < 1 || b.i0 + b.ni > m.ni ||
+        b.j0 + b.nj > m.nj || b.k0 + b.nk > m.nk)
+        throw Error(EU3D_ERR_ARG, "WriteTecplot: block outside the grid");
     std::ofstream out(filename);
     if (!out.is_open()) throw Error(EU3D_ERR_ARG, "cannot write " + filename);
-    const int NX = m.ni + 2 * m.bc, NY = m.nj + 2 * m.bc, NZ = m.nk + 2 * m.bc;
-    (void)NX;
+    const int NY = m.nj + 2 * m.bc, NZ = m.nk + 2 * m.bc;
     out << "Variables = X,Y,Z,T,D,P,U,V,Ma,Y1,Y2,Y3,Y4,Y5,Y6,Y7,L,WaitTime\n";
-    out << "ZONE I=" << m.ni << '\t' << "J=" << m.nj << '\t' << "K=" << m.nk << '\n';
+    out << "ZONE I=" << b.ni << '\t' << "J=" << b.nj << '\t' << "K=" << b.nk << '\n';
     out << "datapacking=block\n";
-    // one block per variable, i fastest over the interior; reference layout index ((i*NY)+j)*NZ+k
+    // one block per variable, i fastest over the block's nodes; reference layout index ((i*NY)+j)*NZ+k
+    const int ib = m.bc + b.i0, jb = m.bc + b.j0, kb = m.bc + b.k0;
     auto block = [&](auto value) {
-        for (int k = m.bc; k < m.nk + m.bc; k++)
-            for (int j = m.bc; j < m.nj + m.bc; j++)
-                for (int i = m.bc; i < m.ni + m.bc; i++) out << value(i, j, k) << '\n';
+        for (int k = kb; k < kb + b.nk; k++)
+            for (int j = jb; j < jb + b.nj; j++)
+                for (int i = ib; i < ib + b.ni; i++) out << value(i, j, k) << '\n';
     };
     auto cell = [&](int i, int j, int k) { return ((size_t)i * NY + j) * NZ + k; };
     block([&](int i, int, int) { return m.xnode[i]; });

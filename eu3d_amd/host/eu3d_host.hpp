@@ -103,6 +103,13 @@ class TimeAdv {
     enum Scheme { EE = 0, TVD_RK3 = 1 };
 };
 
+// Shock deck values of Flowfield::InputRead (src/Flowfield.cpp:33-58): pre-shock pressure [Pa]
+// and temperature [K] and the Mach number of the moving shock.
+struct InitialDeck {
+    double P_pre = 0, T_pre = 0, M_wave = 0;
+    void InputRead(const std::string& initialization);
+};
+
 class Reaction;
 
 class Flowfield {
@@ -113,6 +120,10 @@ class Flowfield {
     void SetPrimitives(const std::vector<double>& T, const std::vector<double>& P, const std::vector<double>& U,
                        const std::vector<double>& V, const std::vector<double>& W, const std::vector<double>& Yi);
     void FieldInitial();
+    // The whole of FieldInitial (src/Flowfield.cpp:154-229): the moving shock's state first
+    // (at rest at P_pre, T_pre where x >= 0.01 m, the post-shock state of the Mach M_wave shock
+    // behind it, V = W = 0, over every cell), then the second loop above from those and Yi.
+    void FieldInitial(const InitialDeck& in, const Mesh& m, const std::vector<double>& Yi);
     void CFLcondition(double cfl, double Final_Time);
     void FieldBoundary_3dRSBI();
     void FieldBoundary(const int kinds[6]);
@@ -141,6 +152,14 @@ class Flowfield {
 class Reaction {
   public:
     explicit Reaction(Context& c) : ctx(c) {}
+    // ReactionInitial (src/Reaction.cpp:340-397): the mass fractions of the 3dRSBI start over
+    // every cell including ghosts, [cells_with_ghosts * NS] in reference layout. A cylinder
+    // (axis along z through x = 0.038 m, y = 0) of radius 0.022 m holds pure H2; outside 1.2
+    // times that radius is air (O2 + 3.76 N2 by moles); between them H2 falls off as
+    // exp(-5 ((r - r_i) / (r_o - r_i))^2) and O2 / N2 share the rest in the air's mass ratio. The
+    // reference indexes O2, H2 and N2 as species 4, 5 and 8 of its mechanism; here they are
+    // found by name, which is the same for that mechanism.
+    static std::vector<double> ReactionInitial(const Mesh& m, const Mechanism& me);
     // TwoDim.Di = React.Trapezoid(TwoDim.Mc, TwoDim.Di, TwoDim.T, TwoDim.dt): full field,
     // one step of dt (src/Reaction.cpp:412-485); the caller follows with Explicit().
     void Trapezoid(Flowfield& f);
@@ -165,6 +184,9 @@ struct Control {
     int type = 0;           // Trapezoid: 0 full field, 1 adaptive; DNN: 1 surrogate
     int Diff_Sch = 1;       // 0 MUSCL_1, 1 MUSCL_2
     int TimeAdv_Sch = 0;    // EE
+    // MPI blocks of the reference run (m_block_x/y/z, src/Euler.cpp:197-215). One GPU holds the
+    // whole domain; Output writes one file per block as the reference's ranks do.
+    int m_block_x = 1, m_block_y = 1, m_block_z = 1;
     std::string react = "IMEX";
     void FileRead(const std::string& ctrlfile);
 };
@@ -184,7 +206,14 @@ class Euler {
     void Trapezoid(int type);
     void IMEX();
     void DNN(int type);
-    // Tecplot block file as src/Euler.cpp:792-973 writes it (one rank: Series<react><num>_0.dat).
+    // The reference constructor's initial state (src/Euler.cpp:56-68): the shock deck named by
+    // the control deck's "initialization", ReactionInitial, then FieldInitial.
+    void Initialize();
+    // Tecplot block file as src/Euler.cpp:792-973 writes it. One block: <out>/Series<react><num>_0.dat.
+    // With m_block_x * m_block_y * m_block_z > 1, one file per block as the reference's ranks
+    // write them, <out>/output_<blocks>/results<react><num>_<id>.dat, with
+    // id = bx + m_block_x (by + m_block_y bz) (src/Euler.cpp:213-215) and each block's nodes
+    // (Mesh::Initial, src/Mesh.cpp:75-96: neighbouring blocks share their end nodes).
     void Output(int num);
     void set_output_dir(const std::string& d) { outdir = d; }
     void set_surrogate(eu3d_mlp* m);
@@ -229,10 +258,15 @@ class Euler {
 // Tecplot block writer: variables X, Y, Z, T, D, P, U, V, Ma, Y1..Y7, L (Nchem), WaitTime over
 // the interior, each a block of values in i-fastest order, printed as std::ostream prints a
 // double (6 significant digits).
+// With blk, only the interior nodes [i0, i0 + ni) x [j0, j0 + nj) x [k0, k0 + nk) (interior
+// indices, from 0) are written, as the block that holds them writes its file.
+struct OutputBlock {
+    int i0 = 0, j0 = 0, k0 = 0, ni = 0, nj = 0, nk = 0;
+};
 void WriteTecplot(const std::string& filename, const Mesh& m, const std::vector<double>& T, const std::vector<double>& D,
                   const std::vector<double>& P, const std::vector<double>& U, const std::vector<double>& V,
                   const std::vector<double>& Ma, const std::vector<double>& Yi, int ns, const std::vector<double>& Nchem,
-                  double wait_time);
+                  double wait_time, const OutputBlock* blk = nullptr);
 
 }  // namespace eu3d_host
 

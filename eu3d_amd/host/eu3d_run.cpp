@@ -2,13 +2,15 @@
 // on the GPU through the host layer. The flow is eu3d_run_main in libeu3d_host.so, callable
 // in-process with a command line; the eu3d_run program (eu3d_run_main.cpp) only forwards to it.
 //
-//   eu3d_run <ctrl deck> --ic <file> [--bc k0,k1,k2,k3,k4,k5] [--steps N] [--fused RK]
+//   eu3d_run <ctrl deck> [--ic <file>] [--bc k0,k1,k2,k3,k4,k5] [--steps N] [--fused RK]
 //            [--quirks ref|fixed] [--out DIR] [--dump FILE] [--time-record]
 //   eu3d_run <ctrl deck> --print-decks     (parse only, no GPU: JSON of what the decks yield)
 //
 // The control deck names the grid deck, the mechanism and the thermo file as in the reference.
 // --ic gives the primitive fields in reference layout (ghosts included) as raw float64: T, P, U,
-// V, W (one value per cell each) then Yi (NS values per cell). --bc replaces
+// V, W (one value per cell each) then Yi (NS values per cell). Without --ic the run starts from
+// the reference's own initial state: the shock deck that the control deck's "initialization"
+// names, ReactionInitial and FieldInitial (the 3dRSBI start). --bc replaces
 // FieldBoundary_3dRSBI with the given ghost kinds. --fused runs the steps as one eu3d_step call
 // with RK stages (1 or 2) instead of the method-by-method loop. --dump writes CS (13 per cell)
 // and T after the run as raw float64 in reference layout. --time-record times the phases of the
@@ -81,7 +83,7 @@ static std::vector<double> read_block(std::ifstream& in, size_t n, const char* w
 
 extern "C" int eu3d_run_main(int argc, char** argv) {
     if (argc < 2) {
-        std::fprintf(stderr, "usage: %s <ctrl deck> --ic FILE [--bc k,k,k,k,k,k] [--steps N] [--fused RK] "
+        std::fprintf(stderr, "usage: %s <ctrl deck> [--ic FILE] [--bc k,k,k,k,k,k] [--steps N] [--fused RK] "
                              "[--quirks ref|fixed] [--out DIR] [--dump FILE]\n", argv[0]);
         return 2;
     }
@@ -131,14 +133,21 @@ extern "C" int eu3d_run_main(int argc, char** argv) {
         e.set_time_record(time_record);
         const double t1 = clock();
         const size_t n3 = e.mesh.cells_with_ghosts();
-        std::ifstream in(ic, std::ios::binary);
-        if (!in.is_open()) throw Error(EU3D_ERR_ARG, "cannot read initial condition " + ic);
-        std::vector<double> T = read_block(in, n3, "T"), P = read_block(in, n3, "P"), U = read_block(in, n3, "U"),
-                            V = read_block(in, n3, "V"), W = read_block(in, n3, "W"),
-                            Y = read_block(in, n3 * e.mech.NS, "Yi");
-        const double t2 = clock();
-        e.TwoDim.SetPrimitives(T, P, U, V, W, Y);
-        e.TwoDim.FieldInitial();
+        double t2;
+        if (ic.empty()) {
+            // the reference's own start: the shock deck, ReactionInitial, FieldInitial
+            t2 = clock();
+            e.Initialize();
+        } else {
+            std::ifstream in(ic, std::ios::binary);
+            if (!in.is_open()) throw Error(EU3D_ERR_ARG, "cannot read initial condition " + ic);
+            std::vector<double> T = read_block(in, n3, "T"), P = read_block(in, n3, "P"),
+                                U = read_block(in, n3, "U"), V = read_block(in, n3, "V"),
+                                W = read_block(in, n3, "W"), Y = read_block(in, n3 * e.mech.NS, "Yi");
+            t2 = clock();
+            e.TwoDim.SetPrimitives(T, P, U, V, W, Y);
+            e.TwoDim.FieldInitial();
+        }
         e.TwoDim.IsNan();   // waits for the device
         const double t3 = clock();
         if (fused) {

@@ -287,3 +287,34 @@ def test_driver_stepwise_imex_bubble(tmp_path):
     assert float(cmp_[0]) == loop
     for f in ("All_wait.txt", "All_react.txt", "All_advert.txt", "All_DLB.txt"):
         assert (rec / "all" / f).exists()
+
+
+@pytest.mark.gpu
+def test_driver_reference_start_writes_shipped_output(tmp_path):
+    """The reference's own run from its own decks: the 3dRSBI grid (241 x 81 x 2 nodes), its shock
+    deck and 4 x 2 x 1 MPI blocks, started without an initial-condition file (ReactionInitial +
+    FieldInitial on the device). One GPU holds the whole grid, and Output(0) at the first step
+    writes one file per block as the reference's ranks do. Each of the eight files equals the
+    reference's output_8/resultsIMEX0_<rank>.dat line for line (header, 18 x 5002 values)."""
+    _need_driver()
+    import gzip
+    gold = os.path.join(ROOT, "tests", "golden", "output8")
+    decks = json.load(open(os.path.join(gold, "decks.json")))
+    init = tmp_path / "input-3dRSBI.txt"
+    init.write_text("".join(f"{k}\t{v}\n" for k, v in decks["initialization"].items()))
+    c = decks["ctrl"]
+    extra = f"initialization\t{init}\n" + "".join(f"m_block_{a}\t{c[f'm_block_{a}']}\n" for a in "xyz")
+    ctrl = _decks(tmp_path, decks["grid"], reaction=c["reaction"], cfl=float(c["cfl"]), extra=extra)
+    out = tmp_path / "run"
+    out.mkdir()
+    cmd = [ctrl, "--out", str(out), "--steps", "1"]
+    say("run", " ".join(cmd))
+    rc, text = _driver(cmd)
+    assert rc == 0, text
+    for rank in range(8):
+        mine = (out / "output_8" / f"resultsIMEX0_{rank}.dat").read_text().split("\n")
+        with gzip.open(os.path.join(gold, f"resultsIMEX0_{rank}.dat.gz"), "rt") as f:
+            ref = f.read().split("\n")
+        assert len(mine) == len(ref), (rank, len(mine), len(ref))
+        bad = [i for i, (a, b) in enumerate(zip(mine, ref)) if a != b]
+        assert not bad, (rank, len(bad), [(i, mine[i], ref[i]) for i in bad[:5]])

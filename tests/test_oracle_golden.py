@@ -3,8 +3,9 @@
 The reference ships no tests and its Flowfield/Reaction translation units need <mpi.h>, which
 this image lacks, so the oracle is pinned by: the reference's own Differ.hpp/TimeAdv.cpp/
 Function.cpp built by oracle/Makefile.ref (bit-exact), the reference's shipped initial-state
-output (tests/golden/output8_initial.json), and closed forms / invariants chosen so a dropped
-term, wrong sign or transposed index fails.
+output (every printed value of output_8/resultsIMEX0_*.dat, tests/golden/output8/, and the
+sampled points of tests/golden/output8_initial.json), and closed forms / invariants chosen so a
+dropped term, wrong sign or transposed index fails.
 """
 import ctypes as C
 import json
@@ -459,6 +460,49 @@ def test_field_initial_matches_shipped_output():
     post = [q for q in pts if q["class"] == 0]
     assert post and all(abs(q["U"] - up) < 5e-6 * up and abs(q["P"] - pp) < 5e-6 * pp and
                         abs(q["T"] - tp) < 5e-6 * tp for q in post)
+
+
+OUTPUT8 = os.path.join(HERE, "golden", "output8")
+TECPLOT_VARS = "X,Y,Z,T,D,P,U,V,Ma,Y1,Y2,Y3,Y4,Y5,Y6,Y7,L,WaitTime".split(",")
+
+
+def output8_values(rank):
+    """The printed values of the reference's output_8/resultsIMEX0_<rank>.dat (18 blocks of
+    61 x 41 x 2 values, i fastest), as the text lines."""
+    import gzip
+    with gzip.open(os.path.join(OUTPUT8, f"resultsIMEX0_{rank}.dat.gz"), "rt") as f:
+        lines = f.read().split("\n")
+    return lines[:3], [v for v in lines[3:] if v.strip()]
+
+
+def test_reference_initial_state_whole_field():
+    """The reference's own initial state, every printed value of its 8-rank run: for each rank's
+    block (m_block 4 x 2 x 1, rank = bx + 4 by, src/Euler.cpp:213-215) the nodes of Mesh::Initial,
+    ReactionInitial's H2 cylinder and FieldInitial's Mach-3 shock (configs.rsbi_reference), the
+    oracle's FieldInitial (D from the mixture gas constant), Ma = 0 and Nchem = 1 as the reference
+    leaves them at time 0, printed with 6 significant digits (std::ostream's default, C's %g),
+    equal to output_8/resultsIMEX0_<rank>.dat string for string: 8 x 18 x 5002 values."""
+    decks = json.load(open(os.path.join(OUTPUT8, "decks.json")))
+    I = (slice(2, -2),) * 3
+    for rank in range(8):
+        wl = configs.rsbi_reference(decks, (rank % 4, rank // 4, 0))
+        o = OracleFlow(wl.xnode, wl.ynode, wl.znode, quirks=Q_REF)
+        for k in ("T", "P", "U", "V", "W", "YI"):
+            o.set(k, wl.prims[k])
+        o.field_initial()
+        X, Y, Z = np.meshgrid(wl.xnode, wl.ynode, wl.znode, indexing="ij")
+        Yi = o.get("YI")
+        cols = ([X, Y, Z] + [o.get(f) for f in ("T", "D", "P", "U", "V", "MA")] + [Yi[..., s] for s in range(7)]
+                + [o.get("NCHEM"), np.zeros_like(X)])
+        head, ref = output8_values(rank)
+        ni, nj, nk = (len(a) - 4 for a in (wl.xnode, wl.ynode, wl.znode))
+        assert head[1].split() == ["ZONE", f"I={ni}", f"J={nj}", f"K={nk}"]
+        n = ni * nj * nk
+        assert len(ref) == 18 * n
+        for v, (name, col) in enumerate(zip(TECPLOT_VARS, cols)):
+            mine = ["%g" % x for x in col[I].transpose(2, 1, 0).reshape(-1)]
+            bad = [i for i in range(n) if mine[i] != ref[v * n + i]]
+            assert not bad, (rank, name, len(bad), [(mine[i], ref[v * n + i]) for i in bad[:3]])
 
 
 # ------------------------------------------------------------------ ghost kinds and dT/drho_s
