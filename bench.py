@@ -21,7 +21,8 @@ The line carries:
 - roofline: the dominant kernel of the plain step (k_trap_first), timed live with HIP events on
   the context's stream;
 - kernels, bubble3d_imex (the 3-D shock / H2-bubble IMEX configuration), batched_chemistry,
-  surrogate_network: further sections;
+  surrogate_network, surrogate_stiff_chemistry (the hybrid against plain FP64 chemistry on the
+  stiff batched states, where the network takes cells): further sections;
 - cpu_baseline: the CPU restatement of the reference arithmetic (the oracle) on a bounded sample
   of the same workload, on the host's cores.
 """
@@ -421,6 +422,16 @@ def bench_surrogate_network(n=1 << 20, reps=10):
                     "the loss read-back"}
 
 
+def bench_surrogate_stiff():
+    """The surrogate where the chemistry dominates: 2^21 equilibrium-line cells, one step of
+    dt = 1e-8 s, plain FP64 chemistry against the hybrid (cells needing 9..512 FP64 sub-steps through
+    the network trained first on other seeds), for the whole batch and for the same states without
+    the igniting cells (more than 512 sub-steps), each leg at its best tail threshold; errors against
+    the FP64 result (eu3d_amd.surrogate_eval.stiff_gain)."""
+    from eu3d_amd.surrogate_eval import stiff_gain
+    return stiff_gain(log=lambda s: None)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -537,6 +548,8 @@ def main():
         res["batched_chemistry"] = bench_batched_chemistry()
         log("surrogate network")
         res["surrogate_network"] = bench_surrogate_network()
+        log("surrogate on stiff chemistry")
+        res["surrogate_stiff_chemistry"] = bench_surrogate_stiff()
     if not a.no_cpu:
         log("cpu baseline")
         res["cpu_baseline"] = cpu_baseline(a.cpu_grid, a.cpu_steps)

@@ -18,7 +18,10 @@ Two workloads, each a JSON report:
 - plain-detonation: the plain FP64 detonation step alone (its time per step over the last 40 of
   140 steps), the driver of the detonation kernel profile.
 
-Usage: python -m eu3d_amd.surrogate_eval batch|detonation|plain-detonation [options]
+- stiff-gain: what the trained network gains on the stiff batched chemistry (stiff_gain), the
+  bench's and the gain test's measurement.
+
+Usage: python -m eu3d_amd.surrogate_eval batch|detonation|plain-detonation|stiff-gain [options]
 """
 from __future__ import annotations
 
@@ -142,6 +145,99 @@ def eval_batch(n=1 << 22, dtm=1e-8, threshold=8, train_seeds=(101, 102, 103), n_
     return out
 
 
+STIFF_TAILS = (512, 256, 128, 64)
+
+
+def stiff_gain(n=1 << 21, dtm=1e-8, lo=8, hi=512, train_seeds=(301, 302, 303), n_train=1 << 20, epochs=30,
+               seed=2024, tails=STIFF_TAILS, log=print):
+    """What the trained network gains on the stiff batched chemistry (equilibrium-line states, one
+    step of dtm), and its error against the FP64 result cell by cell.
+
+    The network (log-increment outputs, abundance-weighted element projection, |dT| <= 40 K) is
+    trained on the FP64 increments of other seeds, on the cells needing more than 4 and at most
+    2 hi FP64 sub-steps. In the hybrid the cells needing lo+1..hi sub-steps go through the network;
+    the rest, and the rejects of the checks, take the FP64 path. Two cases: the whole batch, and
+    the same states without the cells needing more than hi sub-steps (the igniting cells, whose
+    FP64 chains up to the 10000 cap stay on FP64 in both runs and set the whole batch's time).
+    Each leg is timed at each tail threshold in `tails` (the sub-step count above which a cell goes
+    to the tail kernel) and reported at its own best, so neither is timed at a schedule tuned for
+    the other."""
+    torch = _torch()
+    x = configs.make_nodes(8, 1e-4)
+    ctx = Context(x, x, x, quirks=0)
+    ctx.set_schedule(256, 4, 2, 512)
+    t_start = time.time()
+    H = surrogate.Harvest()
+    for sd in train_seeds:
+        (r, t, e), _ = chemistry.states(ctx, n_train, seed=sd, burnt="equilibrium")
+        r0, t0 = r.clone(), t.clone()
+        nch = torch.empty(n_train, dtype=torch.int32, device="cuda")
+        ctx.chem_batch(n_train, r.data_ptr(), t.data_ptr(), e.data_ptr(), dtm, 10000, nch.data_ptr())
+        H.add(r0, r, t0, dtm, keep=(nch > 4) & (nch <= 2 * hi))
+    r0, r1, t0, dt = H.cat()
+    m = MLP(seed=11)
+    p = surrogate.fit_params(r0, r1, t0, dt, nchem_threshold=lo, nchem_threshold_hi=hi, out_eps=1e-10, dT_max=40.0)
+    m.set_surrogate(p)
+    X, Y = surrogate.make_dataset(m, r0, r1, t0, dt)
+    hist = surrogate.train(m, X, Y, epochs=epochs, batch=4096, lr=2e-3)
+    log(f"surrogate on stiff chemistry: {H.n} pairs, loss {hist[0]:.4f} -> {hist[-1]:.4f}")
+    out = {"training": {"pairs": int(H.n), "epochs": epochs, "loss_first": hist[0], "loss_last": hist[-1],
+                        "harvest_and_train_s": time.time() - t_start, "band_substeps": [lo + 1, hi],
+                        "checks": {"dT_max_K": p.dT_max, "elem_tol": p.elem_tol, "y_neg_tol": p.y_neg_tol,
+                                   "T_cold_K": p.T_cold}}}
+    del H, r0, r1, t0, dt, X, Y
+    (re, te, ee), _ = chemistry.states(ctx, n, seed=seed, burnt="equilibrium")
+    nch = torch.empty(n, dtype=torch.int32, device="cuda")
+    r, t, e = re.clone(), te.clone(), ee.clone()
+    ctx.chem_batch(n, r.data_ptr(), t.data_ptr(), e.data_ptr(), dtm, 10000, nch.data_ptr())
+    keep = (nch <= hi).nonzero().squeeze(1)
+    cases = {"whole_batch": (re, te, ee),
+             "band": (re[:, keep].contiguous(), te[keep].contiguous(), ee[keep].contiguous())}
+    for name, (ra, ta, ea) in cases.items():
+        k = ta.shape[0]
+        rf, tf, ef = ra.clone(), ta.clone(), ea.clone()
+        rs, ts, es = ra.clone(), ta.clone(), ea.clone()
+        best_f = best_s = None
+        for tail in tails:
+            ctx.set_schedule(256, 4, 2, tail)
+            for leg in ("fp64", "hybrid"):
+                ms_best = None
+                for _ in range(2):
+                    if leg == "fp64":
+                        rf.copy_(ra), tf.copy_(ta), ef.copy_(ea)
+                        st, ms = _timed(ctx.stream(), lambda: ctx.chem_batch(k, rf.data_ptr(), tf.data_ptr(),
+                                                                             ef.data_ptr(), dtm, 10000))
+                    else:
+                        rs.copy_(ra), ts.copy_(ta), es.copy_(ea)
+                        st, ms = _timed(ctx.stream(), lambda: ctx.surrogate_batch(m, k, rs.data_ptr(), ts.data_ptr(),
+                                                                                  es.data_ptr(), dtm, 10000))
+                    ms_best = ms if ms_best is None else min(ms_best, ms)
+                log(f"  {name}, tail threshold {tail}: {leg} {ms_best:.3f} ms")
+                if leg == "fp64" and (best_f is None or ms_best < best_f[0]):
+                    best_f = (ms_best, tail, st)
+                if leg == "hybrid" and (best_s is None or ms_best < best_s[0]):
+                    best_s = (ms_best, tail, st)
+        T_f, T_s = tf.cpu().numpy(), ts.cpu().numpy()
+        fin = np.isfinite(T_f)
+        dT = np.abs(T_s - T_f)[fin]
+        changed = dT > 1e-9 * T_f[fin]
+        sts = best_s[2]
+        res = {"cells": int(k), "fp64_ms": best_f[0], "fp64_tail": best_f[1], "hybrid_ms": best_s[0],
+               "hybrid_tail": best_s[1], "speedup": best_f[0] / best_s[0],
+               "fp64_cells_per_s": k / (best_f[0] * 1e-3), "hybrid_cells_per_s": k / (best_s[0] * 1e-3),
+               "fp64_substeps": best_f[2]["substeps_total"], "network": sts["network"], "rejected": sts["fallback"],
+               "cells_changed": int(changed.sum()), "fp64_nonfinite": int((~fin).sum()),
+               "errors_vs_fp64": errors(T_s, T_f, rs.cpu().numpy(), rf.cpu().numpy())}
+        if changed.any():
+            res["T_err_network_cells_K"] = {"p50": float(np.median(dT[changed])),
+                                            "p99": float(np.percentile(dT[changed], 99)), "max": float(dT.max()),
+                                            "max_rel": float((dT / T_f[fin]).max())}
+        out[name] = res
+        log(f"  {name}: {res}")
+    ctx.close()
+    return out
+
+
 def front_x(T, xnode, bc=2, T_front=1500.0):
     """Mean over (y, z) of the largest interior x with T > T_front."""
     Ti = T[bc:-bc, bc:-bc, bc:-bc]
@@ -245,7 +341,7 @@ def time_plain_detonation(grid=(512, 256, 128), nsteps=140, window=40, log=print
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("workload", choices=("batch", "detonation", "plain-detonation"))
+    ap.add_argument("workload", choices=("batch", "detonation", "plain-detonation", "stiff-gain"))
     ap.add_argument("--threshold", type=int, default=None)
     ap.add_argument("--steps", type=int, default=150)
     ap.add_argument("--grid", type=str, default="512x256x128")
@@ -268,6 +364,8 @@ def main():
             kw["dT_max"] = a.dT_max
         kw["train_seeds"] = tuple(101 + i for i in range(a.seeds))
         res = eval_batch(log=log, stiff_only=a.stiff_only, **kw)
+    elif a.workload == "stiff-gain":
+        res = stiff_gain(log=log)
     elif a.workload == "plain-detonation":
         res = time_plain_detonation(grid=tuple(int(v) for v in a.grid.split("x")), nsteps=a.steps, log=log)
     else:

@@ -16,105 +16,28 @@ Two measurements:
   without shortening that critical path (and its own stage adds to it);
 - the same states without those igniting cells (at most 512 FP64 sub-steps): the FP64 time is the
   sub-step throughput of the stiff cells, which the network replaces.
-The error is measured against the FP64 result cell by cell."""
-import numpy as np
+The error is measured against the FP64 result cell by cell. The measurement itself is
+eu3d_amd.surrogate_eval.stiff_gain, which bench.py also reports."""
 import pytest
 
 from tests.common import say
 
 pytestmark = pytest.mark.gpu
-DT = 1e-8
-LO, HI = 8, 512
-
-
-def _t():
-    import torch
-    assert torch.cuda.is_available()
-    return torch
-
-
-TAILS = (512, 256, 128, 64)
-
-
-def _time(ctx, setup, fn, reps=2):
-    torch = _t()
-    s = torch.cuda.ExternalStream(ctx.stream())
-    best = None
-    for _ in range(reps):
-        setup()
-        torch.cuda.synchronize()
-        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        a.record(s)
-        out = fn()
-        b.record(s)
-        b.synchronize()
-        best = a.elapsed_time(b) if best is None else min(best, a.elapsed_time(b))
-    return out, best
 
 
 def test_network_gain_on_stiff_chemistry():
-    torch = _t()
-    from eu3d_amd import chemistry, configs, surrogate
-    from eu3d_amd.api import MLP, Context
-    x = configs.make_nodes(8, 1e-4)
-    ctx = Context(x, x, x, quirks=0)
-    ctx.set_schedule(256, 4, 2, 512)   # harvest; the timed runs sweep the tail threshold
-    say("case surrogate gain: harvest")
-    H = surrogate.Harvest()
-    for sd in (301, 302, 303):
-        (r, t, e), _ = chemistry.states(ctx, 1 << 20, seed=sd, burnt="equilibrium")
-        r0, t0 = r.clone(), t.clone()
-        nch = torch.empty(1 << 20, dtype=torch.int32, device="cuda")
-        ctx.chem_batch(1 << 20, r.data_ptr(), t.data_ptr(), e.data_ptr(), DT, 10000, nch.data_ptr())
-        H.add(r0, r, t0, DT, keep=(nch > 4) & (nch <= 2 * HI))
-    r0, r1, t0, dt = H.cat()
-    m = MLP(seed=11)
-    p = surrogate.fit_params(r0, r1, t0, dt, nchem_threshold=LO, nchem_threshold_hi=HI, out_eps=1e-10, dT_max=40.0)
-    m.set_surrogate(p)
-    X, Y = surrogate.make_dataset(m, r0, r1, t0, dt)
-    hist = surrogate.train(m, X, Y, epochs=30, batch=4096, lr=2e-3)
-    say(f"  {H.n} pairs, loss {hist[0]:.4f} -> {hist[-1]:.4f}")
-    assert hist[-1] < 0.4 * hist[0]
-
-    n = 1 << 21
-    (re, te, ee), _ = chemistry.states(ctx, n, seed=2024, burnt="equilibrium")
-    nch = torch.empty(n, dtype=torch.int32, device="cuda")
-    r, t, e = re.clone(), te.clone(), ee.clone()
-    ctx.chem_batch(n, r.data_ptr(), t.data_ptr(), e.data_ptr(), DT, 10000, nch.data_ptr())
-    keep = (nch <= HI).nonzero().squeeze(1)
-    cases = {"whole batch": (re, te, ee), f"at most {HI} sub-steps": (re[:, keep].contiguous(), te[keep].contiguous(),
-                                                                        ee[keep].contiguous())}
-    res = {}
-    for name, (ra, ta, ea) in cases.items():
-        k = ta.shape[0]
-        rf, tf, ef = ra.clone(), ta.clone(), ea.clone()
-
-        rs, ts, es = ra.clone(), ta.clone(), ea.clone()
-        ms_f = ms_s = None
-        for tail in TAILS:
-            ctx.set_schedule(256, 4, 2, tail)
-            stf, mf = _time(ctx, lambda: (rf.copy_(ra), tf.copy_(ta), ef.copy_(ea)),
-                            lambda: ctx.chem_batch(k, rf.data_ptr(), tf.data_ptr(), ef.data_ptr(), DT, 10000))
-            sts, mh = _time(ctx, lambda: (rs.copy_(ra), ts.copy_(ta), es.copy_(ea)),
-                            lambda: ctx.surrogate_batch(m, k, rs.data_ptr(), ts.data_ptr(), es.data_ptr(), DT, 10000))
-            say(f"  {name}, tail threshold {tail}: FP64 {mf:.3f} ms, hybrid {mh:.3f} ms")
-            if ms_f is None or mf < ms_f:
-                ms_f, tail_f = mf, tail
-            if ms_s is None or mh < ms_s:
-                ms_s, tail_s = mh, tail
-        T_f, T_s = tf.cpu().numpy(), ts.cpu().numpy()
-        fin = np.isfinite(T_f)
-        dT = np.abs(T_s - T_f)[fin]
-        changed = dT > 1e-9 * T_f[fin]
-        res[name] = dict(cells=k, fp64_ms=ms_f, fp64_tail=tail_f, hybrid_ms=ms_s, hybrid_tail=tail_s,
-                         speedup=ms_f / ms_s, fp64_substeps=stf["substeps_total"],
-                         network=sts["network"], rejected=sts["fallback"], T_err_p50=float(np.median(dT[changed])),
-                         T_err_p99=float(np.percentile(dT[changed], 99)), T_err_max=float(dT.max()),
-                         T_err_max_rel=float((dT / T_f[fin]).max()), fp64_nonfinite=int((~fin).sum()))
-        say(f"  {name}: {res[name]}")
+    import torch
+    assert torch.cuda.is_available()
+    from eu3d_amd.surrogate_eval import stiff_gain
+    say("case surrogate gain: harvest, train, time both legs")
+    res = stiff_gain(log=say)
+    tr = res["training"]
+    assert tr["loss_last"] < 0.4 * tr["loss_first"]
+    for name in ("whole_batch", "band"):
+        r = res[name]
         # every cell the network did not answer is the FP64 result
-        assert int(changed.sum()) <= sts["network"] - sts["fallback"]
-    whole, band = res["whole batch"], res[f"at most {HI} sub-steps"]
+        assert r["cells_changed"] <= r["network"] - r["rejected"], (name, r)
+    whole, band = res["whole_batch"], res["band"]
     # measured on one MI355X: band 4.49 ms (FP64, tail 256) -> 2.72 ms (hybrid, tail 128), 1.65x;
     # the hybrid's own critical path is then the chain of the cells the |dT| check rejects, redone
     # on FP64 in the tail kernel. Whole batch 30.8 -> 29.6 ms (1.04x): the longest FP64 chains (up
@@ -123,4 +46,5 @@ def test_network_gain_on_stiff_chemistry():
     assert band["speedup"] > 1.3
     assert whole["speedup"] > 0.9
     assert band["network"] > 0.03 * band["cells"]
-    assert band["T_err_p50"] < 10.0 and band["T_err_p99"] < 60.0 and band["T_err_max_rel"] < 0.35
+    err = band["T_err_network_cells_K"]
+    assert err["p50"] < 10.0 and err["p99"] < 60.0 and err["max_rel"] < 0.35
