@@ -595,9 +595,9 @@ __global__ void __launch_bounds__(256) k_explicit(G3 g, Fields F, uint32_t q) {
 //   k_flux_xy (second): one z plane per block. An X x Y thread block (32 x 8 by default) stages
 //     the primitives of its tile plus the two-cell halo ((X+3) x (Y+3) cells x 16 variables,
 //     48 KB at 32 x 8, three blocks per CU) in LDS with coalesced loads, computes the x face left
-//     of and the y face below every cell from LDS, exchanges the face fluxes through the same LDS
-//     and writes RHS = (-dt/dx dF - dt/dy dG) - dt/dz dQ for (X-1) x (Y-1) cells, EE's order of
-//     operations.
+//     of and the y face below every cell from LDS, exchanges the x and y face fluxes together
+//     through the same LDS (sized for both: 53 KB at 16 x 16, three blocks per CU) and writes
+//     RHS = (-dt/dx dF - dt/dy dG) - dt/dz dQ for (X-1) x (Y-1) cells, EE's order of operations.
 constexpr int NV = 16;
 constexpr int ZX = 64, ZY = 4;   // k_flux_z block: 64 x 4 columns; the z chunk is a template parameter
 
@@ -623,9 +623,12 @@ template <int XY_X, int XY_Y, bool XCD, bool MM>
 __global__ void __launch_bounds__(XY_X * XY_Y) k_flux_xy(G3 g, Fields F, int order_a, int limiter_a, uint32_t q,
                                                  const double* __restrict__ xn, const double* __restrict__ yn,
                                                  const Scalars* __restrict__ sc) {
-    constexpr int XY_TX = XY_X + 3, XY_TY = XY_Y + 3;
+    constexpr int XY_TX = XY_X + 3, XY_TY = XY_Y + 3, NT = XY_X * XY_Y;
     const int order = MM ? 1 : order_a, limiter = MM ? 0 : limiter_a;
-    __shared__ double sp[NV][XY_TY][XY_TX];
+    // the staged tile, then (reused) the x and y face fluxes side by side
+    constexpr int NSTAGE = NV * XY_TY * XY_TX, NEX = 2 * NC * NT;
+    __shared__ double lds[NSTAGE > NEX ? NSTAGE : NEX];
+    auto sp = reinterpret_cast<double (*)[XY_TY][XY_TX]>(lds);
     int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
     if (XCD) {
         const unsigned nbx = gridDim.x, nby = gridDim.y, total = nbx * nby * gridDim.z;
@@ -674,31 +677,25 @@ __global__ void __launch_bounds__(XY_X * XY_Y) k_flux_xy(G3 g, Fields F, int ord
         ausm_stencil<1>(ld, order, limiter, f1b && x >= g.ni, fy);
     }
     __syncthreads();
-    double* sf = &sp[0][0][0];   // reuse the staging area for the face-flux exchange
+    double* sfx = lds;             // the staging area, reused for the face-flux exchange
+    double* sfy = lds + NC * NT;
     if (xface) {
 #pragma unroll
-        for (int c = 0; c < NC; c++) sf[c * (XY_X * XY_Y) + t] = fx[c];
+        for (int c = 0; c < NC; c++) sfx[c * NT + t] = fx[c];
     }
-    __syncthreads();
-    double rhs[NC];
-    const double dt = sc->dt;
-    if (out) {
-        const double cx = -dt / (xn[x] - xn[x - 1]);
-#pragma unroll
-        for (int c = 0; c < NC; c++) rhs[c] = cx * (sf[c * (XY_X * XY_Y) + t + 1] - sf[c * (XY_X * XY_Y) + t]);
-    }
-    __syncthreads();
     if (yface) {
 #pragma unroll
-        for (int c = 0; c < NC; c++) sf[c * (XY_X * XY_Y) + t] = fy[c];
+        for (int c = 0; c < NC; c++) sfy[c * NT + t] = fy[c];
     }
     __syncthreads();
     if (out) {
-        const double cy = dt / (yn[y] - yn[y - 1]);
+        const double dt = sc->dt;
+        const double cx = -dt / (xn[x] - xn[x - 1]), cy = dt / (yn[y] - yn[y - 1]);
 #pragma unroll
         for (int c = 0; c < NC; c++) {
-            rhs[c] -= cy * (sf[c * (XY_X * XY_Y) + t + XY_X] - sf[c * (XY_X * XY_Y) + t]);
-            Rh[c * g.n3 + cc] = rhs[c] - zt[c];
+            double rhs = cx * (sfx[c * NT + t + 1] - sfx[c * NT + t]);
+            rhs -= cy * (sfy[c * NT + t + XY_X] - sfy[c * NT + t]);
+            Rh[c * g.n3 + cc] = rhs - zt[c];
         }
     }
 }
