@@ -124,8 +124,7 @@ int eu3d_create(eu3d_ctx** out, const eu3d_grid* grid, const eu3d_mech* mech, ui
     c->quirks = quirks;
     G3& g = c->g;
     g.ni = grid->ni; g.nj = grid->nj; g.nk = grid->nk; g.bc = grid->bc;
-    g.NX = g.ni + 2 * g.bc; g.NY = g.nj + 2 * g.bc; g.NZ = g.nk + 2 * g.bc;
-    g.n3 = (size_t)g.NX * g.NY * g.NZ;
+    grid_layout(g);
     if (g.n3 >= (1ull << 32)) { delete c; return EU3D_ERR_ARG; }
     if (const char* v = getenv("EU3D_FLUX_XY_TILE")) c->flux_variant = atoi(v);
     if (const char* v = getenv("EU3D_FLUX_Z_CHUNK")) c->zchunk = atoi(v);
@@ -178,7 +177,13 @@ int eu3d_create(eu3d_ctx** out, const eu3d_grid* grid, const eu3d_mech* mech, ui
 
 int eu3d_grid_sizes(int ni, int nj, int nk, int bc, long long* cfl_partials, unsigned long long* device_bytes) {
     if (bc != 2 || ni < 1 || nj < 1 || nk < 1) return EU3D_ERR_ARG;
-    const unsigned long long NX = ni + 2ull * bc, NY = nj + 2ull * bc, NZ = nk + 2ull * bc, n3 = NX * NY * NZ;
+    G3 g{};
+    g.ni = ni;
+    g.nj = nj;
+    g.nk = nk;
+    g.bc = bc;
+    grid_layout(g);
+    const unsigned long long NX = g.NX, NY = g.NY, NZ = g.NZ, n3 = g.n3;
     const long long parts = interior_blocks(ni, nj, nk);
     if (cfl_partials) *cfl_partials = parts;
     if (device_bytes) {
@@ -237,7 +242,7 @@ int eu3d_field_components(const eu3d_ctx* c, int field) {
 int eu3d_set_field(eu3d_ctx* c, int field, const double* host) {
     if (!c || !host || ncomp(field) == 0) return fail(c, EU3D_ERR_ARG, "eu3d_set_field: bad field or pointer");
     const int nc = ncomp(field);
-    const size_t bytes = c->g.n3 * nc * sizeof(double);
+    const size_t bytes = c->g.ncell * nc * sizeof(double);
     // stage through the RHS-sized save buffer (n3 * NC doubles, the largest field)
     HIPCHK(c, hipMemcpyAsync(c->rk_save, host, bytes, hipMemcpyHostToDevice, c->stream), "set_field copy");
     HIPCHK(c, launch_aos_to_soa(c, c->rk_save, c->fl.f[field], nc), "set_field transpose");
@@ -248,7 +253,7 @@ int eu3d_set_field(eu3d_ctx* c, int field, const double* host) {
 int eu3d_get_field(eu3d_ctx* c, int field, double* host) {
     if (!c || !host || ncomp(field) == 0) return fail(c, EU3D_ERR_ARG, "eu3d_get_field: bad field or pointer");
     const int nc = ncomp(field);
-    const size_t bytes = c->g.n3 * nc * sizeof(double);
+    const size_t bytes = c->g.ncell * nc * sizeof(double);
     HIPCHK(c, launch_soa_to_aos(c, c->fl.f[field], c->rk_save, nc), "get_field transpose");
     HIPCHK(c, hipMemcpyAsync(host, c->rk_save, bytes, hipMemcpyDeviceToHost, c->stream), "get_field copy");
     HIPCHK(c, hipStreamSynchronize(c->stream), "get_field sync");
@@ -546,7 +551,7 @@ int eu3d_chem_batch_noalloc(long long n, double* rho_s, double* T, double* E, do
     return rc;
 }
 
-size_t eu3d_cells_with_ghosts(const eu3d_ctx* c) { return c ? c->g.n3 : 0; }
+size_t eu3d_cells_with_ghosts(const eu3d_ctx* c) { return c ? c->g.ncell : 0; }
 
 void* eu3d_stream(eu3d_ctx* c) { return c ? (void*)c->stream : nullptr; }
 

@@ -63,21 +63,23 @@ __device__ __forceinline__ void store_prims(const Fields& F, const G3& g, size_t
 }
 
 // ---------------------------------------------------------------- layout conversion
+// Reference Array layout (((i NY) + j) NZ + k) nc + c, ghosts included, to and from the device's
+// SoA rows (IDX).
 __global__ void k_aos_to_soa(G3 g, const double* __restrict__ aos, double* __restrict__ soa, int nc) {
-    const size_t total = g.n3 * (size_t)nc;
+    const size_t total = g.ncell * (size_t)nc;
     for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
-        const size_t c = t / g.n3, cell = t - c * g.n3;
+        const size_t c = t / g.ncell, cell = t - c * g.ncell;
         const size_t i = cell % g.NX, jk = cell / g.NX, j = jk % g.NY, k = jk / g.NY;
-        soa[t] = aos[((i * g.NY + j) * g.NZ + k) * nc + c];
+        soa[c * g.n3 + IDX(g, i, j, k)] = aos[((i * g.NY + j) * g.NZ + k) * nc + c];
     }
 }
 
 __global__ void k_soa_to_aos(G3 g, const double* __restrict__ soa, double* __restrict__ aos, int nc) {
-    const size_t total = g.n3 * (size_t)nc;
+    const size_t total = g.ncell * (size_t)nc;
     for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
         const size_t c = t % nc, cell = t / nc;
         const size_t k = cell % g.NZ, ij = cell / g.NZ, j = ij % g.NY, i = ij / g.NY;
-        aos[t] = soa[c * g.n3 + (k * g.NY + j) * g.NX + i];
+        aos[t] = soa[c * g.n3 + IDX(g, i, j, k)];
     }
 }
 
@@ -119,8 +121,10 @@ __global__ void __launch_bounds__(256) k_partial_t(G3 g, Fields F, uint32_t q) {
 // Flowfield::FieldInitial second loop (src/Flowfield.cpp:193-226) over every cell including
 // ghosts: from T, P, U, V, W, Yi to Wav, Rgas, D, Di, Mc, Cp, H, E, Gamma, C and CS.
 __global__ void k_field_initial(G3 g, Fields F, uint32_t q) {
-    const size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-    if (c >= g.n3) return;
+    const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (t >= g.ncell) return;
+    const size_t row = t / g.NX;
+    const size_t c = row * g.PX + g.OX + (t - row * g.NX);
     double Y[NS];
 #pragma unroll
     for (int s = 0; s < NS; s++) Y[s] = FLD(F, EU3D_FIELD_YI)[s * g.n3 + c];
@@ -413,7 +417,7 @@ __global__ void __launch_bounds__(256) k_face_flux(G3 g, Fields F, int order, in
     const int k = (dir == 2 ? g.bc - 1 : g.bc) + (int)blockIdx.z;
     if (i >= g.ni + g.bc || j >= g.nj + g.bc || k >= g.nk + g.bc) return;
     const size_t c = IDX(g, i, j, k);
-    const size_t st = dir == 0 ? 1 : (dir == 1 ? (size_t)g.NX : (size_t)g.NX * g.NY);
+    const size_t st = dir == 0 ? 1 : (dir == 1 ? (size_t)g.PX : (size_t)g.PX * g.NY);
     Face L, R;
     auto rec = [&](const double* f, double& l, double& r) {
         muscl(order, limiter, f[c - st], f[c], f[c + st], f[c + 2 * st], l, r);
@@ -444,7 +448,7 @@ __global__ void __launch_bounds__(256) k_ee(G3 g, Fields F, const double* __rest
     const double cx = -dt / (xn[i] - xn[i - 1]);
     const double cy = dt / (yn[j] - yn[j - 1]);
     const double cz = dt / (zn[k] - zn[k - 1]);
-    const size_t sy = g.NX, sz = (size_t)g.NX * g.NY;
+    const size_t sy = g.PX, sz = (size_t)g.PX * g.NY;
     const double *Fx = FLD(F, EU3D_FIELD_F), *Fy = FLD(F, EU3D_FIELD_G), *Fz = FLD(F, EU3D_FIELD_Q);
     double* R = FLD(F, EU3D_FIELD_RHS);
 #pragma unroll
@@ -709,7 +713,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
     const int x = g.bc + blockIdx.x * ZX + threadIdx.x, y = g.bc + blockIdx.y * ZY + threadIdx.y;
     if (x >= g.ni + g.bc || y >= g.nj + g.bc) return;
     const int z0 = g.bc + blockIdx.z * ZCH, z1 = min(z0 + ZCH, g.nk + g.bc);
-    const size_t base = IDX(g, x, y, 0), sz = (size_t)g.NX * g.NY;
+    const size_t base = IDX(g, x, y, 0), sz = (size_t)g.PX * g.NY;
     const bool zy = (q & Q_F1B) && x >= g.ni;
     double w[NV][4], nxt[NV];
 #pragma unroll
@@ -1110,8 +1114,8 @@ struct GridCellsStep {
     // block_min_store writes: 64 x 4 blocks over (i, j), one grid layer per k); the bounds are
     // positive, so their bit patterns order as unsigned integers
     __device__ void cfl_min(unsigned id, const Prim& o) const {
-        const size_t cc = id, plane = (size_t)g.NX * g.NY;
-        const int k = (int)(cc / plane), j = (int)((cc / g.NX) % g.NY), i = (int)(cc % g.NX);
+        const size_t row = (id - (size_t)g.OX) / g.PX;   // the inverse of IDX
+        const int k = (int)(row / g.NY), j = (int)(row % g.NY), i = (int)(id - g.OX - row * g.PX);
         const int gx = (g.ni + BX - 1) / BX, gy = (g.nj + BY - 1) / BY;
         const size_t b = ((size_t)(k - g.bc) * gy + (j - g.bc) / BY) * gx + (i - g.bc) / BX;
         const double v = cfl_bound(cdx, cdy, cdz, o.U, o.V, o.W, o.C, q);
@@ -1977,12 +1981,12 @@ static int grid1d(size_t n, int blk) {
 }
 
 hipError_t launch_aos_to_soa(eu3d_ctx* c, const double* aos, double* soa, int nc) {
-    k_aos_to_soa<<<grid1d(c->g.n3 * nc, 256), 256, 0, c->stream>>>(G(c), aos, soa, nc);
+    k_aos_to_soa<<<grid1d(c->g.ncell * nc, 256), 256, 0, c->stream>>>(G(c), aos, soa, nc);
     return hipGetLastError();
 }
 
 hipError_t launch_soa_to_aos(eu3d_ctx* c, const double* soa, double* aos, int nc) {
-    k_soa_to_aos<<<grid1d(c->g.n3 * nc, 256), 256, 0, c->stream>>>(G(c), soa, aos, nc);
+    k_soa_to_aos<<<grid1d(c->g.ncell * nc, 256), 256, 0, c->stream>>>(G(c), soa, aos, nc);
     return hipGetLastError();
 }
 
@@ -1992,7 +1996,7 @@ hipError_t launch_update_after_cs(eu3d_ctx* c) {
 }
 
 hipError_t launch_field_initial(eu3d_ctx* c) {
-    k_field_initial<<<(unsigned)((c->g.n3 + 255) / 256), 256, 0, c->stream>>>(G(c), c->fl, c->quirks);
+    k_field_initial<<<(unsigned)((c->g.ncell + 255) / 256), 256, 0, c->stream>>>(G(c), c->fl, c->quirks);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     return launch_partial_t(c);
