@@ -497,7 +497,7 @@ int eu3d_time_kernel(eu3d_ctx* c, int kernel, int reps, int diff, int limiter, d
         case EU3D_KERNEL_FLUX_Z: e = launch_flux_z(c, diff, limiter); break;
         case EU3D_KERNEL_FLUX_XY: e = launch_flux_xy(c, diff, limiter); break;
         case EU3D_KERNEL_TRAP_FIRST: e = launch_trap_first_step(c, 10000); break;
-        default: e = launch_imex_fused(c); break;
+        default: e = launch_imex_fused(c, false); break;   // as on a step that is not the last of its call
         }
         if (e != hipSuccess || (e = hipEventRecord(e1, c->stream)) != hipSuccess || (e = hipEventSynchronize(e1)) != hipSuccess)
             break;

@@ -112,7 +112,9 @@ hipError_t launch_cfl_finish(eu3d_ctx* c, double cfl, double final_time);
 hipError_t launch_flux_fused(eu3d_ctx* c, int diff, int limiter);
 hipError_t launch_flux_z(eu3d_ctx* c, int diff, int limiter);
 hipError_t launch_flux_xy(eu3d_ctx* c, int diff, int limiter);
-hipError_t launch_imex_fused(eu3d_ctx* c);
+// last: the step is the last of its eu3d_step call (C, the sound speed, is stored then; within a
+// call nothing reads it)
+hipError_t launch_imex_fused(eu3d_ctx* c, bool last);
 hipError_t launch_derive(eu3d_ctx* c);
 hipError_t launch_clear_nchem_flag(eu3d_ctx* c);
 hipError_t launch_update_after_adv(eu3d_ctx* c);

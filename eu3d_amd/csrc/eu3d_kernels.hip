@@ -758,12 +758,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
 // the start of the step (CS, T) instead of being read back; the outputs are the new CS, T, E and
 // the primitives the next advection and CFL bound read. Each block writes its CFL partial.
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) k_imex_fused(G3 g, Fields F, uint32_t q, const Scalars* __restrict__ sc,
-                                                    double dx, double dy, double dz, double* part) {
+                                                    double dx, double dy, double dz, double* part, int store_c) {
     int i, j, k;
     double v = 1e10;
     if (interior_ijk(g, i, j, k)) {
         const size_t c = IDX(g, i, j, k);
-        const double T0 = FLD(F, EU3D_FIELD_T)[c], E0 = FLD(F, EU3D_FIELD_E)[c];
+        // the old E enters the Newton only under REF quirk F3
+        const double T0 = FLD(F, EU3D_FIELD_T)[c], E0 = (q & Q_F3) ? FLD(F, EU3D_FIELD_E)[c] : 0.0;
         const TPow p = tpow(T0);
         KinOut ko;
         // the conserved state stays in registers through the kinetics (224 VGPRs, still two waves
@@ -822,7 +823,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
         FLD(F, EU3D_FIELD_W)[c] = n.W;
         FLD(F, EU3D_FIELD_H)[c] = n.H;
         FLD(F, EU3D_FIELD_GAMMA)[c] = n.Gamma;
-        FLD(F, EU3D_FIELD_C)[c] = n.C;
+        if (store_c) FLD(F, EU3D_FIELD_C)[c] = n.C;
 #pragma unroll
         for (int s = 0; s < NS; s++) FLD(F, EU3D_FIELD_YI)[s * g.n3 + c] = n.Y[s];
         v = cfl_bound(dx, dy, dz, n.U, n.V, n.W, n.C, q);
@@ -1087,6 +1088,9 @@ struct GridCellsStep {
     G3 g;
     Fields F;
     bool src = true;           // store the chemical source (CMS): on the last step of an eu3d_step call
+    // store C (the sound speed): on the last step of a call. Within a call nothing reads it: the
+    // chemistry passes leave the next step's CFL partials themselves.
+    bool store_c = true;
     double* cfl = nullptr;     // fold each stored cell's CFL bound into the next step's block partials
     double cdx = 0, cdy = 0, cdz = 0;
     uint32_t q = 0;
@@ -1140,7 +1144,7 @@ struct GridCellsStep {
         FLD(F, EU3D_FIELD_P)[cc] = o.P;
         FLD(F, EU3D_FIELD_H)[cc] = o.H;
         FLD(F, EU3D_FIELD_GAMMA)[cc] = o.Gamma;
-        FLD(F, EU3D_FIELD_C)[cc] = o.C;
+        if (store_c) FLD(F, EU3D_FIELD_C)[cc] = o.C;
         if (cfl) cfl_min(id, o);
     }
     __device__ void store_state(unsigned id, const double (&cs)[NC], const Prim& o) const {
@@ -1282,8 +1286,9 @@ k_trap_first(G3 g, Fields F, uint32_t q, const Scalars* __restrict__ scc, Scalar
 #pragma unroll
             for (int s = 0; s < NC; s++)
                 if (s < nadd) cs0[s] = cs0[s] + R[s * g.n3 + c];
-            const Prim o0 = prim_from_cs_p(cs0, tpow(FLD(F, EU3D_FIELD_T)[c]), FLD(F, EU3D_FIELD_E)[c], q,
-                                           nullptr, NoPre(), &tp);
+            // the old E enters the Newton only under REF quirk F3
+            const double E_old = (q & Q_F3) ? FLD(F, EU3D_FIELD_E)[c] : 0.0;
+            const Prim o0 = prim_from_cs_p(cs0, tpow(FLD(F, EU3D_FIELD_T)[c]), E_old, q, nullptr, NoPre(), &tp);
 #pragma unroll
             for (int s = 0; s < NS; s++) {
                 cell.Di[s] = cs0[s];
@@ -1333,7 +1338,7 @@ k_trap_first(G3 g, Fields F, uint32_t q, const Scalars* __restrict__ scc, Scalar
         if (!predict_only && nch > 0 && !to_net) {
             using Acc = typename std::conditional<ADV, GridCellsStep, GridCells>::type;
             Acc acc{g, F};
-            if constexpr (ADV) acc.src = so.src != 0;
+            if constexpr (ADV) acc.src = acc.store_c = so.src != 0;
             acc.store_src((unsigned)c, ko);
             double cs[NC];
             const Prim o = chem_apply(cell, ko, Dr, dtm / nch, q, cs, tp,
@@ -2150,11 +2155,11 @@ hipError_t launch_flux_fused(eu3d_ctx* c, int diff, int limiter) {
     return e != hipSuccess ? e : launch_flux_xy(c, diff, limiter);
 }
 
-hipError_t launch_imex_fused(eu3d_ctx* c) {
+hipError_t launch_imex_fused(eu3d_ctx* c, bool last) {
     double dx, dy, dz;
     cfl_spacing(c, dx, dy, dz);
     k_imex_fused<<<interior_grid(c->g), dim3(BX, BY), 0, c->stream>>>(G(c), c->fl, c->quirks, c->sc, dx, dy, dz,
-                                                                     c->part);
+                                                                     c->part, last ? 1 : 0);
     return hipGetLastError();
 }
 
@@ -2267,6 +2272,7 @@ hipError_t launch_trap_adaptive(eu3d_ctx* c, int nchem_max, bool predict_only, i
     if (!predict_only && adv) {   // the step: the derived fields are refreshed after its last step
         GridCellsStep acc{c->g, c->fl};
         acc.src = step_src;
+        acc.store_c = step_src;
         acc.cfl = so.part;
         acc.cdx = so.dx;
         acc.cdy = so.dy;
@@ -2424,7 +2430,7 @@ hipError_t launch_step(eu3d_ctx* c, const eu3d_step_params* p, bool first, bool 
     if ((e = launch_flux_fused(c, p->diff, p->limiter)) != hipSuccess) return e;
     switch (p->chem) {
     case EU3D_CHEM_IMEX:
-        e = launch_imex_fused(c);
+        e = launch_imex_fused(c, last);
         c->part_ready = true;
         break;
     case EU3D_CHEM_TRAPEZOID:
