@@ -59,7 +59,7 @@ KERNEL_BYTES_PER_CELL = {
     "flux_xy": (16 + 13 + 13) * 8,
 }
 KERNEL_NAMES = {"trap_first": "k_trap_first<false, true>", "imex_fused": "k_imex_fused",
-                "flux_z": "k_flux_z<8, true>", "flux_xy": "k_flux_xy<16, 16, true, true>"}
+                "flux_z": "k_flux_z<32, true>", "flux_xy": "k_flux_xy<32, 8, true, true>"}
 PROFILES = {"trap_first": "trap_first_det3d_512x256x128_pmc.json",
             "imex_fused": "imex_fused_bubble3d_384x128x128_pmc.json"}
 SURROGATE_THRESHOLD = 4

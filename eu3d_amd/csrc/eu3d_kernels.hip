@@ -2127,25 +2127,29 @@ static void launch_z(eu3d_ctx* c, int diff, int limiter) {
 
 hipError_t launch_flux_z(eu3d_ctx* c, int diff, int limiter) {
     // z planes per column chunk: shorter chunks give more waves to hide latency with, at the cost
-    // of re-reading 3 planes of stencil per chunk (bubble3d 384x128x128, MUSCL_2 + minmod, one
-    // wave per SIMD: 8 planes 0.391 ms, 16 planes 0.401 ms, 32 planes 0.403 ms)
+    // of re-reading 3 planes of stencil per chunk. With the 128-byte-aligned rows (MUSCL_2 +
+    // minmod, one wave per SIMD): detonation 512x256x128 8 planes 0.873 ms, 32 planes 0.806 ms,
+    // 64 planes 0.782 ms; bubble3d 384x128x128 0.327, 0.303, 0.326 ms. (Unaligned rows had
+    // favoured 8 planes: bubble3d 0.391 against 0.403 ms for 32.)
     switch (c->zchunk) {
+    case 8: launch_z<8>(c, diff, limiter); break;
     case 16: launch_z<16>(c, diff, limiter); break;
-    case 32: launch_z<32>(c, diff, limiter); break;
-    default: launch_z<8>(c, diff, limiter);
+    case 64: launch_z<64>(c, diff, limiter); break;
+    default: launch_z<32>(c, diff, limiter);
     }
     return hipGetLastError();
 }
 
 hipError_t launch_flux_xy(eu3d_ctx* c, int diff, int limiter) {
-    // tile shape of the x/y pass (threads in x, y) and the XCD-aware tile order. On bubble3d
-    // 384x128x128: 16x16 with the XCD order 0.671 ms, 32x8 0.693 ms (0.697 with the XCD order),
-    // 64x8 1.185 ms; 64x6, 64x4, 32x4 were slower than 32x8 in an earlier sweep.
+    // tile shape of the x/y pass (threads in x, y) and the XCD-aware tile order. With the
+    // 128-byte-aligned rows: 32x8 1.490 ms against 1.514 ms for 16x16 on the detonation
+    // 512x256x128, 0.550 against 0.558 ms on bubble3d 384x128x128. (With unaligned rows 16x16 had
+    // been faster on bubble3d, 0.671 against 0.697 ms; 64x8 1.185 ms; 64x6, 64x4, 32x4 slower.)
     switch (c->flux_variant) {
-    case 1: launch_xy<32, 8, true>(c, diff, limiter); break;
+    case 1: launch_xy<16, 16, true>(c, diff, limiter); break;
     case 2: launch_xy<64, 8, true>(c, diff, limiter); break;
     case 3: launch_xy<32, 8, false>(c, diff, limiter); break;
-    default: launch_xy<16, 16, true>(c, diff, limiter);
+    default: launch_xy<32, 8, true>(c, diff, limiter);
     }
     return hipGetLastError();
 }
