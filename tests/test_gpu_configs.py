@@ -58,10 +58,19 @@ def test_batch_equilibrium_states_vs_oracle():
     nch = torch.empty(n, dtype=torch.int32, device="cuda")
     g.chem_batch(n, r.data_ptr(), t.data_ptr(), e.data_ptr(), 1e-8, 10000, nch.data_ptr())
     nch = nch.cpu().numpy()
+    # cells the adaptive trapezoid takes out of the finite range: the oracle does the same on the
+    # same inputs (with the same sub-step counts), so the GPU reproduces where the method fails
+    bad_all = np.flatnonzero(~np.isfinite(t.cpu().numpy()) | ~np.isfinite(r.cpu().numpy()).all(0))
+    bad = bad_all[:64]
+    say("  non-finite cells after the step:", len(bad_all))
+    if len(bad):
+        _, _, ro_b, To_b, nch_b = oracle_batch(rho[bad], T[bad], E[bad], 1e-8)
+        assert (~np.isfinite(To_b) | ~np.isfinite(ro_b).all(1)).all()
+        assert (nch_b == nch[bad]).all()
     order = np.argsort(-nch, kind="stable")
     heavy = order[:48]
     rng = np.random.default_rng(5)
-    rest = rng.choice(order[48:], 2000, replace=False)
+    rest = rng.choice(np.setdiff1d(order[48:], bad_all), 2000, replace=False)
     sel = np.concatenate([heavy, rest])
     say("  sub-steps of the heavy cells", nch[heavy].min(), "..", nch[heavy].max(),
         "cells at the cap in 2^18:", int((nch == 10000).sum()))
