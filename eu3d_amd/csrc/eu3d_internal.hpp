@@ -9,16 +9,15 @@
 
 namespace eu3d {
 
-// Ghost-inclusive grid extents; device fields are SoA with x fastest:
-// field[c * n3 + (k * NY + j) * NX + i].
-// Grid of one context. NX, NY, NZ are the ghost-inclusive extents (ni + 2 bc, ...). In memory a
-// row of NX cells sits in a slot of PX >= NX doubles (a multiple of 16) at offset OX = 16 - bc,
-// so that the first interior cell of every row, and with it every 64-cell wavefront of a row,
-// starts on a 128-byte line: wave accesses touch 4 lines, not 5 (a 16-in / 13-out FP64 stream
-// ran at 5.4 against 3.6 TB/s). The right ghosts of a row end before the next row's left
-// ghosts begin (PX > NX - 1 + ... : OX + NX <= PX + OX). n3 (a multiple of 32 doubles, so that
-// every component of a field starts on a 256-byte boundary) is the per-component stride of the
-// fields; ncell = NX NY NZ is the number of cells with ghosts.
+// Grid of one context. NX, NY, NZ are the ghost-inclusive extents (ni + 2 bc, ...). Device fields
+// are SoA with x fastest: component c of cell (i, j, k) is field[c * n3 + IDX(g, i, j, k)]. A row
+// of NX cells sits in a slot of PX doubles (NX rounded up to 16) at offset OX = 16 - bc, so the
+// first interior cell of every row, and with it every 64-cell wavefront of a row, starts a
+// 128-byte line: a wavefront's access touches 4 lines, not 5 (a 16-in / 13-out FP64 stream ran
+// at 5.4 against 3.6 TB/s). A row's last ghost (slot offset OX + NX - 1 <= PX + OX - 1) ends
+// before the next row's first ghost (offset PX + OX). n3, the per-component stride, is a multiple
+// of 32 doubles so that every component starts on 256 bytes; ncell = NX NY NZ is the number of
+// cells with ghosts (the reference Array's size).
 struct G3 {
     int ni, nj, nk, bc, NX, NY, NZ, PX, OX;
     size_t n3, ncell;
