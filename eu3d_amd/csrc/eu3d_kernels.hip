@@ -355,12 +355,18 @@ __device__ __forceinline__ GhostCell gc_after_xy(const Fields& F, const G3& g, c
     return gc_after_x(F, g, K, i, j, k);
 }
 
+// The z-ghost planes and y-ghost rows are enumerated PX + 16 threads a row, thread q of a row
+// holding i = q - OX at slot offset q (a row's cells span slot offsets OX .. OX + NX - 1 < PX + 16),
+// so that 16 consecutive threads are one aligned 128-byte line; threads outside the row return.
+__device__ __forceinline__ long long fill_row_threads(const G3& g) { return (long long)g.PX + 16; }
 __global__ void __launch_bounds__(256) k_fill_all(G3 g, Fields F, GhostKinds K, uint32_t q) {
-    const long long nR1 = (long long)g.NX * g.NY * (2 * g.bc), nR2 = (long long)g.NX * (2 * g.bc) * g.nk;
+    const long long RT = fill_row_threads(g);
+    const long long nR1 = RT * g.NY * (2 * g.bc), nR2 = RT * (2 * g.bc) * g.nk;
     const long long nR3 = (long long)(2 * g.bc) * g.nj * g.nk;
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < nR1) {   // z-ghost planes, all (i, j)
-        const int i = (int)(t % g.NX), j = (int)((t / g.NX) % g.NY), kg = (int)(t / ((long long)g.NX * g.NY));
+        const int i = (int)(t % RT) - g.OX, j = (int)((t / RT) % g.NY), kg = (int)(t / (RT * g.NY));
+        if (i < 0 || i >= g.NX) return;
         const int k = kg < g.bc ? kg : g.nk + kg;
         int kind;
         const int sk = gsrc(k, g.nk, g.bc, K.k[4], K.k[5], kind);
@@ -372,7 +378,9 @@ __global__ void __launch_bounds__(256) k_fill_all(G3 g, Fields F, GhostKinds K, 
     }
     t -= nR1;
     if (t < nR2) {   // y-ghost rows, interior k, all i
-        const int i = (int)(t % g.NX), jg = (int)((t / g.NX) % (2 * g.bc)), k = g.bc + (int)(t / ((long long)g.NX * 2 * g.bc));
+        const int i = (int)(t % RT) - g.OX, jg = (int)((t / RT) % (2 * g.bc));
+        const int k = g.bc + (int)(t / (RT * 2 * g.bc));
+        if (i < 0 || i >= g.NX) return;
         const int j = jg < g.bc ? jg : g.nj + jg;
         int kind;
         const int sj = gsrc(j, g.nj, g.bc, K.k[2], K.k[3], kind);
@@ -2017,8 +2025,9 @@ hipError_t launch_fill_ghosts(eu3d_ctx* c, const int kinds[6]) {
     if (g.ni >= g.bc && g.nj >= g.bc && g.nk >= g.bc) {   // else the three passes (a source may be a ghost)
         GhostKinds K;
         for (int f = 0; f < 6; f++) K.k[f] = kinds[f];
-        const long long n = (long long)g.NX * g.NY * (2 * g.bc) + (long long)g.NX * (2 * g.bc) * g.nk +
-                            (long long)(2 * g.bc) * g.nj * g.nk;
+        // the thread count of k_fill_all's three ranges (z planes and y rows PX + 16 threads a row)
+        const long long rt = (long long)g.PX + 16;
+        const long long n = rt * g.NY * (2 * g.bc) + rt * (2 * g.bc) * g.nk + (long long)(2 * g.bc) * g.nj * g.nk;
         k_fill_all<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(g, c->fl, K, c->quirks);
         return hipGetLastError();
     }
