@@ -20,7 +20,8 @@ The line carries:
   and the fraction of cells through the network and rejected;
 - roofline: the dominant kernel of the plain step (k_trap_first), timed live with HIP events on
   the context's stream;
-- kernels, bubble3d_imex (the 3-D shock / H2-bubble IMEX configuration), batched_chemistry,
+- kernels, zn1d_rk2 (the 1-D detonation tube, 2000 steps, with its CPU rate beside it),
+  bubble3d_imex (the 3-D shock / H2-bubble IMEX configuration), batched_chemistry,
   surrogate_network, surrogate_stiff_chemistry (the hybrid against plain FP64 chemistry on the
   stiff batched states, where the network takes cells): further sections;
 - cpu_baseline: the CPU restatement of the reference arithmetic (the oracle) on a bounded sample
@@ -354,6 +355,51 @@ def bench_bubble3d(steps=20, warmup=3):
     return out
 
 
+def bench_zn1d(nsteps=2000, cpu_steps=100, cpu=True):
+    """The 1-D H2-air detonation as stated: 2000 cells of 5e-5 m, 40-cell driver at 30 atm / 3000 K,
+    SSP-RK2, adaptive trapezoid chemistry, the 20-step CFL ramp to 0.5 (one call each), then the
+    remaining steps in one eu3d_step call, timed with HIP events; beside it the CPU restatement
+    (OpenMP over cells on the host cores) timed over the first cpu_steps steps after the ramp."""
+    import torch
+    from eu3d_amd import configs
+    from eu3d_amd.api import Context
+    wl = configs.zn1d()
+    kw = dict(bc_kinds=wl.bc_kinds, rk=2, chem=3, nchem_max=10000)
+    g = Context(wl.xnode, wl.ynode, wl.znode, quirks=0)
+    load_state(g, wl)
+    for s in range(RAMP_STEPS):
+        g.step(1, cfl=configs.cfl_ramp(s), skip_derive=True, **kw)
+    g.sync()
+    ev = events(g.stream())
+    torch.cuda.synchronize()
+    t0 = ev()
+    g.step(nsteps - RAMP_STEPS, cfl=0.5, **kw)
+    t1 = ev()
+    g.sync()
+    ms = t0.elapsed_time(t1) / (nsteps - RAMP_STEPS)
+    out = {"workload": "zn1d 2000 cells (1-D H2-air detonation, RK2, adaptive trapezoid, CFL 0.5 after the ramp)",
+           "cells": wl.ncells, "steps": nsteps, "timed_steps": f"{RAMP_STEPS + 1}..{nsteps}", "ms_per_step": ms,
+           "cells_per_s": wl.ncells / (ms * 1e-3), "finite": g.check_finite() == 0, "time_s": g.get_time()[1]}
+    g.close()
+    if not cpu:
+        return out
+    import oracle
+    cores = host_cores()
+    oracle.set_threads(cores)
+    o = oracle.OracleFlow(wl.xnode, wl.ynode, wl.znode, quirks=0)
+    for k in ("T", "P", "U", "V", "W", "YI"):
+        o.set(k, wl.prims[k])
+    o.field_initial()
+    for s in range(RAMP_STEPS):
+        o.step(1, cfl=configs.cfl_ramp(s), **kw)
+    c0 = time.perf_counter()
+    o.step(cpu_steps, cfl=0.5, **kw)
+    el = time.perf_counter() - c0
+    out["cpu_baseline"] = {"value": wl.ncells * cpu_steps / el, "unit": "cell-updates/s", "cores": cores,
+                           "kind": "port", "sample": f"steps {RAMP_STEPS + 1}..{RAMP_STEPS + cpu_steps}, {el:.2f} s"}
+    return out
+
+
 def bench_batched_chemistry(n=1 << 22, dtm=1e-8, seed=2024):
     """Batched FP64 chemistry: 2^22 seeded cells, one adaptive trapezoid step of dt = 1e-8 s,
     with the persistent kernel's schedule from the autotuner, for both burnt compositions."""
@@ -542,6 +588,8 @@ def main():
     g.close()
 
     if not a.no_extra:
+        log("zn1d")
+        res["zn1d_rk2"] = bench_zn1d(cpu=not a.no_cpu)
         log("bubble3d")
         res["bubble3d_imex"] = bench_bubble3d()
         log("batched chemistry")
