@@ -676,7 +676,9 @@ void Euler::Output(int num) {
         return;
     }
     const std::string dir = outdir + "/output_" + std::to_string(nb);
-    std::filesystem::create_directories(dir);
+    std::error_code ec;
+    std::filesystem::create_directories(dir, ec);
+    if (ec) throw Error(EU3D_ERR_ARG, "cannot create " + dir + ": " + ec.message());
     OutputBlock b;
     b.ni = (mesh.ni - 1) / bx + 1;
     b.nj = (mesh.nj - 1) / by + 1;
