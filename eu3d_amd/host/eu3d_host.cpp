@@ -29,6 +29,12 @@ std::string six_digits(double v) {
     ss << v;
     return ss.str();
 }
+// create a directory and its parents; failure is an Error like the other output failures
+void make_dirs(const std::string& dir) {
+    std::error_code ec;
+    std::filesystem::create_directories(dir, ec);
+    if (ec) throw Error(EU3D_ERR_ARG, "cannot create " + dir + ": " + ec.message());
+}
 }  // namespace
 
 bool FileReader::readFile(const std::string& name) {
@@ -606,10 +612,9 @@ void Euler::ComputingFused(int nsteps, const int bc_kinds[6], int rk, int nchem_
 // src/Euler.cpp:984-1021: the phase times of the run, and the loop time for comparing runs.
 void Euler::Output_Total_Time(double file_reader, double mesh_t, double input, double reaction_read,
                               double field_initial, double computing, double output) const {
-    namespace fs = std::filesystem;
     const std::string base = outdir + "/output_1/time_record/";
-    fs::create_directories(base + "total");
-    fs::create_directories(base + "compare");
+    make_dirs(base + "total");
+    make_dirs(base + "compare");
     std::ofstream t(base + "total/Total_Time_0.txt");
     t << "myid:" << '\t' << 0 << '\n';
     t << "FileReader:" << '\t' << file_reader << '\n';
@@ -629,10 +634,9 @@ void Euler::Output_Total_Time(double file_reader, double mesh_t, double input, d
 
 // src/Euler.cpp:1022-1099: the loop's phases, per reaction scheme, and the per-rank files.
 void Euler::Output_Compute_Time() const {
-    namespace fs = std::filesystem;
     const std::string base = outdir + "/output_1/time_record/";
-    fs::create_directories(base + "compute");
-    fs::create_directories(base + "all");
+    make_dirs(base + "compute");
+    make_dirs(base + "all");
     std::ofstream o(base + "compute/Compute_Time_0.txt");
     o << "myid:" << '\t' << 0 << '\n';
     o << "FieldBoundary:" << '\t' << phase.boundary << '\n';
@@ -676,9 +680,7 @@ void Euler::Output(int num) {
         return;
     }
     const std::string dir = outdir + "/output_" + std::to_string(nb);
-    std::error_code ec;
-    std::filesystem::create_directories(dir, ec);
-    if (ec) throw Error(EU3D_ERR_ARG, "cannot create " + dir + ": " + ec.message());
+    make_dirs(dir);
     OutputBlock b;
     b.ni = (mesh.ni - 1) / bx + 1;
     b.nj = (mesh.nj - 1) / by + 1;
