@@ -35,7 +35,7 @@ __host__ __device__ __forceinline__ float bf2f(bf16_t h) {
 // GELU in its tanh form, 0.5 z (1 + tanh(sqrt(2/pi) (z + 0.044715 z^3))), evaluated as the equal
 // z * sigmoid(s) with s = 2 sqrt(2/pi) (z + 0.044715 z^3): one exp2 and one reciprocal per
 // element, so the epilogue of the fused network kernel stays short next to its MFMA work.
-// Training, inference and the CPU checker all use this form.
+// Training, inference and the CPU oracle (oracle/mlp.py) all use this form.
 constexpr float GELU_C1 = 1.5957691216057308f;                 // 2 sqrt(2/pi)
 constexpr float GELU_C2 = GELU_C1 * 0.044715f;
 constexpr float LOG2E = 1.4426950408889634f;

@@ -1,7 +1,7 @@
 """CPU oracle (TEST INFRASTRUCTURE ONLY): ctypes wrapper of oracle/liboracle.so.
 
-Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this package, as
-the checker. The product (eu3d_amd) never imports it.
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this package, to
+check the HIP path against it. The product (eu3d_amd) never imports it.
 
 ``OracleFlow`` mirrors ``eu3d_amd.api.Context`` method for method so a parity test runs the
 same sequence of reference calls on both and compares the fields.
@@ -15,7 +15,7 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-# EU3D_ORACLE_LIB points the checker at another build of oracle.c (oracle/check_mutants.py uses it
+# EU3D_ORACLE_LIB points the oracle at another build of oracle.c (oracle/check_mutants.py uses it
 # to load deliberately broken copies and confirm the pins fail).
 LIB_PATH = os.environ.get("EU3D_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 

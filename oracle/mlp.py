@@ -1,6 +1,6 @@
 """FP64 CPU restatement of the chemistry-surrogate MLP (TEST INFRASTRUCTURE ONLY).
 
-The checker for the HIP network: forward with GELU (tanh form) and hand-written backward in
+The oracle of the HIP network: forward with GELU (tanh form) and hand-written backward in
 float64 numpy, reading the same parameter vector layout as eu3d_mlp_get_params (per layer
 W^T [dout x din] then bias [dout]; padded inputs 32, hidden 256, padded outputs 16). The
 surrogate has no counterpart in the reference, so its pins are this restatement, finite

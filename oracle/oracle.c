@@ -1,7 +1,7 @@
 /* CPU oracle: FP64 restatement of the EU3D reference hot path (TEST INFRASTRUCTURE ONLY).
  *
  * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load this file's
- * library, and only as the checker: the product path never calls it.
+ * library, and only to check the HIP path: the product path never calls it.
  *
  * Each function restates the reference line by line, keeping the reference's loop ranges,
  * evaluation order (left-to-right sums, pow() calls) and scratch-array semantics, because

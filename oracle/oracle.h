@@ -1,6 +1,6 @@
 /* CPU oracle: FP64 restatement of the EU3D reference hot path.
  *
- * TEST INFRASTRUCTURE ONLY. This library is the checker for the HIP path: only tests/,
+ * TEST INFRASTRUCTURE ONLY. This library checks the HIP path: only tests/,
  * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it. The product library
  * (eu3d_amd/libeu3d.so) never links, loads or calls it.
  *

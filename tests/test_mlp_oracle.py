@@ -1,4 +1,4 @@
-"""Pins for the FP64 MLP checker (oracle/mlp.py): finite differences of its loss."""
+"""Pins for the FP64 MLP oracle (oracle/mlp.py): finite differences of its loss."""
 import numpy as np
 
 from oracle import mlp as om

@@ -1,4 +1,4 @@
-"""Pins for the CPU oracle (the checker of the HIP path).
+"""Pins for the CPU oracle (the reference check of the HIP path).
 
 The reference ships no tests and its Flowfield/Reaction translation units need <mpi.h>, which
 this image lacks, so the oracle is pinned by: the reference's own Differ.hpp/TimeAdv.cpp/
