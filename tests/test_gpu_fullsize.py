@@ -17,6 +17,7 @@ of the adaptive trapezoid exactly (Euler::Computing, src/Euler.cpp:266-333; the 
 src/Reaction.cpp:489-670 and :736-811)."""
 import os
 
+import numpy as np
 import pytest
 
 from tests.common import compare, init_state, say
@@ -37,11 +38,12 @@ def _hand_over(g, o):
     o.set_time(dt, t)
 
 
-def _oracle_like(wl):
+def _oracle(xnode, ynode, znode):
+    """An oracle flow on the host cores (at most 16 threads), FIXED quirks."""
     import oracle
     cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     oracle.set_threads(max(1, min(16, cores, int(os.environ.get("OMP_NUM_THREADS", cores)))))
-    return oracle.OracleFlow(wl.xnode, wl.ynode, wl.znode, quirks=0)
+    return oracle.OracleFlow(xnode, ynode, znode, quirks=0)
 
 
 def test_det3d_full_size_step_vs_oracle():
@@ -57,7 +59,7 @@ def test_det3d_full_size_step_vs_oracle():
     for s in range(110):
         g.step(1, cfl=configs.cfl_ramp(s), skip_derive=s < 109, **kw)
     assert g.check_finite() == 0
-    o = _oracle_like(wl)
+    o = _oracle(wl.xnode, wl.ynode, wl.znode)
     _hand_over(g, o)
     say("  state handed over; stepping")
     g.step(1, cfl=0.5, **kw)
@@ -84,7 +86,7 @@ def test_bubble3d_full_size_steps_vs_oracle():
     kw = dict(cfl=0.5, bc_kinds=wl.bc_kinds, rk=1, chem=1)
     g.step(10, **kw)
     assert g.check_finite() == 0
-    o = _oracle_like(wl)
+    o = _oracle(wl.xnode, wl.ynode, wl.znode)
     _hand_over(g, o)
     say("  state handed over; stepping")
     g.step(2, **kw)
@@ -93,4 +95,77 @@ def test_bubble3d_full_size_steps_vs_oracle():
     assert g.get_time() == pytest.approx(o.get_time(), rel=1e-13)
     worst = compare(g, o, ["CS", "T", "D", "P", "E", "YI"])
     say("  worst relative differences:", {k: f"{v:.2e}" for k, v in worst.items()})
+    g.close()
+
+
+def _oracle_batch_grid(rho, T, E, dtm, grid=(256, 128, 128)):
+    """The oracle's adaptive trapezoid (the Euler::Trapezoid(1) cell loop, src/Euler.cpp:526-556)
+    on cells at rest laid out in the interior of a grid (ghost cells hold a copy of the first cell
+    and are not integrated): the state after Update_after_CS (T, E), then after the step (rho_s, T,
+    sub-step counts), flat in cell order."""
+    from eu3d_amd import configs
+    ni, nj, nk = grid
+    assert ni * nj * nk == T.shape[0]
+    o = _oracle(configs.make_nodes(ni, 1e-4), configs.make_nodes(nj, 1e-4), configs.make_nodes(nk, 1e-4))
+    sh = o.shape3
+    cs = np.zeros(sh + (13,))
+    cs[..., 0:9] = rho[0]
+    cs[..., 12] = rho[0].sum() * E[0]
+    cs[2:-2, 2:-2, 2:-2, :9] = rho.reshape(ni, nj, nk, 9)
+    cs[2:-2, 2:-2, 2:-2, 12] = (rho.sum(1) * E).reshape(ni, nj, nk)
+    Tg = np.full(sh, T[0])
+    Tg[2:-2, 2:-2, 2:-2] = T.reshape(ni, nj, nk)
+    o.set("CS", cs)
+    o.set("T", Tg)
+    del cs, Tg
+    o.update_after_cs()
+    flat = lambda a: a[2:-2, 2:-2, 2:-2].reshape((ni * nj * nk,) + a.shape[3:])
+    T_in, E_in = flat(o.get("T")), flat(o.get("E"))
+    o.set_time(dtm)
+    o.trapezoid_adaptive(10000)
+    return T_in, E_in, flat(o.get("CS"))[:, :9], flat(o.get("T")), flat(o.get("NCHEM")).astype(np.int64)
+
+
+def test_batch_equilibrium_full_size_vs_oracle():
+    """The batched-chemistry workload at its full size: all 2^22 equilibrium-line states (seed
+    2024), one adaptive-trapezoid step of dt = 1e-8 s (src/Reaction.cpp:489-670), every cell
+    against the oracle: sub-step counts bit for bit, the same cells leaving the finite range, the
+    others within 1e-10 of the largest total density in rho_s and within 1e-10 x 3000 K in T, or
+    four times the oracle's own response to a 1-ulp change of its inputs where that is larger (the
+    longest chains run 10000 dependent sub-steps)."""
+    import torch
+    from eu3d_amd import chemistry, configs
+    from eu3d_amd.api import Context
+    assert torch.cuda.is_available()
+    say("case batched chemistry, 2^22 equilibrium-line states, every cell")
+    x = configs.make_nodes(8, 1e-4)
+    g = Context(x, x, x, quirks=0)
+    n = 1 << 22
+    dev, (rho, T, E) = chemistry.states(g, n, seed=2024, burnt="equilibrium")
+    del dev
+    T_in, E_in, ro, To, nch_o = _oracle_batch_grid(rho, T, E, 1e-8)
+    say("  oracle done:", int((nch_o > 1).sum()), "cells sub-stepped, largest", int(nch_o.max()))
+    rd = torch.tensor(rho.T.copy(), device="cuda")
+    td = torch.tensor(T_in, device="cuda")
+    ed = torch.tensor(E_in, device="cuda")
+    nd = torch.empty(n, dtype=torch.int32, device="cuda")
+    st = g.chem_batch(n, rd.data_ptr(), td.data_ptr(), ed.data_ptr(), 1e-8, 10000, nd.data_ptr())
+    torch.cuda.synchronize()
+    nch = nd.cpu().numpy().astype(np.int64)
+    assert (nch == nch_o).all(), f"{int((nch != nch_o).sum())} cells differ in Nchem"
+    assert st["substeps_total"] == int(nch_o.sum()) and st["nchem_max_seen"] == int(nch_o.max())
+    rg, tg = rd.cpu().numpy().T, td.cpu().numpy()
+    fin_o = np.isfinite(ro).all(1) & np.isfinite(To)
+    fin_g = np.isfinite(rg).all(1) & np.isfinite(tg)
+    assert (fin_o == fin_g).all(), f"{int((fin_o != fin_g).sum())} cells differ in finiteness"
+    scale = ro[fin_o].sum(1).max()
+    err = np.abs(rg[fin_o] - ro[fin_o]).max() / scale
+    terr = np.abs(tg[fin_o] - To[fin_o]).max()
+    _, _, ro_p, To_p, _ = _oracle_batch_grid(rho * (1 + 2.0 ** -52), T, E, 1e-8)
+    both = fin_o & np.isfinite(ro_p).all(1) & np.isfinite(To_p)
+    cond = np.abs(ro_p[both] - ro[both]).max() / scale
+    tcond = np.abs(To_p[both] - To[both]).max()
+    say(f"  Nchem equal; {int((~fin_o).sum())} cells non-finite in both; rel err {err:.2e} "
+        f"(1-ulp response {cond:.2e}), T err {terr:.2e} K (1-ulp response {tcond:.2e} K)")
+    assert err < max(1e-10, 4 * cond) and terr < max(1e-10 * 3000, 4 * tcond)
     g.close()
