@@ -23,17 +23,20 @@ def _t():
     return torch
 
 
-def test_zn1d_2000_as_stated_100_steps():
+def test_zn1d_2000_as_stated_250_steps():
     """The 1-D detonation configuration exactly as stated (2000 cells of 5e-5 m, driver zone of 40
-    cells at 30 atm and 3000 K, RK2, adaptive trapezoid, CFL ramp to 0.5) for 100 steps."""
+    cells at 30 atm and 3000 K, RK2, adaptive trapezoid, CFL ramp to 0.5) for 250 steps. Beyond
+    that the flow amplifies rounding: two oracle runs, one from a 1-ulp change of the initial
+    densities, differ by 1.5e-13 after 250 steps, 2e-9 after 500 and 1e-3 after 1500, so the
+    configuration's 2000 steps are compared over their well-conditioned start."""
     from eu3d_amd import configs
-    say("case zn1d 2000 as stated, 100 steps")
+    say("case zn1d 2000 as stated, 250 steps")
     wl = configs.zn1d()
     assert wl.ncells == 2000 and (wl.prims["T"][2:-2, 2, 2] == 3000.0).sum() == 40
     g, o = make_pair(wl, Q_FIXED)
-    for s in range(100):
+    for s in range(250):
         kw = dict(cfl=configs.cfl_ramp(s), bc_kinds=wl.bc_kinds, rk=2, chem=3, nchem_max=10000)
-        g.step(1, skip_derive=s < 99, **kw)
+        g.step(1, skip_derive=s < 249, **kw)
         o.step(1, **kw)
     assert g.check_finite() == 0
     dg, do = g.get_time(), o.get_time()
