@@ -568,10 +568,17 @@ __device__ __forceinline__ void kinetics_g(const TPow& p, const double (&gx)[NS]
         if (TRAP) { o.P[s] = 0.0; o.Q[s] = 0.0; }
         if (DIAG) o.md[s] = 0.0;
     }
-    double rDr[NS];   // 1 / Dr, or 0 where Dr = 0 (those species contribute no Jacobian term)
+    // x / Dr as (x * rDr) * sDr: rDr = 1 / (Dr sDr), or 0 where Dr = 0 (those species contribute
+    // no Jacobian term). sDr is 1, or 2^64 for |Dr| below 2^-1000, whose own reciprocal would
+    // overflow (a trace species at a denormal density, where the reference's quotient is finite).
+    // Scaling by powers of two is exact, so for every other Dr the value is x * (1 / Dr).
+    double rDr[NS], sDr[NS];
     if (DIAG) {
 #pragma unroll
-        for (int s = 0; s < NS; s++) rDr[s] = (Dr[s] != 0) ? 1.0 / Dr[s] : 0.0;
+        for (int s = 0; s < NS; s++) {
+            sDr[s] = fabs(Dr[s]) < 0x1p-1000 ? 0x1p64 : 1.0;
+            rDr[s] = (Dr[s] != 0) ? 1.0 / (Dr[s] * sDr[s]) : 0.0;
+        }
     }
     const double RT = R_UNIV * T;
     constexpr double P0R = P0_PA * 1e-6;
@@ -628,8 +635,8 @@ __device__ __forceinline__ void kinetics_g(const TPow& p, const double (&gx)[NS]
                 }
             }
             if (DIAG && dn != 0) {
-                const double a = nf ? (kf * (double)nf * rf) * rDr[s] : 0.0;
-                const double b = nb ? (kb * (double)nb * rb) * rDr[s] : 0.0;
+                const double a = nf ? ((kf * (double)nf * rf) * rDr[s]) * sDr[s] : 0.0;
+                const double b = nb ? ((kb * (double)nb * rb) * rDr[s]) * sDr[s] : 0.0;
                 const double w1 = a - b;
                 double term = (w1 + dRdT * pts[s]) * tb;
                 if (TB[s][r] != 0.0) term += rr * (TB[s][r] / MW[s]);
