@@ -8,7 +8,7 @@
 - The surrogate chemistry on the grid with cells the network accepts: element conservation, the
   |dT| bound, and every rejected or non-network cell equal to the FP64 step.
 - The stated step counts at full size where no restatement can follow them: the shock-bubble's
-  500 and a detonation section's 1000 steps stay finite and their fronts advance.
+  500 and the detonation's 1000 steps stay finite and their fronts advance.
 """
 import numpy as np
 import pytest
@@ -229,15 +229,15 @@ def _grid_step_check(wl, m, p, mech, expect_rejects):
 
 
 def test_det3d_1000_steps_finite():
-    """The planar detonation on a 512 x 64 x 64 section (the same generator) for the configuration's
-    1000 steps with the FIXED unsplit CFL bound: every interior cell stays finite (with the
-    per-direction bound one cell's internal energy fell below the mixture's energy at 300 K at step
-    180), and the leading front advances every 100 steps."""
+    """The planar detonation as stated (512 x 256 x 128) for its 1000 steps with the FIXED unsplit
+    CFL bound: every interior cell stays finite (with the per-direction bound one cell's internal
+    energy fell below the mixture's energy at 300 K at step 180), and the leading front advances
+    every 100 steps."""
     from eu3d_amd import configs
     from eu3d_amd.api import Context
     from tests.common import init_state
-    say("case det3d 512x64x64, 1000 steps")
-    wl = configs.det3d(512, 64, 64)
+    say("case det3d 512x256x128, 1000 steps")
+    wl = configs.det3d(512, 256, 128)
     g = Context(wl.xnode, wl.ynode, wl.znode, quirks=Q_FIXED)
     init_state(g, wl)
     kw = dict(bc_kinds=wl.bc_kinds, rk=1, chem=3, nchem_max=10000)
