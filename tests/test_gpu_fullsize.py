@@ -126,9 +126,11 @@ def _oracle_batch_grid(rho, T, E, dtm, grid=(256, 128, 128)):
     return T_in, E_in, flat(o.get("CS"))[:, :9], flat(o.get("T")), flat(o.get("NCHEM")).astype(np.int64)
 
 
-def test_batch_equilibrium_full_size_vs_oracle():
-    """The batched-chemistry workload at its full size: all 2^22 equilibrium-line states (seed
-    2024), one adaptive-trapezoid step of dt = 1e-8 s (src/Reaction.cpp:489-670), every cell
+@pytest.mark.parametrize("burnt", ["equilibrium", "complete"])
+def test_batch_full_size_vs_oracle(burnt):
+    """The batched-chemistry workload at its full size: all 2^22 states (seed 2024) of either
+    family, equilibrium-line or complete-combustion products, one adaptive-trapezoid step of
+    dt = 1e-8 s (src/Reaction.cpp:489-670), every cell
     against the oracle: sub-step counts bit for bit, the same cells leaving the finite range, the
     others within 1e-10 of the largest total density in rho_s and within 1e-10 x 3000 K in T, or
     four times the oracle's own response to a 1-ulp change of its inputs where that is larger (the
@@ -137,11 +139,11 @@ def test_batch_equilibrium_full_size_vs_oracle():
     from eu3d_amd import chemistry, configs
     from eu3d_amd.api import Context
     assert torch.cuda.is_available()
-    say("case batched chemistry, 2^22 equilibrium-line states, every cell")
+    say(f"case batched chemistry, 2^22 {burnt} states, every cell")
     x = configs.make_nodes(8, 1e-4)
     g = Context(x, x, x, quirks=0)
     n = 1 << 22
-    dev, (rho, T, E) = chemistry.states(g, n, seed=2024, burnt="equilibrium")
+    dev, (rho, T, E) = chemistry.states(g, n, seed=2024, burnt=burnt)
     del dev
     T_in, E_in, ro, To, nch_o = _oracle_batch_grid(rho, T, E, 1e-8)
     say("  oracle done:", int((nch_o > 1).sum()), "cells sub-stepped, largest", int(nch_o.max()))
