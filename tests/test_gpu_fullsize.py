@@ -171,3 +171,38 @@ def test_batch_full_size_vs_oracle(burnt):
         f"(1-ulp response {cond:.2e}), T err {terr:.2e} K (1-ulp response {tcond:.2e} K)")
     assert err < max(1e-10, 4 * cond) and terr < max(1e-10 * 3000, 4 * tcond)
     g.close()
+
+
+def test_network_forward_full_batch_vs_oracle():
+    """The surrogate network's inference at its stated batch, 2^20 rows (12 -> 4 x 256 GELU -> 9,
+    the fused BF16-MFMA kernel over 8,192 row tiles of 128), against the FP64 restatement in
+    oracle/mlp.py with and without the BF16 rounding of weights and activations, at the bounds of
+    the 3000-row check. The restatement runs on one row in 16 (row 16 k + k mod 16: eight rows
+    of every tile, every position within a tile), which keeps it to seconds."""
+    import torch
+    from eu3d_amd.api import MLP
+    from oracle import mlp as om
+    from tests.test_gpu_mlp import FWD_TOL_BF16, FWD_TOL_FP64, _random_params, bf16_bits
+    assert torch.cuda.is_available()
+    say("case network forward, batch 2^20")
+    m = MLP(seed=3)
+    p = _random_params(m, 1)
+    rng = np.random.default_rng(12)
+    n = 1 << 20
+    X = np.zeros((n, 32))
+    X[:, :12] = rng.normal(size=(n, 12))
+    out = torch.empty((n, 16), dtype=torch.float32, device="cuda")
+    m.forward(bf16_bits(X), out)
+    torch.cuda.synchronize()
+    Y = out.cpu().numpy().astype(np.float64)
+    k = np.arange(n // 16)
+    rows = 16 * k + k % 16
+    Yb, _ = om.forward(p, X[rows], bf16=True)
+    Yf, _ = om.forward(p, X[rows], bf16=False)
+    sc = np.abs(Yf[:, :9]).max()
+    eb = np.abs(Y[rows, :9] - Yb[:, :9]).max() / sc
+    ef = np.abs(Y[rows, :9] - Yf[:, :9]).max() / sc
+    say(f"  max error / scale: vs BF16-rounded FP64 {eb:.2e}, vs FP64 {ef:.2e}")
+    assert eb <= FWD_TOL_BF16 and ef <= FWD_TOL_FP64
+    assert np.abs(Y[:, 9:]).max() == 0.0
+    m.close()
