@@ -20,8 +20,10 @@ Two workloads, each a JSON report:
 
 - stiff-gain: what the trained network gains on the stiff batched chemistry (stiff_gain), the
   bench's and the gain test's measurement.
+- train-at-scale: the network trained on about 1e7 pairs harvested from the 1-D tube, the
+  shock-bubble and the batched chemistry (train_at_scale).
 
-Usage: python -m eu3d_amd.surrogate_eval batch|detonation|plain-detonation|stiff-gain [options]
+Usage: python -m eu3d_amd.surrogate_eval batch|detonation|plain-detonation|stiff-gain|train-at-scale [options]
 """
 from __future__ import annotations
 
@@ -339,9 +341,95 @@ def time_plain_detonation(grid=(512, 256, 128), nsteps=140, window=40, log=print
     return out
 
 
+def train_at_scale(zn_steps=2000, bub_steps=500, bub_every=50, bub_cells=1 << 19, batch_seeds=(401, 402, 403),
+                   n_batch=1 << 20, epochs=30, batch=1 << 16, holdout=1 << 16, log=print):
+    """The network trained at the scale of the surrogate's configuration: about 1e7 (state,
+    increment) pairs harvested from the solver's FP64 adaptive trapezoid on the other three
+    workloads, trained by the GPU kernels (fused forward, backward, Adam).
+
+    Sources: the 1-D detonation tube (2000 cells, a snapshot after every one of its `zn_steps`
+    steps, cells at 600 K and above); the shock-bubble (384 x 128 x 128, every `bub_every`-th of
+    its `bub_steps` steps, `bub_cells` cells a snapshot, every temperature); the batched chemistry
+    (`n_batch` equilibrium-line states of each seed in `batch_seeds`, one step of dt = 1e-8 s).
+    A snapshot's pairs advance the interior state after the step's flow and chemistry update by
+    the step's dt. `holdout` random pairs are kept out of training; the report gives the pairs
+    per source, the training loss per epoch, the held-out loss before and after, and the time."""
+    torch = _torch()
+    t_all = time.time()
+    H = surrogate.Harvest()
+    counts = {}
+    t = time.time()
+    surrogate.harvest_run(H, configs.zn1d(), zn_steps, 1, max_cells=1 << 20)
+    counts["zn1d_2000"] = H.n
+    log(f"harvest zn1d: {H.n} pairs ({time.time() - t:.1f} s)")
+    t = time.time()
+    surrogate.harvest_run(H, configs.bubble3d(384, 128, 128), bub_steps, bub_every, min_T=0.0, max_cells=bub_cells)
+    counts["bubble3d_384x128x128"] = H.n - sum(counts.values())
+    log(f"harvest bubble3d: {counts['bubble3d_384x128x128']} pairs ({time.time() - t:.1f} s)")
+    t = time.time()
+    x = configs.make_nodes(8, 1e-4)
+    ctx = Context(x, x, x, quirks=0)
+    for sd in batch_seeds:
+        (r, tt, e), _ = chemistry.states(ctx, n_batch, seed=sd, burnt="equilibrium")
+        r0, t0 = r.clone(), tt.clone()
+        ctx.chem_batch(n_batch, r.data_ptr(), tt.data_ptr(), e.data_ptr(), 1e-8, 10000)
+        H.add(r0, r, t0, 1e-8)
+    ctx.close()
+    counts["batch_equilibrium"] = H.n - sum(counts.values())
+    log(f"harvest batch: {counts['batch_equilibrium']} pairs ({time.time() - t:.1f} s)")
+    t_harvest = time.time() - t_all
+    r0, r1, t0, dt = H.cat()
+    del H
+    m = MLP(seed=11)
+    p = surrogate.fit_params(r0, r1, t0, dt)
+    m.set_surrogate(p)
+    X, Y = surrogate.make_dataset(m, r0, r1, t0, dt)
+    del r0, r1, t0, dt
+    n = X.shape[0]
+    g = torch.Generator(device="cuda")
+    g.manual_seed(5)
+    perm = torch.randperm(n, device="cuda", generator=g)
+    hv, tr_idx = perm[:holdout], perm[holdout:]
+    Xh, Yh = X[hv].contiguous(), Y[hv].contiguous()
+    Xt, Yt = X[tr_idx].contiguous(), Y[tr_idx].contiguous()
+    del X, Y, perm
+
+    def loss_of(Xs, Ys):
+        """Inference MSE over the 9 outputs, and the share of it from the worst 0.1 % of pairs."""
+        out = torch.empty((Xs.shape[0], MLP.NOUT), dtype=torch.float32, device="cuda")
+        m.forward(Xs, out)
+        se = ((out[:, :surrogate.NOUT_REAL] - Ys[:, :surrogate.NOUT_REAL]) ** 2).mean(1)
+        top = se.topk(max(1, se.shape[0] // 1000)).values
+        return {"mse": float(se.mean()), "share_of_worst_0.1pct": float(top.sum() / se.sum())}
+
+    def held_out_loss():
+        return loss_of(Xh, Yh)
+
+    h0 = held_out_loss()
+    torch.cuda.synchronize()
+    t = time.time()
+    hist = surrogate.train(m, Xt, Yt, epochs=epochs, batch=batch, lr=2e-3, log=log)
+    torch.cuda.synchronize()
+    t_train = time.time() - t
+    h1 = held_out_loss()
+    sub = torch.randperm(Xt.shape[0], device="cuda", generator=g)[:holdout]
+    t1 = loss_of(Xt[sub].contiguous(), Yt[sub].contiguous())
+    full = {"mse": 0.0}
+    for a in range(0, Xt.shape[0], 1 << 20):   # the whole training set in inference, 2^20 rows at a time
+        part = loss_of(Xt[a:a + (1 << 20)].contiguous(), Yt[a:a + (1 << 20)].contiguous())
+        full["mse"] += part["mse"] * min(1 << 20, Xt.shape[0] - a) / Xt.shape[0]
+    steps = epochs * (Xt.shape[0] // batch)
+    return {"pairs": {"total": int(n), "training": int(Xt.shape[0]), "held_out": int(holdout), **counts},
+            "harvest_s": t_harvest, "epochs": epochs, "minibatch": batch, "optimizer_steps": steps,
+            "train_s": t_train, "pairs_per_s": steps * batch / t_train,
+            "loss_per_epoch": hist, "held_out_loss": {"before": h0, "after": h1},
+            "training_loss_after": {"random_subset_of_held_out_size": t1, "whole_set": full},
+            "outputs": "normalised dY_k of the 9 species (MSE over the 9 outputs)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("workload", choices=("batch", "detonation", "plain-detonation", "stiff-gain"))
+    ap.add_argument("workload", choices=("batch", "detonation", "plain-detonation", "stiff-gain", "train-at-scale"))
     ap.add_argument("--threshold", type=int, default=None)
     ap.add_argument("--steps", type=int, default=150)
     ap.add_argument("--grid", type=str, default="512x256x128")
@@ -366,6 +454,8 @@ def main():
         res = eval_batch(log=log, stiff_only=a.stiff_only, **kw)
     elif a.workload == "stiff-gain":
         res = stiff_gain(log=log)
+    elif a.workload == "train-at-scale":
+        res = train_at_scale(log=log) if a.epochs is None else train_at_scale(epochs=a.epochs, log=log)
     elif a.workload == "plain-detonation":
         res = time_plain_detonation(grid=tuple(int(v) for v in a.grid.split("x")), nsteps=a.steps, log=log)
     else:
