@@ -46,15 +46,17 @@ def _oracle(xnode, ynode, znode):
     return oracle.OracleFlow(xnode, ynode, znode, quirks=0)
 
 
-@pytest.mark.parametrize("quirks", [0, 0x7FF])
-def test_det3d_full_size_step_vs_oracle(quirks):
+@pytest.mark.parametrize("quirks,rk", [(0, 1), (0x7FF, 1), (0, 2)])
+def test_det3d_full_size_step_vs_oracle(quirks, rk):
     """quirks 0x7FF: the handed-over state stepped once with every REF quirk (the prediction pass's
-    C3 instance and the old-E Newton of F3 among them); non-finite cells must coincide."""
+    C3 instance and the old-E Newton of F3 among them); non-finite cells must coincide. rk 2: the
+    step as SSP-RK2 (two advection stages, then Update_after_CS and the prediction pass without
+    the fused update)."""
     import torch
     from eu3d_amd import configs
     from eu3d_amd.api import Context
     assert torch.cuda.is_available()
-    say(f"case det3d 512x256x128, quirks {quirks:#x}: 110 FIXED steps on the GPU, then one step on both")
+    say(f"case det3d 512x256x128, quirks {quirks:#x}, rk {rk}: 110 FIXED steps on the GPU, then one step on both")
     wl = configs.det3d(512, 256, 128)
     g = Context(wl.xnode, wl.ynode, wl.znode, quirks=0)
     init_state(g, wl)
@@ -67,6 +69,7 @@ def test_det3d_full_size_step_vs_oracle(quirks):
     g.set_quirks(quirks)
     o.set_quirks(quirks)
     say("  state handed over; stepping")
+    kw["rk"] = rk
     g.step(1, cfl=0.5, **kw)
     o.step(1, cfl=0.5, **kw)
     say("  comparing")
