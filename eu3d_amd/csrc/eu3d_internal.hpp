@@ -88,7 +88,7 @@ struct eu3d_ctx {
     unsigned int* coop_list = nullptr;                             // cells with the longest sub-step chains
     long long coop_cap = 0;
     int flux_variant = 0;                                          // tile shape / order of the x/y advection pass
-    int zchunk = 32;                                               // z planes per column of the z pass
+    int zchunk = 0;                                                // z planes per column chunk of the z pass (0: by grid size)
     struct eu3d_mlp* mlp = nullptr;                                // surrogate used by EU3D_CHEM_SURROGATE
     unsigned* sur_list = nullptr;                                  // fallback list, work list, counters, keep
     long long sur_cap = 0;

@@ -2139,8 +2139,16 @@ hipError_t launch_flux_z(eu3d_ctx* c, int diff, int limiter) {
     // of re-reading 3 planes of stencil per chunk. With the 128-byte-aligned rows (MUSCL_2 +
     // minmod, one wave per SIMD): detonation 512x256x128 8 planes 0.873 ms, 32 planes 0.806 ms,
     // 64 planes 0.782 ms; bubble3d 384x128x128 0.327, 0.303, 0.326 ms. (Unaligned rows had
-    // favoured 8 planes: bubble3d 0.391 against 0.403 ms for 32.)
-    switch (c->zchunk) {
+    // favoured 8 planes: bubble3d 0.391 against 0.403 ms for 32.) With zchunk 0 the chunk is 64
+    // planes when that still gives every CU four blocks (one runs at a time, at one wave per
+    // SIMD), else 32: the detonation's 1,024 blocks of 64 planes fill four rounds exactly, the
+    // bubble's 384 would leave half the GPU idle in their second round.
+    int zc = c->zchunk;
+    if (zc == 0) {
+        const long long cols = (long long)((c->g.ni + ZX - 1) / ZX) * ((c->g.nj + ZY - 1) / ZY);
+        zc = cols * ((c->g.nk + 63) / 64) >= 4LL * c->num_cu ? 64 : 32;
+    }
+    switch (zc) {
     case 8: launch_z<8>(c, diff, limiter); break;
     case 16: launch_z<16>(c, diff, limiter); break;
     case 64: launch_z<64>(c, diff, limiter); break;
